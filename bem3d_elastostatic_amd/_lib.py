@@ -1,0 +1,95 @@
+"""ctypes binding of libquadesth.so (include/quadesth.h).
+
+The library is the only compute path: when it is missing, or when no GPU is
+visible, every entry point raises instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libquadesth.so")
+
+QD_OK = 0
+QD_FP64, QD_FP32_IR, QD_FP16_IR = 0, 1, 2
+PRECISIONS = {"fp64": QD_FP64, "fp32": QD_FP32_IR, "fp16": QD_FP16_IR}
+SOMIGLIANA_REFERENCE, SOMIGLIANA_CORRECTED = 0, 1
+
+
+class SolveReport(C.Structure):
+    _fields_ = [("precision", C.c_int), ("info", C.c_int), ("iterations", C.c_int),
+                ("converged", C.c_int), ("gmres", C.c_int), ("pad_", C.c_int),
+                ("residual", C.c_double), ("t_factor_ms", C.c_double), ("t_solve_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for (k, _) in self._fields_ if k != "pad_"}
+
+
+class QuadesthError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"libquadesth error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+_vp = C.c_void_p
+_i = C.c_int
+_ll = C.c_longlong
+_d = C.c_double
+_pd = C.POINTER(C.c_double)
+_pi = C.POINTER(C.c_int)
+_pll = C.POINTER(C.c_longlong)
+
+# name -> (restype, argtypes): the declarations of include/quadesth.h
+_SIGNATURES = {
+    "qd_create": (_i, [C.POINTER(_vp), _i]),
+    "qd_destroy": (None, [_vp]),
+    "qd_last_error": (C.c_char_p, [_vp]),
+    "qd_set_stream": (_i, [_vp, _vp]),
+    "qd_synchronize": (_i, [_vp]),
+    "qd_set_mesh": (_i, [_vp, _i, _i, _pd, _pd, _pd, _pi, _pi]),
+    "qd_set_material": (_i, [_vp, _d, _d]),
+    "qd_set_bc": (_i, [_vp, _pi, _pd]),
+    "qd_precompute_element_geometry": (_i, [_vp] + [_pd] * 8),
+    "qd_ghmat8": (_i, [_vp]),
+    "qd_ghmat8_rhs": (_i, [_vp, _pd, _pd]),
+    "qd_get_system": (_i, [_vp, _pd, _i, _pd]),
+    "qd_system_device": (_i, [_vp, C.POINTER(_pd), _pll, C.POINTER(_pd)]),
+    "qd_solve": (_i, [_vp, _i, _i, _pd, _i, _d, _i, C.POINTER(SolveReport)]),
+    "qd_solve_device": (_i, [_vp, _i, _i, _vp, _ll, _d, _i, C.POINTER(SolveReport)]),
+    "qd_solve_transpose_device": (_i, [_vp, _i, _i, _vp, _ll, _d, _i, C.POINTER(SolveReport)]),
+    "qd_reorganize_solution": (_i, [_vp, _pd, _pd, _pd]),
+    "qd_internal_points": (_i, [_vp, _i, _pd, _pd, _pd, _pd, _pd, _i, _pd, _pd, _pd, _pd, _pll]),
+    "qd_internal_points_device": (_i, [_vp, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _pll]),
+    "qd_boundary_stresses": (_i, [_vp, _pd, _pd, _pd, _pd, _pd, _pd]),
+    "dgesv_": (None, [_pi, _pi, _pd, _pi, _pi, _pd, _pi, _pi]),
+    "qd_dgesv_device": (_i, [_vp, _i, _i, _vp, _ll, _pi, _vp, _ll]),
+}
+
+
+def exported_symbols():
+    return list(_SIGNATURES)
+
+
+def load(path: str = LIB_PATH):
+    """Loads (once) and declares every entry point of the C ABI."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise QuadesthError(-100, f"{path} is not built (python -m bem3d_elastostatic_amd.build)")
+    lib = C.CDLL(path)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(h, rc, what=""):
+    if rc < 0:
+        msg = load().qd_last_error(h)
+        raise QuadesthError(rc, (msg.decode() if msg else "") + (f" [{what}]" if what else ""))
+    return rc

@@ -1,0 +1,166 @@
+// qd_internal.h -- shared state of libquadesth (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "quadesth.h"
+
+namespace qd {
+
+// Device buffer owned by the handle.
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t count = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    count = 0;
+  }
+  // (Re)allocates when the size grows; contents are not preserved.
+  hipError_t ensure(size_t n) {
+    if (n <= count && p) return hipSuccess;
+    release();
+    if (n == 0) return hipSuccess;
+    hipError_t e = hipMalloc((void**)&p, n * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      return e;
+    }
+    count = n;
+    return hipSuccess;
+  }
+};
+
+// Per-element data the assembly kernels read with scalar (wave-uniform) loads.
+struct ElemInfo {
+  int node[9];   // 0-based node numbers
+  int nconec;    // 9 or 6
+  int kode;      // bit ld set when kode(e, ld) == 1 (traction prescribed)
+  int k0mask;    // bit 3k+ld set when kode0(node k, ld) != 0 (assembly_mod.f90:149-161)
+  int needG;     // G integrals contribute somewhere (a kode==0 dof or a non-zero bc)
+  int pad[3];
+};
+static_assert(sizeof(ElemInfo) == 64, "ElemInfo layout");
+
+// Singular (collocation node inside element) pair.
+struct SingPair {
+  int node;  // collocation node, 0-based
+  int elem;  // element, 0-based
+  int nodo;  // local node number of `node` in `elem`, 1-based
+  int nsub;  // sub-triangles of the Duffy split (assembly_mod.f90:386-490)
+};
+
+// Sub-triangle table: per (element kind, nodo): up to 4 triangles of
+// (v1[3], v2[3], area), assembly_mod.f90:399-489.
+struct SubTri {
+  double v1[3], v2[3], area, pad;
+};
+
+constexpr int kGP = 36;         // Gauss points per element slot (6x6; tri6 uses 16)
+constexpr int kAsmStride = 16;  // doubles per Gauss point in the assembly table
+constexpr int kNearGL = 40;     // near-field rule of the internal points (stress_mod.f90:513)
+
+struct Material {
+  double mu, nu, young, c3, c4, c5, c6;
+};
+
+}  // namespace qd
+
+struct qd_lu_state;
+
+struct qd_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // problem data (host copies)
+  int n = 0, ne = 0;
+  std::vector<double> hx, hy, hz;
+  std::vector<int> hnconec, hkconec, hkode;
+  std::vector<double> hbc;
+  std::vector<int> nesim, ke;  // ke(8, n) 1-based element numbers
+  qd::Material mat{};
+  bool have_mesh = false, have_mat = false, have_bc = false, have_geom = false, have_sys = false;
+
+  // device mesh
+  qd::DevBuf<double> x, y, z;
+  qd::DevBuf<int> nconec, kconec0;  // kconec0: 0-based, (ne, 9)
+  qd::DevBuf<double> geo;           // reference tables: 8 blocks of (36, ne) + ff (9, 36, ne)
+  qd::DevBuf<double> asmtab;        // (ne*36, 16): xg(3) eta(3) p12(9) pad
+  qd::DevBuf<qd::ElemInfo> einfo;
+  qd::DevBuf<double> ebc, evp;      // (ne, 27): bc and vp per element dof
+  qd::DevBuf<int> node_k0;          // (n, 3) kode0 flag per node/direction
+  qd::DevBuf<double> node_vp;       // (n, 3)
+  qd::DevBuf<qd::SubTri> subtri;    // [2 kinds][10 nodo][4]
+
+  // assembly schedule
+  int chunk_elems = 32;
+  std::vector<int> h_chunk_start;                 // nchunk + 1
+  std::vector<int> h_colour_chunks;               // chunk ids sorted by colour
+  std::vector<int> h_colour_start;                // ncolour + 1 offsets into colour_chunks
+  std::vector<int> h_gchunks;                     // chunks holding needG elements, colour order
+  std::vector<int> h_gcolour_start;
+  qd::DevBuf<int> chunk_start, colour_chunks, gchunks;
+  std::vector<qd::SingPair> h_pairs;              // sorted by node, then element
+  std::vector<int> h_pair_ptr;                    // n + 1
+  qd::DevBuf<qd::SingPair> pairs;
+  qd::DevBuf<int> pair_ptr;
+  qd::DevBuf<double> sing_hg;                     // (npairs, 162)
+  qd::DevBuf<double> dpart, fpart_h, fpart_g;     // assembly partial sums
+
+  // assembled system (column-major, leading dimension lda)
+  qd::DevBuf<double> A, FI;
+  long long lda = 0;
+
+  // solver state (qd_lu.hip)
+  qd_lu_state* lu = nullptr;
+
+  // Somigliana tables
+  qd::DevBuf<double> far_tab, near_tab, el_centre, somi_part;
+  qd::DevBuf<int> far_start;
+  qd::DevBuf<double> tmp_fi, tmp_dfi, tmp_pts, tmp_out;
+  qd::DevBuf<long long> counters;
+};
+
+namespace qd {
+int set_error(qd_context* h, int code, const char* fmt, ...);
+#define QD_HIP(h, call)                                                                 \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return qd::set_error((h), QD_ERR_HIP, "%s:%d %s: %s", __FILE__, __LINE__, #call, \
+                           hipGetErrorString(e_));                                      \
+  } while (0)
+#define QD_ALLOC(h, buf, n)                                                                    \
+  do {                                                                                         \
+    hipError_t e_ = (buf).ensure(n);                                                           \
+    if (e_ != hipSuccess)                                                                      \
+      return qd::set_error((h), QD_ERR_NOMEM, "%s:%d allocating %zu elements: %s", __FILE__, \
+                           __LINE__, (size_t)(n), hipGetErrorString(e_));                     \
+  } while (0)
+
+// phases implemented in the .hip files
+int geometry_run(qd_context* h);
+int assembly_run(qd_context* h, const double* bc_dev_override, double* fi_out_dev, bool write_A);
+int somigliana_run(qd_context* h, int l, const double* pts_dev, const double* fi_dev,
+                   const double* dfi_dev, int variant, double* u_dev, double* sig_dev,
+                   double* prin_dev, double* vm_dev, long long* counts);
+int boundary_stress_run(qd_context* h, const double* fi_dev, const double* dfi_dev,
+                        double* se_dev, double* sn_dev, double* prin_dev, double* vm_dev);
+int lu_solve(qd_context* h, int precision, int nrhs, double* b_dev, long long ldb, double tol,
+             int max_iter, bool transpose, qd_solve_report* rep);
+void lu_free(qd_context* h);
+int lu_invalidate(qd_context* h);
+int dgesv_device(qd_context* h, int n, int nrhs, double* a_dev, long long lda, int* ipiv_host,
+                 double* b_dev, long long ldb);
+}  // namespace qd

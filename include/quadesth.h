@@ -1,0 +1,160 @@
+/*
+ * quadesth.h -- C ABI of libquadesth, the MI355X (gfx950) implementation of
+ * QUADESTH's hot path: the ghmat8 collocation assembly, the dense DGESV
+ * solve and the Somigliana internal-point integrals.
+ *
+ * Every entry point replaces one routine of the reference (cited file:line
+ * into the QUADESTH sources).  Arrays use the reference's own layouts (the
+ * data_mod.f90:38-52 module globals, Fortran column-major, element-node
+ * arrays (27, ne) with dof 3*(k-1)+ld, kconec(9, ne) with 1-based node
+ * numbers), so a Fortran caller passes its module arrays unchanged through
+ * ISO_C_BINDING and a Python caller passes numpy/torch buffers.
+ *
+ * Pointers named *_dev are device (HBM) pointers; all others are host
+ * pointers.  Sizes are int (the reference's default integer); matrix
+ * leading dimensions and offsets are 64-bit internally (76,806^2 > 2^31).
+ *
+ * Return value: 0 = success; > 0 = LAPACK-style singular pivot (1-based);
+ * < 0 = error (see QD_ERR_*; qd_last_error gives the message).  The
+ * reference has no error returns: its DGESV failure prints and stops
+ * (solver_mod.f90:30-33); a Fortran shim keeps that behaviour on non-zero.
+ */
+#ifndef QUADESTH_H
+#define QUADESTH_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qd_context* qd_handle;
+
+enum {
+  QD_OK = 0,
+  QD_ERR_ARG = -1,     /* bad argument */
+  QD_ERR_HIP = -2,     /* HIP runtime error */
+  QD_ERR_STATE = -3,   /* called before the phase it depends on */
+  QD_ERR_MESH = -4,    /* invalid mesh, e.g. a node shared by more than 8
+                          elements (overflows ke(8, n), io_mod.f90:131-133) */
+  QD_ERR_NOMEM = -5,   /* device allocation failed */
+  QD_ERR_NOCONV = -6   /* iterative refinement missed its tolerance */
+};
+
+/* Precision of the factorisation used by qd_solve.  Residuals of the
+ * iterative refinement are always formed with the FP64 matrix. */
+enum {
+  QD_FP64 = 0,      /* FP64 blocked LU (MFMA f64 trailing updates) */
+  QD_FP32_IR = 1,   /* FP32 LU (MFMA f32), FP64 iterative refinement */
+  QD_FP16_IR = 2    /* FP32 panel + FP16 MFMA trailing updates with
+                       per-block scaling, FP64 refinement (GMRES fallback) */
+};
+
+/* Somigliana displacement identity variant (stress_mod.f90:838-843). */
+enum {
+  QD_SOMIGLIANA_REFERENCE = 0,  /* the reference's sign of the T* term */
+  QD_SOMIGLIANA_CORRECTED = 1   /* the sign the assembly kernel uses
+                                   (assembly_mod.f90:646-647) */
+};
+
+typedef struct {
+  int precision;        /* QD_FP64 / QD_FP32_IR / QD_FP16_IR */
+  int info;             /* LAPACK-style info of the factorisation */
+  int iterations;       /* refinement sweeps taken (0 for QD_FP64) */
+  int converged;        /* 1 when the residual met tol (or QD_FP64) */
+  int gmres;            /* 1 when the GMRES fallback was used */
+  int pad_;
+  double residual;      /* max over rhs of ||b-Ax||_inf/(||A||_inf||x||_inf+||b||_inf) */
+  double t_factor_ms;   /* device time of the factorisation */
+  double t_solve_ms;    /* device time of substitutions + refinement */
+} qd_solve_report;
+
+/* ---- lifecycle ----------------------------------------------------------- */
+int qd_create(qd_handle* out, int device);
+void qd_destroy(qd_handle h);
+const char* qd_last_error(qd_handle h);
+/* Work is queued on this HIP stream (default: the null stream). */
+int qd_set_stream(qd_handle h, void* hip_stream);
+int qd_synchronize(qd_handle h);
+
+/* ---- problem data ---------------------------------------------------------
+ * Replaces input_data's fill of the data_mod globals (io_mod.f90:36-146) and
+ * the nesim/ke adjacency (io_mod.f90:368-378).  Copies to the device. */
+int qd_set_mesh(qd_handle h, int n, int ne, const double* x, const double* y, const double* z,
+                const int* nconec, const int* kconec);
+/* update_material_constants (data_mod.f90:97-108): mu is the shear modulus
+ * the deck carries (rge), nu Poisson's ratio. */
+int qd_set_material(qd_handle h, double mu, double nu);
+/* kode(3*ne): 0 = displacement prescribed, 1 = traction prescribed;
+ * bc(27, ne).  The system matrix depends on kode only; bc enters FI. */
+int qd_set_bc(qd_handle h, const int* kode, const double* bc);
+
+/* ---- hot path ------------------------------------------------------------ */
+/* precompute_element_geometry (assembly_mod.f90:28-51).  Optionally returns
+ * the all_* tables (each (36, ne), ff (9, 36, ne)); pass NULL to skip. */
+int qd_precompute_element_geometry(qd_handle h, double* eta1, double* eta2, double* eta3,
+                                   double* xja, double* xg1, double* xg2, double* xg3, double* ff);
+
+/* ghmat8 (assembly_mod.f90:53-223): assembles A (3n x 3n) and FI (3n) into
+ * device memory owned by the handle. */
+int qd_ghmat8(qd_handle h);
+/* FI only, for another set of prescribed values with the same kode
+ * (assembly_mod.f90:126-218 restricted to FI); A is untouched. */
+int qd_ghmat8_rhs(qd_handle h, const double* bc, double* fi /* host, 3n */);
+/* Copies of the assembled system (host, column-major with leading dim lda). */
+int qd_get_system(qd_handle h, double* A /* nullable */, int lda, double* FI /* nullable */);
+/* Device views of the handle's A (column-major, leading dimension *lda) and FI. */
+int qd_system_device(qd_handle h, double** A_dev, long long* lda, double** FI_dev);
+
+/* solve_linear_system (solver_mod.f90:17-36) on the device-resident A.
+ * b: host (N x nrhs, column-major, leading dim ldb) right-hand sides on
+ * input (NULL: solve for the assembled FI and return nothing), solutions on
+ * output.  The factorisation is kept for later qd_solve calls with the same
+ * precision until qd_ghmat8 runs again. */
+int qd_solve(qd_handle h, int precision, int nrhs, double* b, int ldb, double tol, int max_iter,
+             qd_solve_report* rep);
+/* Same with device right-hand sides (overwritten by the solutions). */
+int qd_solve_device(qd_handle h, int precision, int nrhs, double* b_dev, long long ldb,
+                    double tol, int max_iter, qd_solve_report* rep);
+/* Adjoint solve A^T y = c with the factors of the last qd_solve of this
+ * precision (no reference counterpart: the gradient of the solve). */
+int qd_solve_transpose_device(qd_handle h, int precision, int nrhs, double* c_dev,
+                              long long ldc, double tol, int max_iter, qd_solve_report* rep);
+
+/* reorganize_solution (io_mod.f90:148-183): X (3n) -> FI (3n) displacements,
+ * DFI (27, ne) tractions. */
+int qd_reorganize_solution(qd_handle h, const double* x, double* fi, double* dfi);
+
+/* compute_internal_points (stress_mod.f90:511-781) for l points.
+ * Outputs u_int (3, l), sigma_int (6, l) in xx,yy,zz,xy,yz,zx order,
+ * prin (3, l) descending, vm (l); any output may be NULL.
+ * counts (nullable, 2 entries): far and near Gauss-point evaluations. */
+int qd_internal_points(qd_handle h, int l, const double* cx, const double* cy, const double* cz,
+                       const double* fi, const double* dfi, int variant, double* u_int,
+                       double* sigma_int, double* prin, double* vm, long long* counts);
+/* Same, every array a device pointer (points as (l, 3) row-major xyz). */
+int qd_internal_points_device(qd_handle h, int l, const double* pts_dev, const double* fi_dev,
+                              const double* dfi_dev, int variant, double* u_dev,
+                              double* sigma_dev, double* prin_dev, double* vm_dev,
+                              long long* counts);
+
+/* compute_boundary_stresses (stress_mod.f90:366-406): nodal averaged stress
+ * (6, n), principal (3, n), von Mises (n); sigma_elem (6, 9, ne) nullable. */
+int qd_boundary_stresses(qd_handle h, const double* fi, const double* dfi, double* sigma_elem,
+                         double* sigma_node, double* prin, double* vm);
+
+/* ---- LAPACK drop-in ------------------------------------------------------
+ * DGESV with the LAPACK Fortran ABI (the call at solver_mod.f90:28): a gets
+ * the LU factors (column-major), ipiv the 1-based row interchanges, b the
+ * solutions, info the LAPACK status.  Runs on device 0 (QUADESTH_DEVICE
+ * overrides) with FP64 factorisation. */
+void dgesv_(const int* n, const int* nrhs, double* a, const int* lda, int* ipiv, double* b,
+            const int* ldb, int* info);
+
+/* Device-side DGETRF/DGETRS pair used by dgesv_: a_dev column-major
+ * (n x n, leading dim lda), ipiv host (1-based), b_dev column-major. */
+int qd_dgesv_device(qd_handle h, int n, int nrhs, double* a_dev, long long lda, int* ipiv,
+                    double* b_dev, long long ldb);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
