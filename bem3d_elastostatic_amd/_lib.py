@@ -15,6 +15,8 @@ QD_OK = 0
 QD_FP64, QD_FP32_IR, QD_FP16_IR = 0, 1, 2
 PRECISIONS = {"fp64": QD_FP64, "fp32": QD_FP32_IR, "fp16": QD_FP16_IR}
 SOMIGLIANA_REFERENCE, SOMIGLIANA_CORRECTED = 0, 1
+PROF_KINDS = {"lu_trailing_gemm": 0, "lu_panel": 1, "assembly_h": 2, "assembly_g": 3,
+              "somigliana_far": 4, "somigliana_near": 5}
 
 
 class SolveReport(C.Structure):
@@ -63,6 +65,8 @@ _SIGNATURES = {
     "qd_internal_points": (_i, [_vp, _i, _pd, _pd, _pd, _pd, _pd, _i, _pd, _pd, _pd, _pd, _pll]),
     "qd_internal_points_device": (_i, [_vp, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _pll]),
     "qd_boundary_stresses": (_i, [_vp, _pd, _pd, _pd, _pd, _pd, _pd]),
+    "qd_profile_enable": (_i, [_vp, C.c_uint]),
+    "qd_profile_read": (_i, [_vp, _i, _pll, _pd, _pd]),
     "dgesv_": (None, [_pi, _pi, _pd, _pi, _pi, _pd, _pi, _pi]),
     "qd_dgesv_device": (_i, [_vp, _i, _i, _vp, _ll, _pi, _vp, _ll]),
 }

@@ -656,6 +656,37 @@ int qd_internal_points(qd_handle h, int l, const double* cx, const double* cy, c
   return QD_OK;
 }
 
+int qd_profile_enable(qd_handle h, unsigned mask) {
+  if (!h) return QD_ERR_ARG;
+  h->prof_mask = mask;
+  return QD_OK;
+}
+
+int qd_profile_read(qd_handle h, int kind, long long* launches, double* ms, double* work) {
+  if (!h || kind < 0 || kind >= QD_PROF_KINDS) return QD_ERR_ARG;
+  QD_HIP(h, hipSetDevice(h->device));
+  QD_HIP(h, hipStreamSynchronize(h->stream));
+  for (auto& r : h->prof_pending) {
+    float t = 0.0f;
+    if (hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) {
+      h->prof_launches[r.kind] += 1;
+      h->prof_ms[r.kind] += t;
+      h->prof_work[r.kind] += r.work;
+    }
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  h->prof_pending.clear();
+  if (launches) *launches = h->prof_launches[kind];
+  if (ms) *ms = h->prof_ms[kind];
+  if (work) *work = h->prof_work[kind];
+  // reading resets the class
+  h->prof_launches[kind] = 0;
+  h->prof_ms[kind] = 0.0;
+  h->prof_work[kind] = 0.0;
+  return QD_OK;
+}
+
 int qd_boundary_stresses(qd_handle h, const double* fi, const double* dfi, double* sigma_elem,
                          double* sigma_node, double* prin, double* vm) {
   if (!h || !fi || !dfi) return QD_ERR_ARG;

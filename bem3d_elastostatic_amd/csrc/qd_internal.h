@@ -130,7 +130,42 @@ struct qd_context {
   qd::DevBuf<int> far_start;
   qd::DevBuf<double> tmp_fi, tmp_dfi, tmp_pts, tmp_out;
   qd::DevBuf<long long> counters;
+
+  // instrumentation (qd_profile_enable / qd_profile_read)
+  struct ProfRec {
+    int kind;
+    hipEvent_t a, b;
+    double work;
+  };
+  unsigned prof_mask = 0;
+  std::vector<ProfRec> prof_pending;
+  long long prof_launches[QD_PROF_KINDS] = {};
+  double prof_ms[QD_PROF_KINDS] = {};
+  double prof_work[QD_PROF_KINDS] = {};
 };
+
+namespace qd {
+// Brackets a launch of class `kind` with events when profiling is on.
+struct ProfScope {
+  qd_context* h;
+  int kind;
+  double work;
+  hipEvent_t a = nullptr;
+  ProfScope(qd_context* h_, int kind_, double work_) : h(h_), kind(kind_), work(work_) {
+    if ((h->prof_mask >> kind) & 1) {
+      if (hipEventCreate(&a) == hipSuccess) (void)hipEventRecord(a, h->stream);
+      else a = nullptr;
+    }
+  }
+  ~ProfScope() {
+    if (!a) return;
+    hipEvent_t b;
+    if (hipEventCreate(&b) != hipSuccess) return;
+    (void)hipEventRecord(b, h->stream);
+    h->prof_pending.push_back(qd_context::ProfRec{kind, a, b, work});
+  }
+};
+}  // namespace qd
 
 namespace qd {
 int set_error(qd_context* h, int code, const char* fmt, ...);

@@ -285,6 +285,8 @@ struct MfmaTraits<double> {
   static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  // output row of register r in lane group lk (rows interleave by 4)
+  static __device__ __forceinline__ int out_row(int lk, int r) { return lk + 4 * r; }
 };
 template <>
 struct MfmaTraits<float> {
@@ -293,6 +295,7 @@ struct MfmaTraits<float> {
   static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int out_row(int lk, int r) { return 4 * lk + r; }
 };
 
 template <typename T>
@@ -353,7 +356,7 @@ __global__ __launch_bounds__(256) void k_gemm_sub(long long M, long long N, int 
     for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const long long gr = m0 + wm + 16 * i + 4 * lk + r;
+        const long long gr = m0 + wm + 16 * i + MT::out_row(lk, r);
         const long long gc = n0 + wn + 16 * j + lr;
         if (gr < M && gc < N) C[gr * ldc + gc] -= acc[i][j][r];
       }
@@ -549,7 +552,11 @@ static int factor(LuCtx<T>& c, int NB) {
   int rc;
   for (long long j = 0; j < N; j += NB) {
     const int jb = (int)std::min<long long>(NB, N - j);
-    if ((rc = panel(c, j, jb, j, j + jb))) return rc;
+    {
+      const double m = (double)(N - j);
+      ProfScope ps(c.h, QD_PROF_LU_PANEL, m * jb * (double)jb - (double)jb * jb * jb / 3.0);
+      if ((rc = panel(c, j, jb, j, j + jb))) return rc;
+    }
     // interchanges on the columns left and right of the panel
     if (j > 0)
       hipLaunchKernelGGL(k_laswp<T>, dim3((unsigned)((j + 255) / 256)), dim3(256), 0, c.h->stream, c.W, c.ld, 0LL,
@@ -559,6 +566,8 @@ static int factor(LuCtx<T>& c, int NB) {
                          c.ld, j + jb, N, (int)j, (int)(j + jb), s->ipiv_dev.p);
       QD_HIP(c.h, hipGetLastError());
       if ((rc = trsm_lower(c, j, jb, j + jb, N))) return rc;
+      const double mm = (double)(N - j - jb);
+      ProfScope ps(c.h, QD_PROF_LU_TRAILING_GEMM, 2.0 * mm * mm * jb);
       if ((rc = gemm_sub(c, N - j - jb, N - j - jb, jb, c.W + (j + jb) * c.ld + j, c.W + j * c.ld + j + jb,
                          c.W + (j + jb) * c.ld + j + jb, true)))
         return rc;

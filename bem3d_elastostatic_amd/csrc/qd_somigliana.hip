@@ -406,10 +406,17 @@ int somigliana_run(qd_context* h, int l, const double* pts, const double* fi, co
   k.sg = variant ? -1.0 : 1.0;
   double* part = h->somi_part.p;
   double* nearp = part + (size_t)nslice * l * 9;
-  hipLaunchKernelGGL(k_somi_far, dim3((l + 63) / 64, nslice), dim3(64), 0, h->stream, l, ne, slice, pts,
-                     h->el_centre.p, h->far_start.p, h->far_tab.p, k, part);
-  hipLaunchKernelGGL(k_somi_near, dim3(l), dim3(64), 0, h->stream, l, ne, pts, h->el_centre.p,
-                     h->nconec.p, h->near_tab.p, k, nearp, (unsigned long long*)h->counters.p);
+  // ~250 flops per Gauss-point evaluation of accumulate_somigliana_range
+  {
+    ProfScope ps(h, QD_PROF_SOMIGLIANA_FAR, 250.0 * (double)tot * l);
+    hipLaunchKernelGGL(k_somi_far, dim3((l + 63) / 64, nslice), dim3(64), 0, h->stream, l, ne, slice, pts,
+                       h->el_centre.p, h->far_start.p, h->far_tab.p, k, part);
+  }
+  {
+    ProfScope ps(h, QD_PROF_SOMIGLIANA_NEAR, 0.0);
+    hipLaunchKernelGGL(k_somi_near, dim3(l), dim3(64), 0, h->stream, l, ne, pts, h->el_centre.p,
+                       h->nconec.p, h->near_tab.p, k, nearp, (unsigned long long*)h->counters.p);
+  }
   hipLaunchKernelGGL(k_somi_final, dim3((l + 127) / 128), dim3(128), 0, h->stream, l, nslice, part, nearp,
                      u, sig, prin, vm);
   QD_HIP(h, hipGetLastError());

@@ -73,6 +73,21 @@ class Quadesth:
     def synchronize(self):
         check(self.h, self.lib.qd_synchronize(self.h))
 
+    def profile_enable(self, kinds=()):
+        mask = 0
+        for k in kinds:
+            mask |= 1 << _lib.PROF_KINDS[k]
+        check(self.h, self.lib.qd_profile_enable(self.h, mask))
+
+    def profile_read(self, kind):
+        """(launches, device ms, algorithmic work) accumulated for a kernel
+        class since the last read."""
+        n = C.c_longlong()
+        ms, work = C.c_double(), C.c_double()
+        check(self.h, self.lib.qd_profile_read(self.h, _lib.PROF_KINDS[kind], C.byref(n), C.byref(ms),
+                                               C.byref(work)))
+        return n.value, ms.value, work.value
+
     def set_bc(self, kode, bc):
         self._kode = _i32(kode).reshape(-1)
         self._bc = _f64(bc).reshape(-1)

@@ -1,0 +1,283 @@
+"""Benchmark of the QUADESTH hot path on one MI355X.
+
+One step = the whole hot path over the synthetic thick-cantilever box
+(10 x 2 x 1, h = 0.1: 25,602 nodes, 2,400 quad9 + 8,000 tri6, 76,806 DOF,
+65,536 internal points): ghmat8 assembly, the mixed-precision solve (FP16
+MFMA trailing updates, FP64 refinement to 1e-14), reorganisation, and the
+Somigliana internal-point integrals.  Mesh, codes and points are resident in
+HBM before the timed region.  After the timed steps the same GPU's FP64 LU
+path is timed once on the same system, for the speed-up and for the error of
+the mixed path against it.
+
+Prints ONE JSON line (rank 0).  Usage:
+    python bench.py [--steps K] [--warmup W] [--precision fp16|fp32|fp64]
+                    [--config c3|c1] [--no-fp64] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "s/solve @76.8k DOF: mixed-IR vs FP64 LU @1e-10 rel; H,G entries/s; LU MFMA %"
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "fp16": 2500.0, "fp64_vector": 78.6}
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def hg_entries(p):
+    """H,G entries of the reference's assembly: every (collocation node,
+    element) pair yields a 3 x 3*nconec block of H and of G."""
+    return float(p.n) * float(np.sum(18 * p.nconec.astype(np.float64)))
+
+
+class Pipeline:
+    def __init__(self, p, precision, torch):
+        from bem3d_elastostatic_amd.quadesth import Quadesth
+        self.torch = torch
+        self.p = p
+        self.q = Quadesth(p)
+        self.prec = precision
+        dev = torch.device("cuda:0")
+        self.pts = torch.tensor(np.ascontiguousarray(p.points), dtype=torch.float64, device=dev)
+        l = p.l
+        self.u = torch.empty(3 * l, dtype=torch.float64, device=dev)
+        self.s = torch.empty(6 * l, dtype=torch.float64, device=dev)
+        self.pr = torch.empty(3 * l, dtype=torch.float64, device=dev)
+        self.vm = torch.empty(l, dtype=torch.float64, device=dev)
+        self.b = torch.empty(3 * p.n, dtype=torch.float64, device=dev)
+        self.fi = torch.empty(3 * p.n, dtype=torch.float64, device=dev)
+        self.dfi = torch.empty(27 * p.ne, dtype=torch.float64, device=dev)
+        self.q.precompute_element_geometry()
+        torch.cuda.synchronize()
+
+    def step(self, precision=None, phases=None):
+        from bem3d_elastostatic_amd import _lib
+        torch, q = self.torch, self.q
+        prec = precision or self.prec
+        t0 = time.perf_counter()
+        q.ghmat8()
+        q.synchronize()
+        t1 = time.perf_counter()
+        _, _, fi_ptr = q.system_device()
+        N = 3 * self.p.n
+        rc = _hip().hipMemcpy(C.c_void_p(self.b.data_ptr()), C.c_void_p(fi_ptr), C.c_size_t(8 * N), 3)
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy (device to device) failed: {rc}")
+        rep = _lib.SolveReport()
+        rc = q.lib.qd_solve_device(q.h, _lib.PRECISIONS[prec], 1, C.c_void_p(self.b.data_ptr()), N, 1e-14, 30,
+                                   C.byref(rep))
+        _lib.check(q.h, rc, "solve")
+        t2 = time.perf_counter()
+        X = self.b.cpu().numpy()
+        FI, DFI = q.reorganize_solution(X)
+        self.fi.copy_(torch.from_numpy(FI))
+        self.dfi.copy_(torch.from_numpy(DFI.reshape(-1)))
+        t3 = time.perf_counter()
+        counts = np.zeros(2, dtype=np.int64)
+        rc = q.lib.qd_internal_points_device(
+            q.h, self.p.l, C.c_void_p(self.pts.data_ptr()), C.c_void_p(self.fi.data_ptr()),
+            C.c_void_p(self.dfi.data_ptr()), 0, C.c_void_p(self.u.data_ptr()), C.c_void_p(self.s.data_ptr()),
+            C.c_void_p(self.pr.data_ptr()), C.c_void_p(self.vm.data_ptr()),
+            counts.ctypes.data_as(C.POINTER(C.c_longlong)))
+        _lib.check(q.h, rc, "internal points")
+        q.synchronize()
+        t4 = time.perf_counter()
+        if phases is not None:
+            phases.append(dict(assembly=t1 - t0, solve=t2 - t1, reorganize=t3 - t2, internal=t4 - t3,
+                               factor_ms=rep.t_factor_ms, refine_ms=rep.t_solve_ms, iterations=rep.iterations,
+                               gmres=rep.gmres, residual=rep.residual, counts=counts.tolist()))
+        return FI, DFI
+
+_HIP = None
+
+
+def _hip():
+    global _HIP
+    if _HIP is None:
+        _HIP = C.CDLL("libamdhip64.so")
+        _HIP.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _HIP.hipMemcpy.restype = C.c_int
+    return _HIP
+
+
+def cpu_baseline(p, FI, DFI, budget_s=25.0):
+    """The oracle (a C/OpenMP restatement of the reference's loops) and numpy
+    LAPACK timed on bounded samples of the same workload, extrapolated to one
+    whole solve."""
+    from oracle.pyoracle import Oracle
+    o = Oracle()
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t = o.geometry(p)
+    rng = np.random.default_rng(0)
+    # assembly: sampled collocation nodes, all elements
+    k = 48
+    nodes = np.sort(rng.choice(p.n, k, replace=False))
+    t0 = time.perf_counter()
+    o.ghmat8_rows(p, nodes, t)
+    ta = time.perf_counter() - t0
+    per_node = ta / k
+    t_asm = per_node * p.n
+    # LU: numpy (LAPACK gesv) at a size that runs in seconds, scaled by N^3
+    ns = 5000
+    A = rng.normal(size=(ns, ns))
+    bb = rng.normal(size=ns)
+    t0 = time.perf_counter()
+    np.linalg.solve(A, bb)
+    tl = time.perf_counter() - t0
+    N = 3 * p.n
+    t_lu = tl * (N / ns) ** 3
+    # Somigliana: two sample sizes, slope per point (the near table is built per call)
+    s1, s2 = 16, 48
+    pts = p.points
+    t0 = time.perf_counter()
+    o.internal_points(p, t, FI, DFI, points=pts[:s1])
+    t1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.internal_points(p, t, FI, DFI, points=pts[:s2])
+    t2 = time.perf_counter() - t0
+    slope = max(t2 - t1, 1e-9) / (s2 - s1)
+    fixed = max(t1 - slope * s1, 0.0)
+    t_som = fixed + slope * p.l
+    total = t_asm + t_lu + t_som
+    return dict(value=round(total, 3), unit="s/solve", cores=threads, kind="port",
+                sample=(f"oracle ghmat8 rows of {k} of {p.n} collocation nodes ({ta:.2f} s), numpy LAPACK "
+                        f"gesv N={ns} ({tl:.2f} s) scaled by (N/{ns})^3, oracle Somigliana at {s1} and {s2} of "
+                        f"{p.l} points ({t1:.2f}+{t2:.2f} s); extrapolated phases s: assembly {t_asm:.1f}, "
+                        f"LU {t_lu:.1f}, internal points {t_som:.1f}"),
+                phases=dict(assembly=t_asm, lu=t_lu, internal=t_som))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", default="fp16", choices=["fp16", "fp32", "fp64"])
+    ap.add_argument("--config", default="c3", choices=["c3", "c1"])
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the dense LU of this path is not distributed in this build "
+              "(the sharding plan is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from bem3d_elastostatic_amd import mesh
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    if args.config == "c3":
+        p = mesh.config3()
+        workload = ("thick cantilever box [0,10]x[0,2]x[0,1], h=0.1: 25602 nodes, 2400 quad9 + 8000 tri6, "
+                    "76806 DOF, 65536 internal points; E=210e3 nu=0.3, x=0 clamped, t_z=-1 on x=10")
+    else:
+        p = mesh.config1()
+        workload = "unit cube 16x16 quad9 per face: 6146 nodes, 18438 DOF, 4096 internal points"
+    N = 3 * p.n
+    log(f"workload {args.config}: n={p.n} ne={p.ne} N={N} l={p.l}")
+    pipe = Pipeline(p, args.precision, torch)
+    q = pipe.q
+
+    for w in range(args.warmup):
+        ph = []
+        pipe.step(phases=ph)
+        log(f"warmup {w}: {ph[-1]}")
+    q.profile_enable(["lu_trailing_gemm", "lu_panel", "assembly_h", "assembly_g", "somigliana_far",
+                      "somigliana_near"])
+    for k in ("lu_trailing_gemm", "lu_panel", "assembly_h", "assembly_g", "somigliana_far", "somigliana_near"):
+        q.profile_read(k)
+    phases = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        FI, DFI = pipe.step(phases=phases)
+        log(f"step {s}: {phases[-1]}")
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    prof = {k: q.profile_read(k) for k in ("lu_trailing_gemm", "lu_panel", "assembly_h", "assembly_g",
+                                          "somigliana_far", "somigliana_near")}
+    q.profile_enable([])
+    u_mixed, s_mixed = pipe.u.cpu().numpy(), pipe.s.cpu().numpy()
+    ms_per_step = 1000.0 * dt / args.steps
+
+    def mean(key):
+        return float(np.mean([ph[key] for ph in phases]))
+
+    t_asm = mean("assembly")
+    t_solve = mean("solve")
+    entries = hg_entries(p)
+    lu_flops = 2.0 / 3.0 * N ** 3
+    t_factor = mean("factor_ms") / 1000.0
+    extra = dict(
+        phases_s=dict(assembly=t_asm, solve=t_solve, factor=t_factor, refine=mean("refine_ms") / 1000.0,
+                      reorganize=mean("reorganize"), internal_points=mean("internal")),
+        hg_entries_per_s=entries / t_asm,
+        lu_mfma_pct=100.0 * lu_flops / t_factor / (PEAK_TFLOPS[args.precision] * 1e12),
+        ir_iterations=phases[-1]["iterations"], ir_gmres=phases[-1]["gmres"],
+        ir_residual=phases[-1]["residual"],
+        somigliana_gp_evals=dict(far=phases[-1]["counts"][0], near=phases[-1]["counts"][1]),
+    )
+    prof_out = {}
+    for k, (n_l, ms, work) in prof.items():
+        prof_out[k] = dict(launches=n_l, ms=ms, work=work,
+                           tflops=(work / (ms * 1e-3) / 1e12) if ms > 0 else None)
+    extra["kernels"] = prof_out
+    # dominant kernel class by device time
+    dom = max(prof_out, key=lambda k: prof_out[k]["ms"])
+    d = prof_out[dom]
+    peak = PEAK_TFLOPS[args.precision] if dom.startswith("lu") else PEAK_TFLOPS["fp64_vector"]
+    achieved = d["tflops"] or 0.0
+    roofline = dict(bound="mfma" if dom.startswith("lu") else "valu", kernel=dom, achieved=round(achieved, 3),
+                    peak=peak, unit="TFLOP/s", frac=round(achieved / peak, 4), traffic=None,
+                    launches=d["launches"], avg_launch_ms=d["ms"] / max(d["launches"], 1),
+                    work_per_launch=d["work"] / max(d["launches"], 1))
+
+    if not args.no_fp64 and args.precision != "fp64":
+        log("FP64 LU path, one step")
+        ph64 = []
+        FI64, DFI64 = pipe.step(precision="fp64", phases=ph64)
+        log(f"fp64 step: {ph64[-1]}")
+        u64, s64 = pipe.u.cpu().numpy(), pipe.s.cpu().numpy()
+        t64 = ph64[-1]["assembly"] + ph64[-1]["solve"] + ph64[-1]["reorganize"] + ph64[-1]["internal"]
+
+        def rel(a, b):
+            return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+        mask = np.arange(27)[None, :] < 3 * p.nconec[:, None]
+        extra["fp64_path_s_per_solve"] = t64
+        extra["fp64_factor_s"] = ph64[-1]["factor_ms"] / 1000.0
+        extra["fp64_lu_mfma_pct"] = 100.0 * lu_flops / (ph64[-1]["factor_ms"] / 1000.0) / (PEAK_TFLOPS["fp64"] * 1e12)
+        extra["speedup_vs_fp64"] = t64 / (ms_per_step / 1000.0)
+        extra["errors_vs_fp64"] = dict(u_boundary=rel(FI, FI64), t_boundary=rel(DFI[mask], DFI64[mask]),
+                                       u_internal=rel(u_mixed, u64), sigma_internal=rel(s_mixed, s64))
+    cpu = None
+    if not args.no_cpu:
+        log("CPU baseline sample")
+        cpu = cpu_baseline(p, FI, DFI)
+        log(f"cpu baseline {cpu}")
+    out = {
+        "metric": METRIC, "value": round(ms_per_step / 1000.0, 4), "unit": "s/solve", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"fp16": "f16 MFMA LU (f32 acc) + f64 refine/assembly/Somigliana",
+                  "fp32": "f32 MFMA LU + f64 refine/assembly/Somigliana", "fp64": "f64"}[args.precision],
+        "data": "synthetic", "config": {"workload": workload, "precision": args.precision, "dof": N,
+                                        "internal_points": p.l},
+        "roofline": roofline, "cpu_baseline": cpu, **extra,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -141,6 +141,25 @@ int qd_internal_points_device(qd_handle h, int l, const double* pts_dev, const d
 int qd_boundary_stresses(qd_handle h, const double* fi, const double* dfi, double* sigma_elem,
                          double* sigma_node, double* prin, double* vm);
 
+/* ---- instrumentation (no reference counterpart: the reference only has
+ * cpu_time stopwatches, main.f90:27-71) ------------------------------------
+ * With profiling on for a kernel class, every launch of that class is
+ * bracketed by HIP events on the handle's stream; qd_profile_read returns
+ * the launch count, the summed device time (ms) and the summed algorithmic
+ * work (flops for GEMMs, H,G entries for the assembly passes, Gauss-point
+ * evaluations for the Somigliana passes). */
+enum {
+  QD_PROF_LU_TRAILING_GEMM = 0,  /* A22 -= L21 U12 of the outer LU steps */
+  QD_PROF_LU_PANEL = 1,          /* panel factorisation (all its launches) */
+  QD_PROF_ASSEMBLY_H = 2,        /* H pass of ghmat8 */
+  QD_PROF_ASSEMBLY_G = 3,        /* G pass of ghmat8 */
+  QD_PROF_SOMIGLIANA_FAR = 4,    /* far-field pass of the internal points */
+  QD_PROF_SOMIGLIANA_NEAR = 5,   /* near-field pass of the internal points */
+  QD_PROF_KINDS = 6
+};
+int qd_profile_enable(qd_handle h, unsigned mask /* bit per QD_PROF_* */);
+int qd_profile_read(qd_handle h, int kind, long long* launches, double* ms, double* work);
+
 /* ---- LAPACK drop-in ------------------------------------------------------
  * DGESV with the LAPACK Fortran ABI (the call at solver_mod.f90:28): a gets
  * the LU factors (column-major), ipiv the 1-based row interchanges, b the
