@@ -31,6 +31,7 @@
 #include <chrono>
 #include <cmath>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "qd_internal.h"
@@ -55,6 +56,7 @@ struct qd_lu_state {
   qd::DevBuf<float> vecf;
   qd::DevBuf<double> gm;         // GMRES basis
   qd::DevBuf<double> rowpart;    // residual slice partials
+  qd::DevBuf<int> ts_a, ts_b;    // tournament candidate lists
   double anorm = -1.0;           // ||A||_inf of the FP64 matrix
 };
 
@@ -62,7 +64,10 @@ namespace qd {
 
 constexpr int kLeaf = 32;
 constexpr int kColThreads = 256;
-constexpr int kColRowsPerThread = 2;
+#ifndef QD_COLSTEP_ROWS
+#define QD_COLSTEP_ROWS 64
+#endif
+constexpr int kColRowsPerBlock = QD_COLSTEP_ROWS;
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -110,30 +115,35 @@ __global__ __launch_bounds__(256) void k_transpose_out(long long N, const double
 //    on leaf columns [c, lend);
 //  - when c < lend: pivot search in column c; the last block picks the pivot,
 //    swaps rows c and p across panel columns [pstart, pend) and stores 1/pivot.
+// 32 lanes cover one row's leaf segment (columns lstart + lane), so every row
+// access is one contiguous 128/256-byte segment; 8 rows per pass.
 template <typename T>
 __global__ __launch_bounds__(kColThreads) void k_colstep(T* __restrict__ W, long long ld, long long N, long long c,
-                                                         long long prev, long long lend, long long pstart,
-                                                         long long pend, double* __restrict__ part,
+                                                         long long prev, long long lstart, long long lend,
+                                                         long long pstart, long long pend, double* __restrict__ part,
                                                          int* __restrict__ pidx, int* __restrict__ state,
                                                          double* __restrict__ rpiv, int* __restrict__ ipiv) {
   __shared__ double sv[kColThreads];
   __shared__ int si[kColThreads];
   __shared__ int am_last;
   const int tid = threadIdx.x;
+  const int t = tid & 31, slot = tid >> 5;
+  const long long col = lstart + t;
+  const bool incol = col < lend;
   double best = -1.0;
   int bidx = 0x7fffffff;
   const double rp = (prev >= 0) ? rpiv[0] : 0.0;
-  for (int q = 0; q < kColRowsPerThread; ++q) {
-    const long long r = c + ((long long)blockIdx.x * kColRowsPerThread + q) * kColThreads + tid;
+  const T pv = (prev >= 0 && incol && col > prev) ? W[prev * ld + col] : (T)0;
+  const long long rb = c + (long long)blockIdx.x * kColRowsPerBlock;
+  for (int q = 0; q < kColRowsPerBlock / 8; ++q) {
+    const long long r = rb + q * 8 + slot;
     if (r >= N) break;
     T* row = W + r * ld;
     if (prev >= 0) {
       const T l = (T)((double)row[prev] * rp);
-      row[prev] = l;
-      const T* prow = W + prev * ld;
-      for (long long j = c; j < lend; ++j) row[j] -= l * prow[j];
+      if (incol && col >= prev) row[col] = (col == prev) ? l : row[col] - l * pv;
     }
-    if (c < lend) {
+    if (c < lend && col == c) {
       const double v = fabs((double)row[c]);
       if (v > best) {
         best = v;
@@ -156,13 +166,21 @@ __global__ __launch_bounds__(kColThreads) void k_colstep(T* __restrict__ W, long
     }
     __syncthreads();
   }
+  // Publish the block maximum, then count the block in.  One device-scope
+  // fence per block (by the thread that publishes), after the barrier that
+  // orders the block's row updates before it.
+  __syncthreads();
   if (tid == 0) {
     part[blockIdx.x] = sv[0];
     pidx[blockIdx.x] = si[0];
+#ifdef QD_COLSTEP_FENCE_ALL
   }
   __threadfence();
   __syncthreads();
   if (tid == 0) {
+#else
+    __threadfence();
+#endif
     const int t = atomicAdd(&state[0], 1);
     am_last = (t == (int)gridDim.x - 1);
   }
@@ -381,59 +399,112 @@ __global__ __launch_bounds__(256) void k_colblock_scale(int K, long long N, cons
   if (threadIdx.x == 0) scale[blockIdx.x] = red[0] > 0.0f ? red[0] : 1.0f;
 }
 
-// C -= A * B with FP16 MFMA (f32 accumulate): A (L21, |l|<=1) rounded to fp16
-// directly, B (U12) scaled by 1/colscale per 64-column block; the scale is
-// applied back in the epilogue.
-__global__ __launch_bounds__(256) void k_gemm_sub_f16(long long M, long long N, int K, const float* __restrict__ A,
-                                                      long long lda, const float* __restrict__ B, long long ldb,
-                                                      const float* __restrict__ colscale, float* __restrict__ C,
-                                                      long long ldc) {
-  constexpr int BK = 32, PAD = 8;
-  __shared__ _Float16 sA[GBM][BK + PAD];  // [m][k]
-  __shared__ _Float16 sB[GBN][BK + PAD];  // [n][k]
-  const long long m0 = (long long)blockIdx.y * GBM, n0 = (long long)blockIdx.x * GBN;
-  const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
-  const int wm = (wave / 2) * 64, wn = (wave % 2) * 64;
-  const int lr = lane % 16, lk = lane / 16;
-  f4 acc[4][4];
+// FP16 trailing update.  Operands are converted once per update: A16 = L21
+// rounded to fp16 (|l| <= 1 under partial pivoting, no scale needed), B16 =
+// U12 transposed to (N x K) and divided by the per-64-column maxima colscale.
+// Both are K-contiguous, K padded to a multiple of 32 with zeros.
+__global__ __launch_bounds__(256) void k_cvt_a16(long long M, int K, int Kp, const float* __restrict__ A, long long lda,
+                                                 _Float16* __restrict__ A16) {
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;  // one thread per 8 halves
+  const int per = Kp / 8;
+  if (idx >= M * per) return;
+  const long long r = idx / per;
+  const int k0 = (int)(idx % per) * 8;
+  h8 v;
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int q = 0; q < 8; ++q) v[q] = (_Float16)((k0 + q < K) ? A[r * lda + k0 + q] : 0.0f);
+  *(h8*)(A16 + r * Kp + k0) = v;
+}
+
+__global__ __launch_bounds__(256) void k_cvt_b16t(int K, long long N, int Kp, const float* __restrict__ B, long long ldb,
+                                                  const float* __restrict__ colscale, _Float16* __restrict__ B16) {
+  __shared__ float tile[32][33];
+  const long long n0 = (long long)blockIdx.x * 32;
+  const int k0 = blockIdx.y * 32;
+  const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;
+  for (int q = ty; q < 32; q += 8) {
+    const int k = k0 + q;
+    const long long n = n0 + tx;
+    tile[q][tx] = (k < K && n < N) ? B[(long long)k * ldb + n] / colscale[n >> 6] : 0.0f;  // tile[k][n]
+  }
+  __syncthreads();
+  for (int q = ty; q < 32; q += 8) {
+    const long long n = n0 + q;
+    const int k = k0 + tx;
+    if (n < N && k < Kp) B16[n * Kp + k] = (_Float16)tile[tx][q];
+  }
+}
+
+// C[M x N] (fp32, row-major) -= colscale[n/64] * (A16 B16^T).  256 x 128 tile
+// per 256-thread workgroup, 4 waves of 128 x 64 (8 x 4 MFMA 16x16x32 tiles),
+// K steps of 32 through double-buffered LDS with register prefetch.
+// Workgroups are remapped so that each XCD receives a contiguous band of
+// tiles (its L2 then serves the shared A/B rows).
+constexpr int H_BM = 256, H_BN = 128, H_BK = 32, H_PAD = 8;
+__global__ __launch_bounds__(256) void k_gemm16(long long M, long long N, int Kp, const _Float16* __restrict__ A16,
+                                                const _Float16* __restrict__ B16, const float* __restrict__ colscale,
+                                                float* __restrict__ C, long long ldc) {
+  __shared__ __attribute__((aligned(16))) _Float16 sA[2][H_BM][H_BK + H_PAD];
+  __shared__ __attribute__((aligned(16))) _Float16 sB[2][H_BN][H_BK + H_PAD];
+  const unsigned total = gridDim.x * gridDim.y;
+  const unsigned bid = blockIdx.y * gridDim.x + blockIdx.x;
+  const unsigned per_xcd = total / 8;
+  const unsigned vid = (bid < per_xcd * 8) ? (bid % 8) * per_xcd + bid / 8 : bid;
+  const long long m0 = (long long)(vid / gridDim.x) * H_BM, n0 = (long long)(vid % gridDim.x) * H_BN;
+  const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+  const int wm = (wave / 2) * 128, wn = (wave % 2) * 64;
+  const int lr = lane % 16, lk = lane / 16;
+  f4 acc[8][4];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = (f4){0, 0, 0, 0};
-  float inv_s[2];
+  uint4 ra[4], rb[2];
+  const long long ga = m0 + tid;            // A row loaded by this thread
+  const long long gb = n0 + tid / 2;        // B row
+  const int boff = (tid % 2) * 16;
+  auto gload = [&](int k0) {
+    if (ga < M) {
+      const uint4* p = (const uint4*)(A16 + ga * Kp + k0);
 #pragma unroll
-  for (int q = 0; q < 2; ++q) inv_s[q] = 1.0f / colscale[(n0 >> 6) + q < ((N + 63) >> 6) ? (n0 >> 6) + q : 0];
-  for (int k0 = 0; k0 < K; k0 += BK) {
+      for (int q = 0; q < 4; ++q) ra[q] = p[q];
+    } else {
 #pragma unroll
-    for (int i = 0; i < GBM * BK / 256; ++i) {
-      const int idx = tid + 256 * i;
-      const int r = idx / BK, c = idx % BK;
-      const long long gr = m0 + r;
-      const int gc = k0 + c;
-      sA[r][c] = (_Float16)((gr < M && gc < K) ? A[gr * lda + gc] : 0.0f);
+      for (int q = 0; q < 4; ++q) ra[q] = make_uint4(0, 0, 0, 0);
     }
-#pragma unroll
-    for (int i = 0; i < GBN * BK / 256; ++i) {
-      const int idx = tid + 256 * i;
-      const int r = idx / GBN, c = idx % GBN;
-      const int gr = k0 + r;
-      const long long gc = n0 + c;
-      sB[c][r] = (_Float16)((gr < K && gc < N) ? B[(long long)gr * ldb + gc] * inv_s[c >> 6] : 0.0f);
+    if (gb < N) {
+      const uint4* p = (const uint4*)(B16 + gb * Kp + k0 + boff);
+      rb[0] = p[0];
+      rb[1] = p[1];
+    } else {
+      rb[0] = rb[1] = make_uint4(0, 0, 0, 0);
     }
-    __syncthreads();
-    h8 a[4], b[4];
+  };
+  auto lstore = [&](int buf) {
+    uint4* da = (uint4*)&sA[buf][tid][0];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int q = 0; q < 4; ++q) da[q] = ra[q];
+    uint4* db = (uint4*)&sB[buf][tid / 2][boff];
+    db[0] = rb[0];
+    db[1] = rb[1];
+  };
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  const int nk = Kp / H_BK;
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) gload((kt + 1) * H_BK);
+    h8 b[4];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) a[i][q] = sA[wm + 16 * i + lr][8 * lk + q];
+    for (int j = 0; j < 4; ++j) b[j] = *(const h8*)&sB[cur][wn + 16 * j + lr][8 * lk];
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int i = 0; i < 8; ++i) {
+      const h8 a = *(const h8*)&sA[cur][wm + 16 * i + lr][8 * lk];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) b[j][q] = sB[wn + 16 * j + lr][8 * lk + q];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], b[j], acc[i][j], 0, 0, 0);
+      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b[j], acc[i][j], 0, 0, 0);
+    }
+    if (kt + 1 < nk) lstore(cur ^ 1);
     __syncthreads();
   }
 #pragma unroll
@@ -441,12 +512,194 @@ __global__ __launch_bounds__(256) void k_gemm_sub_f16(long long M, long long N, 
     const long long gc = n0 + wn + 16 * j + lr;
     const float s = (gc < N) ? colscale[gc >> 6] : 0.0f;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 8; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const long long gr = m0 + wm + 16 * i + 4 * lk + r;
         if (gr < M && gc < N) C[gr * ldc + gc] -= s * acc[i][j][r];
       }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tournament pivoting for a leaf of <= 32 columns (the factorisation that
+// only preconditions the FP64 refinement).  k_tslu_select runs Gaussian
+// elimination with partial pivoting on up to 1024 rows per workgroup (4 per
+// thread, held in registers) and emits the 32 winning row numbers in pivot
+// order; applied first to the leaf's rows and then, level by level, to the
+// winners of groups of 32 workgroups, until one workgroup's winners remain.
+// The rows themselves are read from W, which is not modified before the
+// winners are known, so every level works on the original values.
+constexpr int kTsW = 32, kTsRPT = 4, kTsRows = 256 * kTsRPT;
+
+__device__ __forceinline__ void argmax_pair(double& v, int& i, double ov, int oi) {
+  if (ov > v || (ov == v && oi < i)) {
+    v = ov;
+    i = oi;
+  }
+}
+
+struct TsShared {
+  float srow[kTsW];
+  double wv[4];
+  int wi[4];
+};
+
+// One elimination step (column K) of the block's rows; K is a compile-time
+// constant so the register array is indexed statically.
+template <int K>
+__device__ __forceinline__ void ts_step(float (&a)[kTsRPT][kTsW], int (&gidx)[kTsRPT], bool& done, int w,
+                                        TsShared& sh, int* __restrict__ out_idx) {
+  const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  done = done || (K >= w);
+  double best = -1.0;
+  int bi = 0x7fffffff;
+#pragma unroll
+  for (int q = 0; q < kTsRPT; ++q)
+    if (!done && gidx[q] >= 0) argmax_pair(best, bi, fabs((double)a[q][K]), gidx[q]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double ov = __shfl_xor(best, off, 64);
+    const int oi = __shfl_xor(bi, off, 64);
+    argmax_pair(best, bi, ov, oi);
+  }
+  if (lane == 0) {
+    sh.wv[wave] = best;
+    sh.wi[wave] = bi;
+  }
+  __syncthreads();
+  double bb = sh.wv[0];
+  int ib = sh.wi[0];
+#pragma unroll
+  for (int x = 1; x < 4; ++x) argmax_pair(bb, ib, sh.wv[x], sh.wi[x]);
+  if (!done && bb >= 0.0) {
+#pragma unroll
+    for (int q = 0; q < kTsRPT; ++q)
+      if (gidx[q] >= 0 && gidx[q] == ib) {
+#pragma unroll
+        for (int j = 0; j < kTsW; ++j) sh.srow[j] = a[q][j];
+        gidx[q] = -1;
+      }
+  }
+  if (tid == 0) out_idx[(long long)blockIdx.x * kTsW + K] = (!done && bb >= 0.0) ? ib : -1;
+  done = done || (bb < 0.0);
+  __syncthreads();
+  if (!done) {
+    const float piv = sh.srow[K];
+    const float rp = (piv != 0.0f) ? 1.0f / piv : 0.0f;
+#pragma unroll
+    for (int q = 0; q < kTsRPT; ++q) {
+      const float l = (gidx[q] >= 0) ? a[q][K] * rp : 0.0f;
+#pragma unroll
+      for (int j = K + 1; j < kTsW; ++j) a[q][j] -= l * sh.srow[j];
+    }
+  }
+  __syncthreads();
+}
+
+template <int... Ks>
+__device__ __forceinline__ void ts_all(std::integer_sequence<int, Ks...>, float (&a)[kTsRPT][kTsW],
+                                       int (&gidx)[kTsRPT], bool& done, int w, TsShared& sh,
+                                       int* __restrict__ out_idx) {
+  (ts_step<Ks>(a, gidx, done, w, sh, out_idx), ...);
+}
+
+__global__ __launch_bounds__(256) void k_tslu_select(const float* __restrict__ W, long long ld, long long N,
+                                                     long long c0, int w, const int* __restrict__ in_idx, int n_in,
+                                                     int* __restrict__ out_idx) {
+  __shared__ TsShared sh;
+  const int tid = threadIdx.x;
+  float a[kTsRPT][kTsW];
+  int gidx[kTsRPT];
+#pragma unroll
+  for (int q = 0; q < kTsRPT; ++q) {
+    const long long li = (long long)blockIdx.x * kTsRows + tid + 256 * q;
+    long long g;
+    if (in_idx) g = (li < n_in) ? in_idx[li] : -1;
+    else g = (c0 + li < N) ? c0 + li : -1;
+    gidx[q] = (int)g;
+#pragma unroll
+    for (int j = 0; j < kTsW; ++j) a[q][j] = (g >= 0 && j < w) ? W[g * ld + c0 + j] : 0.0f;
+  }
+  bool done = false;
+  ts_all(std::make_integer_sequence<int, kTsW>{}, a, gidx, done, w, sh, out_idx);
+}
+
+// LAPACK interchange sequence that brings the winners p_0..p_{w-1} to rows
+// c0..c0+w-1 in order: ipiv[c0+k] = current position of p_k.
+__global__ void k_tslu_ipiv(const int* __restrict__ win, int w, long long c0, int* __restrict__ ipiv) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int prow[2 * kTsW], ppos[2 * kTsW];  // row -> position, for rows that moved
+  int np = 0;
+  auto pos_of = [&](int row) {
+    for (int i = 0; i < np; ++i)
+      if (prow[i] == row) return ppos[i];
+    return row;
+  };
+  auto set_pos = [&](int row, int pos) {
+    for (int i = 0; i < np; ++i)
+      if (prow[i] == row) {
+        ppos[i] = pos;
+        return;
+      }
+    prow[np] = row;
+    ppos[np] = pos;
+    ++np;
+  };
+  auto row_at = [&](int pos) {
+    for (int i = 0; i < np; ++i)
+      if (ppos[i] == pos) return prow[i];
+    return pos;  // an unmoved row sits at its own index
+  };
+  for (int k = 0; k < w; ++k) {
+    const int t = (int)c0 + k;
+    const int p = win[k];
+    const int cur = pos_of(p);
+    ipiv[t] = cur;
+    if (cur != t) {
+      const int r_t = row_at(t);
+      set_pos(p, t);
+      set_pos(r_t, cur);
+    }
+  }
+}
+
+// After the interchanges: unpivoted LU of the top w x w block (the winners
+// in pivot order) and L21 = A21 U11^-1 for the rows below.  32 lanes per
+// row; 8 rows per pass.
+__global__ __launch_bounds__(256) void k_tslu_finish(float* __restrict__ W, long long ld, long long N, long long c0,
+                                                     int w) {
+  __shared__ float sU[kTsW][kTsW + 1];
+  const int tid = threadIdx.x, t = tid & 31, slot = tid >> 5;
+  for (int i = tid; i < kTsW * kTsW; i += 256) {
+    const int r = i / kTsW, cc = i % kTsW;
+    sU[r][cc] = (r < w && cc < w) ? W[(c0 + r) * ld + c0 + cc] : 0.0f;
+  }
+  __syncthreads();
+  // LU of the top block, all threads: multipliers of column k, then the
+  // rank-1 update of the trailing (w-k-1)^2 entries
+  for (int k = 0; k < w; ++k) {
+    if (tid > k && tid < w) sU[tid][k] = sU[tid][k] / sU[k][k];
+    __syncthreads();
+    for (int i = tid; i < kTsW * kTsW; i += 256) {
+      const int r = i / kTsW, cc = i % kTsW;
+      if (r > k && r < w && cc > k && cc < w) sU[r][cc] -= sU[r][k] * sU[k][cc];
+    }
+    __syncthreads();
+  }
+  if (blockIdx.x == 0)
+    for (int i = tid; i < w * w; i += 256) W[(c0 + i / w) * ld + c0 + i % w] = sU[i / w][i % w];
+  const long long rb = c0 + w + (long long)blockIdx.x * 64;
+  for (int q = 0; q < 8; ++q) {
+    const long long r = rb + q * 8 + slot;
+    if (r >= N) break;
+    float x = (t < w) ? W[r * ld + c0 + t] : 0.0f;
+    for (int k = 0; k < w; ++k) {
+      const float xk = __shfl(x, k, 32) / sU[k][k];
+      if (t == k) x = xk;
+      if (t > k) x -= xk * sU[k][t];
+    }
+    if (t < w) W[r * ld + c0 + t] = x;
   }
 }
 
@@ -458,7 +711,8 @@ struct LuCtx {
   qd_lu_state* s;
   T* W;
   long long ld, N;
-  bool f16;  // trailing updates in fp16
+  bool f16;   // trailing updates in fp16
+  bool calu;  // tournament pivoting in the leaves (preconditioner factorisations)
 };
 
 template <typename T>
@@ -468,10 +722,18 @@ static int gemm_sub(LuCtx<T>& c, long long M, long long Nn, int K, const T* A, c
   if constexpr (std::is_same<T, float>::value) {
     if (c.f16 && allow_f16) {
       const long long nb = (Nn + 63) / 64;
+      const int Kp = (K + 31) / 32 * 32;
       QD_ALLOC(c.h, c.s->colscale, (size_t)nb);
+      QD_ALLOC(c.h, c.s->h16a, (size_t)M * Kp);
+      QD_ALLOC(c.h, c.s->h16b, (size_t)Nn * Kp);
       hipLaunchKernelGGL(k_colblock_scale, dim3((unsigned)nb), dim3(256), 0, c.h->stream, K, Nn, B, c.ld,
                          c.s->colscale.p);
-      hipLaunchKernelGGL(k_gemm_sub_f16, grid, dim3(256), 0, c.h->stream, M, Nn, K, A, c.ld, B, c.ld,
+      hipLaunchKernelGGL(k_cvt_b16t, dim3((unsigned)((Nn + 31) / 32), (unsigned)(Kp / 32)), dim3(256), 0,
+                         c.h->stream, K, Nn, Kp, B, c.ld, c.s->colscale.p, c.s->h16b.p);
+      hipLaunchKernelGGL(k_cvt_a16, dim3((unsigned)((M * (Kp / 8) + 255) / 256)), dim3(256), 0, c.h->stream, M, K,
+                         Kp, A, c.ld, c.s->h16a.p);
+      dim3 g16((unsigned)((Nn + H_BN - 1) / H_BN), (unsigned)((M + H_BM - 1) / H_BM));
+      hipLaunchKernelGGL(k_gemm16, g16, dim3(256), 0, c.h->stream, M, Nn, Kp, c.s->h16a.p, c.s->h16b.p,
                          c.s->colscale.p, C, c.ld);
       QD_HIP(c.h, hipGetLastError());
       return QD_OK;
@@ -503,14 +765,46 @@ static int trsm_lower(LuCtx<T>& c, long long r0, int m, long long c0, long long 
 }
 
 template <typename T>
-static int colstep(LuCtx<T>& c, long long col, long long prev, long long lend, long long pstart, long long pend) {
+static int colstep(LuCtx<T>& c, long long col, long long prev, long long lstart, long long lend, long long pstart,
+                   long long pend) {
   const long long rows = c.N - col;
   if (rows <= 0) return QD_OK;
-  const long long per = (long long)kColThreads * kColRowsPerThread;
-  const unsigned nb = (unsigned)((rows + per - 1) / per);
-  hipLaunchKernelGGL(k_colstep<T>, dim3(nb), dim3(kColThreads), 0, c.h->stream, c.W, c.ld, c.N, col, prev, lend,
-                     pstart, pend, c.s->piv_part.p, c.s->piv_idx.p, c.s->piv_state.p, c.s->rpiv.p,
+  const unsigned nb = (unsigned)((rows + kColRowsPerBlock - 1) / kColRowsPerBlock);
+  hipLaunchKernelGGL(k_colstep<T>, dim3(nb), dim3(kColThreads), 0, c.h->stream, c.W, c.ld, c.N, col, prev, lstart,
+                     lend, pstart, pend, c.s->piv_part.p, c.s->piv_idx.p, c.s->piv_state.p, c.s->rpiv.p,
                      c.s->ipiv_dev.p);
+  QD_HIP(c.h, hipGetLastError());
+  return QD_OK;
+}
+
+// Leaf of <= 32 columns by tournament pivoting: selection levels, the
+// interchange sequence, the interchanges on the panel columns, then the LU of
+// the winners and the multipliers of all other rows.
+static int tslu_leaf(LuCtx<float>& c, long long c0, int w, long long p0, long long p1) {
+  qd_lu_state* s = c.s;
+  const long long m = c.N - c0;
+  int nb = (int)((m + kTsRows - 1) / kTsRows);
+  QD_ALLOC(c.h, s->ts_a, (size_t)nb * kTsW);
+  QD_ALLOC(c.h, s->ts_b, (size_t)nb * kTsW);
+  int* in = s->ts_a.p;
+  int* out = s->ts_b.p;
+  hipLaunchKernelGGL(k_tslu_select, dim3(nb), dim3(256), 0, c.h->stream, c.W, c.ld, c.N, c0, w, (const int*)nullptr,
+                     0, in);
+  int n_in = nb * kTsW;
+  while (nb > 1) {
+    const int nb2 = (n_in + kTsRows - 1) / kTsRows;
+    hipLaunchKernelGGL(k_tslu_select, dim3(nb2), dim3(256), 0, c.h->stream, c.W, c.ld, c.N, c0, w, (const int*)in,
+                       n_in, out);
+    n_in = nb2 * kTsW;
+    std::swap(in, out);
+    nb = nb2;
+  }
+  hipLaunchKernelGGL(k_tslu_ipiv, dim3(1), dim3(1), 0, c.h->stream, (const int*)in, w, c0, s->ipiv_dev.p);
+  hipLaunchKernelGGL(k_laswp<float>, dim3((unsigned)((p1 - p0 + 255) / 256)), dim3(256), 0, c.h->stream, c.W, c.ld,
+                     p0, p1, (int)c0, (int)(c0 + w), s->ipiv_dev.p);
+  const long long below = std::max<long long>(m - w, 1);
+  hipLaunchKernelGGL(k_tslu_finish, dim3((unsigned)((below + 63) / 64)), dim3(256), 0, c.h->stream, c.W, c.ld, c.N,
+                     c0, w);
   QD_HIP(c.h, hipGetLastError());
   return QD_OK;
 }
@@ -519,11 +813,14 @@ static int colstep(LuCtx<T>& c, long long col, long long prev, long long lend, l
 template <typename T>
 static int panel(LuCtx<T>& c, long long c0, int w, long long p0, long long p1) {
   int rc;
+  if constexpr (std::is_same<T, float>::value) {
+    if (w <= kLeaf && c.calu) return tslu_leaf(c, c0, w, p0, p1);
+  }
   if (w <= kLeaf) {
     const long long lend = c0 + w;
     for (long long col = c0; col <= lend; ++col) {
       if (col >= c.N && col - 1 >= c.N) break;
-      rc = colstep(c, std::min(col, c.N), col > c0 ? col - 1 : -1, std::min(lend, c.N), p0, p1);
+      rc = colstep(c, std::min(col, c.N), col > c0 ? col - 1 : -1, c0, std::min(lend, c.N), p0, p1);
       if (rc) return rc;
     }
     return QD_OK;
@@ -541,8 +838,7 @@ template <typename T>
 static int factor(LuCtx<T>& c, int NB) {
   qd_lu_state* s = c.s;
   const long long N = c.N;
-  const long long per = (long long)kColThreads * kColRowsPerThread;
-  const size_t maxblocks = (size_t)((N + per - 1) / per) + 1;
+  const size_t maxblocks = (size_t)((N + kColRowsPerBlock - 1) / kColRowsPerBlock) + 1;
   QD_ALLOC(c.h, s->piv_part, maxblocks);
   QD_ALLOC(c.h, s->piv_idx, maxblocks);
   QD_ALLOC(c.h, s->piv_state, 2);
@@ -837,14 +1133,14 @@ static int ensure_factors(qd_context* h, int precision, qd_solve_report* rep) {
     QD_ALLOC(h, s->W64, (size_t)ld * (size_t)N);
     hipLaunchKernelGGL(k_transpose_in<double>, tg, dim3(256), 0, h->stream, N, h->A.p, h->lda, s->W64.p, ld);
     QD_HIP(h, hipGetLastError());
-    LuCtx<double> c{h, s, s->W64.p, ld, N, false};
+    LuCtx<double> c{h, s, s->W64.p, ld, N, false, false};
     rc = factor(c, NB);
   } else {
     s->W64.release();
     QD_ALLOC(h, s->W32, (size_t)ld * (size_t)N);
     hipLaunchKernelGGL(k_transpose_in<float>, tg, dim3(256), 0, h->stream, N, h->A.p, h->lda, s->W32.p, ld);
     QD_HIP(h, hipGetLastError());
-    LuCtx<float> c{h, s, s->W32.p, ld, N, precision == QD_FP16_IR};
+    LuCtx<float> c{h, s, s->W32.p, ld, N, precision == QD_FP16_IR, true};
     rc = factor(c, NB);
   }
   if (rc) return rc;
@@ -1223,7 +1519,7 @@ int dgesv_device(qd_context* h, int n, int nrhs, double* a, long long lda, int* 
   dim3 tg((unsigned)((N + 31) / 32), (unsigned)((N + 31) / 32));
   hipLaunchKernelGGL(k_transpose_in<double>, tg, dim3(256), 0, h->stream, N, a, lda, s->W64.p, ld);
   QD_HIP(h, hipGetLastError());
-  LuCtx<double> c{h, s, s->W64.p, ld, N, false};
+  LuCtx<double> c{h, s, s->W64.p, ld, N, false, false};
   int rc = factor(c, 512);
   if (rc) return rc;
   std::vector<int> ip(N);
