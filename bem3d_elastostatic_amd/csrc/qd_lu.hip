@@ -29,6 +29,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cmath>
 #include <type_traits>
 #include <utility>
@@ -57,6 +58,8 @@ struct qd_lu_state {
   qd::DevBuf<double> gm;         // GMRES basis
   qd::DevBuf<double> rowpart;    // residual slice partials
   qd::DevBuf<int> ts_a, ts_b;    // tournament candidate lists
+  qd::DevBuf<float> inv32;       // inverted diagonal blocks of L and U
+  qd::DevBuf<double> inv64;
   double anorm = -1.0;           // ||A||_inf of the FP64 matrix
 };
 
@@ -441,7 +444,7 @@ __global__ __launch_bounds__(256) void k_cvt_b16t(int K, long long N, int Kp, co
 // Workgroups are remapped so that each XCD receives a contiguous band of
 // tiles (its L2 then serves the shared A/B rows).
 constexpr int H_BM = 256, H_BN = 128, H_BK = 32, H_PAD = 8;
-__global__ __launch_bounds__(256) void k_gemm16(long long M, long long N, int Kp, const _Float16* __restrict__ A16,
+__global__ __launch_bounds__(256, 2) void k_gemm16(long long M, long long N, int Kp, const _Float16* __restrict__ A16,
                                                 const _Float16* __restrict__ B16, const float* __restrict__ colscale,
                                                 float* __restrict__ C, long long ldc) {
   __shared__ __attribute__((aligned(16))) _Float16 sA[2][H_BM][H_BK + H_PAD];
@@ -759,7 +762,7 @@ static int trsm_lower(LuCtx<T>& c, long long r0, int m, long long c0, long long 
   if (rc) return rc;
   // B2 -= L21 X1
   rc = gemm_sub(c, m - m1, c1 - c0, m1, c.W + (r0 + m1) * c.ld + r0, c.W + r0 * c.ld + c0,
-                c.W + (r0 + m1) * c.ld + c0, false);
+                c.W + (r0 + m1) * c.ld + c0, true);
   if (rc) return rc;
   return trsm_lower(c, r0 + m1, m - m1, c0, c1);
 }
@@ -829,7 +832,7 @@ static int panel(LuCtx<T>& c, long long c0, int w, long long p0, long long p1) {
   if ((rc = panel(c, c0, w1, p0, p1))) return rc;
   if ((rc = trsm_lower(c, c0, w1, c0 + w1, c0 + w))) return rc;
   if ((rc = gemm_sub(c, c.N - c0 - w1, w - w1, w1, c.W + (c0 + w1) * c.ld + c0, c.W + c0 * c.ld + c0 + w1,
-                     c.W + (c0 + w1) * c.ld + c0 + w1, false)))
+                     c.W + (c0 + w1) * c.ld + c0 + w1, true)))
     return rc;
   return panel(c, c0 + w1, w - w1, p0, p1);
 }
@@ -929,6 +932,127 @@ __global__ __launch_bounds__(256) void k_trsv_update(const T* __restrict__ W, lo
     for (int b = 0; b < m; ++b) s += sW[a][b] * sx[v][b];
     x[(long long)v * N + rb + a] -= s;
   }
+}
+
+// ---- triangular solves with inverted diagonal blocks ------------------------
+// After a factorisation the TB x TB diagonal blocks of L (unit lower) and U
+// are inverted once; every solve then advances one block per step with a
+// parallel block product (x_I = inv(D_I) x_I) and an update of the remaining
+// rows, instead of TB sequential substitution steps.
+template <typename T>
+struct TriTB {
+  static constexpr int v = sizeof(T) == 4 ? 128 : 64;
+};
+
+// One workgroup of TB threads per diagonal block; thread j forms column j of
+// the inverse by substitution on e_j.  inv: [block][TB][TB] row-major, zero
+// padded beyond the matrix.
+template <typename T, int TB>
+__global__ __launch_bounds__(TB) void k_diag_inverse(const T* __restrict__ W, long long ld, long long N, int lower,
+                                                     T* __restrict__ inv) {
+  __shared__ T sA[TB][TB + 1];
+  __shared__ T sX[TB][TB + 1];
+  const long long i0 = (long long)blockIdx.x * TB;
+  const int m = (int)std::min<long long>(TB, N - i0);
+  const int j = threadIdx.x;
+  for (int i = 0; i < TB; ++i) sA[i][j] = (i < m && j < m) ? W[(i0 + i) * ld + i0 + j] : (T)0;
+  __syncthreads();
+  if (lower) {
+    for (int i = 0; i < TB; ++i) {
+      T s = (i == j) ? (T)1 : (T)0;
+      for (int k = 0; k < i; ++k) s -= sA[i][k] * sX[k][j];
+      sX[i][j] = (i < m && j < m) ? s : (T)0;
+    }
+  } else {
+    for (int i = TB - 1; i >= 0; --i) {
+      T s = (i == j) ? (T)1 : (T)0;
+      for (int k = i + 1; k < m; ++k) s -= sA[i][k] * sX[k][j];
+      sX[i][j] = (i < m && j < m) ? s / sA[i][i] : (T)0;
+    }
+  }
+  T* out = inv + (size_t)blockIdx.x * TB * TB;
+  for (int i = 0; i < TB; ++i) out[i * TB + j] = sX[i][j];
+}
+
+// x_I <- inv_I x_I for block I (rows i0 .. i0+m), nrhs vectors of stride N.
+template <typename T, int TB>
+__global__ __launch_bounds__(256) void k_apply_inv(const T* __restrict__ invb, long long N, long long i0, int m,
+                                                   int nrhs, T* __restrict__ x) {
+  __shared__ T sx[8][TB];
+  for (int i = threadIdx.x; i < TB * nrhs; i += 256) {
+    const int v = i / TB, r = i % TB;
+    sx[v][r] = (r < m) ? x[(long long)v * N + i0 + r] : (T)0;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < m * nrhs; i += 256) {
+    const int v = i / m, r = i % m;
+    const T* row = invb + (size_t)r * TB;
+    T s = 0;
+    for (int k = 0; k < TB; ++k) s += row[k] * sx[v][k];
+    x[(long long)v * N + i0 + r] = s;
+  }
+}
+
+// x[rows] -= W[rows, c0:c0+m] x[c0:c0+m], rows [r0, r1), m <= TB.
+template <typename T, int TB>
+__global__ __launch_bounds__(256) void k_block_update(const T* __restrict__ W, long long ld, long long N, long long r0,
+                                                      long long r1, long long c0, int m, int nrhs,
+                                                      T* __restrict__ x) {
+  __shared__ T sW[64][TB + 1];
+  __shared__ T sx[8][TB];
+  const long long rb = r0 + (long long)blockIdx.x * 64;
+  for (int i = threadIdx.x; i < m * nrhs; i += 256) sx[i / m][i % m] = x[(long long)(i / m) * N + c0 + i % m];
+  for (int i = threadIdx.x; i < 64 * m; i += 256) {
+    const int a = i / m, b = i % m;
+    sW[a][b] = (rb + a < r1) ? W[(rb + a) * ld + c0 + b] : (T)0;
+  }
+  __syncthreads();
+  const int a = threadIdx.x % 64, q = threadIdx.x / 64;
+  if (rb + a >= r1) return;
+  for (int v = q; v < nrhs; v += 4) {
+    T s = 0;
+    for (int b = 0; b < m; ++b) s += sW[a][b] * sx[v][b];
+    x[(long long)v * N + rb + a] -= s;
+  }
+}
+
+template <typename T>
+static int diag_inverses(qd_context* h, const T* W, long long ld, long long N, T* inv) {
+  constexpr int TB = TriTB<T>::v;
+  const unsigned nblk = (unsigned)((N + TB - 1) / TB);
+  hipLaunchKernelGGL((k_diag_inverse<T, TB>), dim3(nblk), dim3(TB), 0, h->stream, W, ld, N, 1, inv);
+  hipLaunchKernelGGL((k_diag_inverse<T, TB>), dim3(nblk), dim3(TB), 0, h->stream, W, ld, N, 0,
+                     inv + (size_t)nblk * TB * TB);
+  QD_HIP(h, hipGetLastError());
+  return QD_OK;
+}
+
+template <typename T>
+static int tri_solve_inv(qd_context* h, const T* W, long long ld, long long N, int nrhs, T* x, const T* inv) {
+  constexpr int TB = TriTB<T>::v;
+  const long long nblk = (N + TB - 1) / TB;
+  const T* invL = inv;
+  const T* invU = inv + (size_t)nblk * TB * TB;
+  for (long long b = 0; b < nblk; ++b) {
+    const long long i0 = b * TB;
+    const int m = (int)std::min<long long>(TB, N - i0);
+    hipLaunchKernelGGL((k_apply_inv<T, TB>), dim3(1), dim3(256), 0, h->stream, invL + (size_t)b * TB * TB, N, i0, m,
+                       nrhs, x);
+    if (i0 + m < N)
+      hipLaunchKernelGGL((k_block_update<T, TB>), dim3((unsigned)((N - i0 - m + 63) / 64)), dim3(256), 0, h->stream,
+                         W, ld, N, i0 + m, N, i0, m, nrhs, x);
+  }
+  for (long long b = nblk - 1; b >= 0; --b) {
+    const long long i0 = b * TB;
+    const int m = (int)std::min<long long>(TB, N - i0);
+    hipLaunchKernelGGL((k_apply_inv<T, TB>), dim3(1), dim3(256), 0, h->stream, invU + (size_t)b * TB * TB, N, i0, m,
+                       nrhs, x);
+    if (i0 > 0)
+      hipLaunchKernelGGL((k_block_update<T, TB>), dim3((unsigned)((i0 + 63) / 64)), dim3(256), 0, h->stream, W, ld, N,
+                         0LL, i0, i0, m, nrhs, x);
+  }
+  QD_HIP(h, hipGetLastError());
+  return QD_OK;
 }
 
 template <typename T>
@@ -1127,7 +1251,10 @@ static int ensure_factors(qd_context* h, int precision, qd_solve_report* rep) {
   const double t0 = now_ms();
   dim3 tg((unsigned)((N + 31) / 32), (unsigned)((N + 31) / 32));
   int rc;
-  const int NB = 512;
+  // Panel width: the FP16 trailing update streams its FP32 C once per panel,
+  // so it takes a wider panel than the MFMA f64/f32 updates need.
+  int NB = precision == QD_FP16_IR ? 1024 : 512;
+  if (const char* e = std::getenv("QUADESTH_NB")) NB = std::max(64, std::atoi(e) / 32 * 32);
   if (precision == QD_FP64) {
     s->W32.release();
     QD_ALLOC(h, s->W64, (size_t)ld * (size_t)N);
@@ -1144,6 +1271,17 @@ static int ensure_factors(qd_context* h, int precision, qd_solve_report* rep) {
     rc = factor(c, NB);
   }
   if (rc) return rc;
+  {
+    if (precision == QD_FP64) {
+      constexpr int TB = TriTB<double>::v;
+      QD_ALLOC(h, s->inv64, (size_t)2 * ((N + TB - 1) / TB) * TB * TB);
+      if ((rc = diag_inverses<double>(h, s->W64.p, ld, N, s->inv64.p))) return rc;
+    } else {
+      constexpr int TB = TriTB<float>::v;
+      QD_ALLOC(h, s->inv32, (size_t)2 * ((N + TB - 1) / TB) * TB * TB);
+      if ((rc = diag_inverses<float>(h, s->W32.p, ld, N, s->inv32.p))) return rc;
+    }
+  }
   // interchanges -> permutation
   s->ipiv.resize(N);
   int st[2];
@@ -1189,7 +1327,7 @@ static int apply_inverse(qd_context* h, bool trans, int nrhs, const double* r, d
     double* t = s->vec.p + (size_t)4 * 8 * N;  // scratch slot
     if (!trans) {
       hipLaunchKernelGGL((k_gather<double, double>), dim3(nb), dim3(256), 0, h->stream, N, nrhs, s->perm_dev.p, r, N, t);
-      int rc = tri_solve<double>(h, s->W64.p, s->ld, N, nrhs, t);
+      int rc = tri_solve_inv<double>(h, s->W64.p, s->ld, N, nrhs, t, s->inv64.p);
       if (rc) return rc;
       QD_HIP(h, hipMemcpyAsync(d, t, (size_t)nrhs * N * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
     } else {
@@ -1202,7 +1340,7 @@ static int apply_inverse(qd_context* h, bool trans, int nrhs, const double* r, d
     float* t = s->vecf.p;
     if (!trans) {
       hipLaunchKernelGGL((k_gather<double, float>), dim3(nb), dim3(256), 0, h->stream, N, nrhs, s->perm_dev.p, r, N, t);
-      int rc = tri_solve<float>(h, s->W32.p, s->ld, N, nrhs, t);
+      int rc = tri_solve_inv<float>(h, s->W32.p, s->ld, N, nrhs, t, s->inv32.p);
       if (rc) return rc;
       hipLaunchKernelGGL((k_convert<float, double>), dim3(nb), dim3(256), 0, h->stream, N, nrhs, t, d);
     } else {
