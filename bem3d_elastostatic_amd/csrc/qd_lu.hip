@@ -383,6 +383,82 @@ __global__ __launch_bounds__(256) void k_gemm_sub(long long M, long long N, int 
       }
 }
 
+// FP64 C[M x N] -= A[M x K] B[K x N] (row-major, inside W) with MFMA
+// f64 16x16x4.  128 x 128 tile per 256-thread workgroup (4 waves of 64 x 64,
+// 4 x 4 MFMA tiles), K steps of 16 through double-buffered LDS with register
+// prefetch, two workgroups per CU, XCD-banded tile order.
+constexpr int D_BM = 128, D_BN = 128, D_BK = 16;
+__global__ __launch_bounds__(256, 2) void k_gemm64(long long M, long long N, int K, const double* __restrict__ A,
+                                                   long long lda, const double* __restrict__ B, long long ldb,
+                                                   double* __restrict__ C, long long ldc) {
+  __shared__ double sA[2][D_BM][D_BK + 1];  // [m][k]
+  __shared__ double sB[2][D_BK][D_BN + 2];  // [k][n]
+  const unsigned total = gridDim.x * gridDim.y;
+  const unsigned bid = blockIdx.y * gridDim.x + blockIdx.x;
+  const unsigned per_xcd = total / 8;
+  const unsigned vid = (bid < per_xcd * 8) ? (bid % 8) * per_xcd + bid / 8 : bid;
+  const long long m0 = (long long)(vid / gridDim.x) * D_BM, n0 = (long long)(vid % gridDim.x) * D_BN;
+  const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+  const int wm = (wave / 2) * 64, wn = (wave % 2) * 64;
+  const int lr = lane % 16, lk = lane / 16;
+  d4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = (d4){0, 0, 0, 0};
+  // A tile: 128 rows x 16: thread -> row tid/2, 8 consecutive k at (tid%2)*8
+  // B tile: 16 rows x 128: thread -> row tid/16, 8 consecutive n at (tid%16)*8
+  const int ar = tid / 2, ak = (tid % 2) * 8;
+  const int br = tid / 16, bn = (tid % 16) * 8;
+  double ra[8], rb[8];
+  auto gload = [&](int k0) {
+    const long long gr = m0 + ar;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) ra[q] = (gr < M && k0 + ak + q < K) ? A[gr * lda + k0 + ak + q] : 0.0;
+    const int gk = k0 + br;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) rb[q] = (gk < K && n0 + bn + q < N) ? B[(long long)gk * ldb + n0 + bn + q] : 0.0;
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) sA[buf][ar][ak + q] = ra[q];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) sB[buf][br][bn + q] = rb[q];
+  };
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  const int nk = (K + D_BK - 1) / D_BK;
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) gload((kt + 1) * D_BK);
+#pragma unroll
+    for (int kk = 0; kk < D_BK; kk += 4) {
+      double a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = sA[cur][wm + 16 * i + lr][kk + lk];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = sB[cur][kk + lk][wn + 16 * j + lr];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    if (kt + 1 < nk) lstore(cur ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long long gr = m0 + wm + 16 * i + lk + 4 * r;  // f64 16x16x4 rows interleave by 4
+        const long long gc = n0 + wn + 16 * j + lr;
+        if (gr < M && gc < N) C[gr * ldc + gc] -= acc[i][j][r];
+      }
+}
+
 // Per-64-column maxima of |U12| (fp16 mode): colscale[b] = max |B[:, 64b..64b+64)|.
 __global__ __launch_bounds__(256) void k_colblock_scale(int K, long long N, const float* __restrict__ B,
                                                         long long ldb, float* __restrict__ scale) {
@@ -741,6 +817,12 @@ static int gemm_sub(LuCtx<T>& c, long long M, long long Nn, int K, const T* A, c
       QD_HIP(c.h, hipGetLastError());
       return QD_OK;
     }
+  }
+  if constexpr (std::is_same<T, double>::value) {
+    dim3 g64((unsigned)((Nn + D_BN - 1) / D_BN), (unsigned)((M + D_BM - 1) / D_BM));
+    hipLaunchKernelGGL(k_gemm64, g64, dim3(256), 0, c.h->stream, M, Nn, K, A, c.ld, B, c.ld, C, c.ld);
+    QD_HIP(c.h, hipGetLastError());
+    return QD_OK;
   }
   hipLaunchKernelGGL(k_gemm_sub<T>, grid, dim3(256), 0, c.h->stream, M, Nn, K, A, c.ld, B, c.ld, C, c.ld);
   QD_HIP(c.h, hipGetLastError());
