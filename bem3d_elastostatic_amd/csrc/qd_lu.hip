@@ -539,6 +539,28 @@ __global__ __launch_bounds__(256, 2) void k_gemm16(long long M, long long N, int
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = (f4){0, 0, 0, 0};
   uint4 ra[4], rb[2];
+#ifndef QD_GEMM16_ROWPERLANE
+  // four lanes per 64-byte row segment: one instruction covers 16 rows
+  const int lrow = tid >> 2, lchunk = (tid & 3) * 8;
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const long long g = m0 + lrow + 64 * q;
+      ra[q] = (g < M) ? *(const uint4*)(A16 + g * Kp + k0 + lchunk) : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const long long g = n0 + lrow + 64 * q;
+      rb[q] = (g < N) ? *(const uint4*)(B16 + g * Kp + k0 + lchunk) : make_uint4(0, 0, 0, 0);
+    }
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) *(uint4*)&sA[buf][lrow + 64 * q][lchunk] = ra[q];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) *(uint4*)&sB[buf][lrow + 64 * q][lchunk] = rb[q];
+  };
+#else
   const long long ga = m0 + tid;            // A row loaded by this thread
   const long long gb = n0 + tid / 2;        // B row
   const int boff = (tid % 2) * 16;
@@ -567,6 +589,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm16(long long M, long long N, int
     db[0] = rb[0];
     db[1] = rb[1];
   };
+#endif
   gload(0);
   lstore(0);
   __syncthreads();
