@@ -152,7 +152,7 @@ struct ProfScope {
   double work;
   hipEvent_t a = nullptr;
   ProfScope(qd_context* h_, int kind_, double work_) : h(h_), kind(kind_), work(work_) {
-    if ((h->prof_mask >> kind) & 1) {
+    if (kind >= 0 && ((h->prof_mask >> kind) & 1)) {
       if (hipEventCreate(&a) == hipSuccess) (void)hipEventRecord(a, h->stream);
       else a = nullptr;
     }

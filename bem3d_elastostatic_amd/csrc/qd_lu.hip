@@ -388,6 +388,7 @@ __global__ __launch_bounds__(256) void k_gemm_sub(long long M, long long N, int 
 // 4 x 4 MFMA tiles), K steps of 16 through double-buffered LDS with register
 // prefetch, two workgroups per CU, XCD-banded tile order.
 constexpr int D_BM = 128, D_BN = 128, D_BK = 16;
+template <int kRole>
 __global__ __launch_bounds__(256, 2) void k_gemm64(long long M, long long N, int K, const double* __restrict__ A,
                                                    long long lda, const double* __restrict__ B, long long ldb,
                                                    double* __restrict__ C, long long ldc) {
@@ -520,6 +521,7 @@ __global__ __launch_bounds__(256) void k_cvt_b16t(int K, long long N, int Kp, co
 // Workgroups are remapped so that each XCD receives a contiguous band of
 // tiles (its L2 then serves the shared A/B rows).
 constexpr int H_BM = 256, H_BN = 128, H_BK = 32, H_PAD = 8;
+template <int kRole>
 __global__ __launch_bounds__(256, 2) void k_gemm16(long long M, long long N, int Kp, const _Float16* __restrict__ A16,
                                                 const _Float16* __restrict__ B16, const float* __restrict__ colscale,
                                                 float* __restrict__ C, long long ldc) {
@@ -818,7 +820,8 @@ struct LuCtx {
 };
 
 template <typename T>
-static int gemm_sub(LuCtx<T>& c, long long M, long long Nn, int K, const T* A, const T* B, T* C, bool allow_f16) {
+static int gemm_sub(LuCtx<T>& c, long long M, long long Nn, int K, const T* A, const T* B, T* C, bool allow_f16,
+                    bool trailing = false) {
   if (M <= 0 || Nn <= 0 || K <= 0) return QD_OK;
   dim3 grid((unsigned)((Nn + GBN - 1) / GBN), (unsigned)((M + GBM - 1) / GBM));
   if constexpr (std::is_same<T, float>::value) {
@@ -835,18 +838,28 @@ static int gemm_sub(LuCtx<T>& c, long long M, long long Nn, int K, const T* A, c
       hipLaunchKernelGGL(k_cvt_a16, dim3((unsigned)((M * (Kp / 8) + 255) / 256)), dim3(256), 0, c.h->stream, M, K,
                          Kp, A, c.ld, c.s->h16a.p);
       dim3 g16((unsigned)((Nn + H_BN - 1) / H_BN), (unsigned)((M + H_BM - 1) / H_BM));
-      hipLaunchKernelGGL(k_gemm16, g16, dim3(256), 0, c.h->stream, M, Nn, Kp, c.s->h16a.p, c.s->h16b.p,
-                         c.s->colscale.p, C, c.ld);
+      ProfScope ps(c.h, trailing ? QD_PROF_LU_TRAILING_GEMM : -1, 2.0 * (double)M * (double)Nn * K);
+      if (trailing)
+        hipLaunchKernelGGL(k_gemm16<1>, g16, dim3(256), 0, c.h->stream, M, Nn, Kp, c.s->h16a.p, c.s->h16b.p,
+                           c.s->colscale.p, C, c.ld);
+      else
+        hipLaunchKernelGGL(k_gemm16<0>, g16, dim3(256), 0, c.h->stream, M, Nn, Kp, c.s->h16a.p, c.s->h16b.p,
+                           c.s->colscale.p, C, c.ld);
       QD_HIP(c.h, hipGetLastError());
       return QD_OK;
     }
   }
   if constexpr (std::is_same<T, double>::value) {
     dim3 g64((unsigned)((Nn + D_BN - 1) / D_BN), (unsigned)((M + D_BM - 1) / D_BM));
-    hipLaunchKernelGGL(k_gemm64, g64, dim3(256), 0, c.h->stream, M, Nn, K, A, c.ld, B, c.ld, C, c.ld);
+    ProfScope ps(c.h, trailing ? QD_PROF_LU_TRAILING_GEMM : -1, 2.0 * (double)M * (double)Nn * K);
+    if (trailing)
+      hipLaunchKernelGGL(k_gemm64<1>, g64, dim3(256), 0, c.h->stream, M, Nn, K, A, c.ld, B, c.ld, C, c.ld);
+    else
+      hipLaunchKernelGGL(k_gemm64<0>, g64, dim3(256), 0, c.h->stream, M, Nn, K, A, c.ld, B, c.ld, C, c.ld);
     QD_HIP(c.h, hipGetLastError());
     return QD_OK;
   }
+  ProfScope ps(c.h, trailing ? QD_PROF_LU_TRAILING_GEMM : -1, 2.0 * (double)M * (double)Nn * K);
   hipLaunchKernelGGL(k_gemm_sub<T>, grid, dim3(256), 0, c.h->stream, M, Nn, K, A, c.ld, B, c.ld, C, c.ld);
   QD_HIP(c.h, hipGetLastError());
   return QD_OK;
@@ -970,10 +983,8 @@ static int factor(LuCtx<T>& c, int NB) {
                          c.ld, j + jb, N, (int)j, (int)(j + jb), s->ipiv_dev.p);
       QD_HIP(c.h, hipGetLastError());
       if ((rc = trsm_lower(c, j, jb, j + jb, N))) return rc;
-      const double mm = (double)(N - j - jb);
-      ProfScope ps(c.h, QD_PROF_LU_TRAILING_GEMM, 2.0 * mm * mm * jb);
       if ((rc = gemm_sub(c, N - j - jb, N - j - jb, jb, c.W + (j + jb) * c.ld + j, c.W + j * c.ld + j + jb,
-                         c.W + (j + jb) * c.ld + j + jb, true)))
+                         c.W + (j + jb) * c.ld + j + jb, true, true)))
         return rc;
     }
   }

@@ -121,7 +121,7 @@ def cpu_baseline(p, FI, DFI, budget_s=25.0):
     t = o.geometry(p)
     rng = np.random.default_rng(0)
     # assembly: sampled collocation nodes, all elements
-    k = 48
+    k = min(p.n, 1024)
     nodes = np.sort(rng.choice(p.n, k, replace=False))
     t0 = time.perf_counter()
     o.ghmat8_rows(p, nodes, t)
@@ -129,7 +129,7 @@ def cpu_baseline(p, FI, DFI, budget_s=25.0):
     per_node = ta / k
     t_asm = per_node * p.n
     # LU: numpy (LAPACK gesv) at a size that runs in seconds, scaled by N^3
-    ns = 5000
+    ns = 8000
     A = rng.normal(size=(ns, ns))
     bb = rng.normal(size=ns)
     t0 = time.perf_counter()
@@ -138,7 +138,7 @@ def cpu_baseline(p, FI, DFI, budget_s=25.0):
     N = 3 * p.n
     t_lu = tl * (N / ns) ** 3
     # Somigliana: two sample sizes, slope per point (the near table is built per call)
-    s1, s2 = 16, 48
+    s1, s2 = 128, 1024
     pts = p.points
     t0 = time.perf_counter()
     o.internal_points(p, t, FI, DFI, points=pts[:s1])
