@@ -168,13 +168,26 @@ using namespace qd;
 
 extern "C" {
 
+// Message of the last qd_create failure on this thread (qd_last_error(NULL)).
+static thread_local std::string g_create_err = "no handle";
+
+static int create_failed(int code, const std::string& msg) {
+  g_create_err = msg;
+  return code;
+}
+
 int qd_create(qd_handle* out, int device) {
-  if (!out) return QD_ERR_ARG;
+  if (!out) return create_failed(QD_ERR_ARG, "qd_create: out is NULL");
   *out = nullptr;
   int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return QD_ERR_HIP;
-  if (device < 0 || device >= ndev) return QD_ERR_ARG;
-  if (hipSetDevice(device) != hipSuccess) return QD_ERR_HIP;
+  const hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev <= 0)
+    return create_failed(QD_ERR_HIP, std::string("qd_create: no HIP device (") +
+                                         (e != hipSuccess ? hipGetErrorString(e) : "0 devices") + ")");
+  if (device < 0 || device >= ndev)
+    return create_failed(QD_ERR_ARG, "qd_create: device " + std::to_string(device) + " out of range [0, " +
+                                         std::to_string(ndev) + ")");
+  if (hipSetDevice(device) != hipSuccess) return create_failed(QD_ERR_HIP, "qd_create: hipSetDevice failed");
   qd_context* h = new qd_context();
   h->device = device;
   std::vector<SubTri> st;
@@ -183,7 +196,7 @@ int qd_create(qd_handle* out, int device) {
       hipMemcpy(h->subtri.p, st.data(), st.size() * sizeof(SubTri), hipMemcpyHostToDevice) !=
           hipSuccess) {
     delete h;
-    return QD_ERR_HIP;
+    return create_failed(QD_ERR_HIP, "qd_create: device allocation failed");
   }
   *out = h;
   return QD_OK;
@@ -197,7 +210,7 @@ void qd_destroy(qd_handle h) {
   delete h;
 }
 
-const char* qd_last_error(qd_handle h) { return h ? h->err.c_str() : "null handle"; }
+const char* qd_last_error(qd_handle h) { return h ? h->err.c_str() : g_create_err.c_str(); }
 
 int qd_set_stream(qd_handle h, void* s) {
   if (!h) return QD_ERR_ARG;

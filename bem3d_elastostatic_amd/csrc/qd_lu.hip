@@ -628,74 +628,55 @@ __global__ __launch_bounds__(256, 2) void k_gemm16(long long M, long long N, int
 // ---------------------------------------------------------------------------
 // Tournament pivoting for a leaf of <= 32 columns (the factorisation that
 // only preconditions the FP64 refinement).  k_tslu_select runs Gaussian
-// elimination with partial pivoting on up to 1024 rows per workgroup (4 per
-// thread, held in registers) and emits the 32 winning row numbers in pivot
-// order; applied first to the leaf's rows and then, level by level, to the
-// winners of groups of 32 workgroups, until one workgroup's winners remain.
-// The rows themselves are read from W, which is not modified before the
-// winners are known, so every level works on the original values.
-constexpr int kTsW = 32, kTsRPT = 4, kTsRows = 256 * kTsRPT;
+// elimination with partial pivoting on up to 256 rows per workgroup of one
+// wave (4 rows per lane, held in registers) and emits the 32 winning row
+// numbers in pivot order; applied first to the leaf's rows and then, level by
+// level, to the winners of groups of 8 workgroups, until one workgroup's
+// winners remain.  The rows themselves are read from W, which is not modified
+// before the winners are known, so every level works on the original values.
+// The pivot search reduces one 64-bit key per lane, (bits of |a| << 32) |
+// ~row: the largest |a| wins and, among equal values, the lowest row.
+constexpr int kTsW = 32, kTsRPT = 4, kTsRows = 64 * kTsRPT;
 
-__device__ __forceinline__ void argmax_pair(double& v, int& i, double ov, int oi) {
-  if (ov > v || (ov == v && oi < i)) {
-    v = ov;
-    i = oi;
-  }
+__device__ __forceinline__ unsigned long long ts_key(float v, int row) {
+  return ((unsigned long long)__float_as_uint(fabsf(v)) << 32) | (unsigned)(~row);
 }
 
-struct TsShared {
-  float srow[kTsW];
-  double wv[4];
-  int wi[4];
-};
-
-// One elimination step (column K) of the block's rows; K is a compile-time
-// constant so the register array is indexed statically.
+// One elimination step (column K); K is a compile-time constant so the
+// register arrays are indexed statically.
 template <int K>
 __device__ __forceinline__ void ts_step(float (&a)[kTsRPT][kTsW], int (&gidx)[kTsRPT], bool& done, int w,
-                                        TsShared& sh, int* __restrict__ out_idx) {
-  const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+                                        float* __restrict__ srow, int* __restrict__ out_idx) {
+  const int lane = threadIdx.x;
   done = done || (K >= w);
-  double best = -1.0;
-  int bi = 0x7fffffff;
+  unsigned long long key = 0;
 #pragma unroll
   for (int q = 0; q < kTsRPT; ++q)
-    if (!done && gidx[q] >= 0) argmax_pair(best, bi, fabs((double)a[q][K]), gidx[q]);
+    if (!done && gidx[q] >= 0) key = max(key, ts_key(a[q][K], gidx[q]));
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const double ov = __shfl_xor(best, off, 64);
-    const int oi = __shfl_xor(bi, off, 64);
-    argmax_pair(best, bi, ov, oi);
-  }
-  if (lane == 0) {
-    sh.wv[wave] = best;
-    sh.wi[wave] = bi;
-  }
-  __syncthreads();
-  double bb = sh.wv[0];
-  int ib = sh.wi[0];
-#pragma unroll
-  for (int x = 1; x < 4; ++x) argmax_pair(bb, ib, sh.wv[x], sh.wi[x]);
-  if (!done && bb >= 0.0) {
+  for (int off = 32; off > 0; off >>= 1) key = max(key, (unsigned long long)__shfl_xor(key, off, 64));
+  const bool found = !done && key != 0;
+  const int ib = (int)(~(unsigned)(key & 0xffffffffull));
+  if (found) {
 #pragma unroll
     for (int q = 0; q < kTsRPT; ++q)
-      if (gidx[q] >= 0 && gidx[q] == ib) {
+      if (gidx[q] == ib) {
 #pragma unroll
-        for (int j = 0; j < kTsW; ++j) sh.srow[j] = a[q][j];
+        for (int j = 0; j < kTsW; ++j) srow[j] = a[q][j];
         gidx[q] = -1;
       }
   }
-  if (tid == 0) out_idx[(long long)blockIdx.x * kTsW + K] = (!done && bb >= 0.0) ? ib : -1;
-  done = done || (bb < 0.0);
+  if (lane == 0) out_idx[(long long)blockIdx.x * kTsW + K] = found ? ib : -1;
+  done = !found;
   __syncthreads();
-  if (!done) {
-    const float piv = sh.srow[K];
+  if (found) {
+    const float piv = srow[K];
     const float rp = (piv != 0.0f) ? 1.0f / piv : 0.0f;
 #pragma unroll
     for (int q = 0; q < kTsRPT; ++q) {
       const float l = (gidx[q] >= 0) ? a[q][K] * rp : 0.0f;
 #pragma unroll
-      for (int j = K + 1; j < kTsW; ++j) a[q][j] -= l * sh.srow[j];
+      for (int j = K + 1; j < kTsW; ++j) a[q][j] -= l * srow[j];
     }
   }
   __syncthreads();
@@ -703,30 +684,50 @@ __device__ __forceinline__ void ts_step(float (&a)[kTsRPT][kTsW], int (&gidx)[kT
 
 template <int... Ks>
 __device__ __forceinline__ void ts_all(std::integer_sequence<int, Ks...>, float (&a)[kTsRPT][kTsW],
-                                       int (&gidx)[kTsRPT], bool& done, int w, TsShared& sh,
+                                       int (&gidx)[kTsRPT], bool& done, int w, float* __restrict__ srow,
                                        int* __restrict__ out_idx) {
-  (ts_step<Ks>(a, gidx, done, w, sh, out_idx), ...);
+  (ts_step<Ks>(a, gidx, done, w, srow, out_idx), ...);
 }
 
-__global__ __launch_bounds__(256) void k_tslu_select(const float* __restrict__ W, long long ld, long long N,
-                                                     long long c0, int w, const int* __restrict__ in_idx, int n_in,
-                                                     int* __restrict__ out_idx) {
-  __shared__ TsShared sh;
-  const int tid = threadIdx.x;
+// Loads the 32-column segment [c0, c0+32) of row g (zero beyond column w).
+// c0 is a multiple of 32 and rows are ld >= roundup(N, 64) floats long, so the
+// segment is 128-byte aligned and inside the row.
+__device__ __forceinline__ void ts_load_row(const float* __restrict__ W, long long ld, long long g, long long c0,
+                                            int w, float (&r)[kTsW]) {
+  const float4* src = reinterpret_cast<const float4*>(W + g * ld + c0);
+#pragma unroll
+  for (int j4 = 0; j4 < kTsW / 4; ++j4) {
+    const float4 v = src[j4];
+    r[4 * j4 + 0] = (4 * j4 + 0 < w) ? v.x : 0.0f;
+    r[4 * j4 + 1] = (4 * j4 + 1 < w) ? v.y : 0.0f;
+    r[4 * j4 + 2] = (4 * j4 + 2 < w) ? v.z : 0.0f;
+    r[4 * j4 + 3] = (4 * j4 + 3 < w) ? v.w : 0.0f;
+  }
+}
+
+__global__ __launch_bounds__(64) void k_tslu_select(const float* __restrict__ W, long long ld, long long N,
+                                                    long long c0, int w, const int* __restrict__ in_idx, int n_in,
+                                                    int* __restrict__ out_idx) {
+  __shared__ float srow[kTsW];
+  const int lane = threadIdx.x;
   float a[kTsRPT][kTsW];
   int gidx[kTsRPT];
 #pragma unroll
   for (int q = 0; q < kTsRPT; ++q) {
-    const long long li = (long long)blockIdx.x * kTsRows + tid + 256 * q;
+    const long long li = (long long)blockIdx.x * kTsRows + lane + 64 * q;
     long long g;
     if (in_idx) g = (li < n_in) ? in_idx[li] : -1;
     else g = (c0 + li < N) ? c0 + li : -1;
     gidx[q] = (int)g;
+    if (g >= 0) {
+      ts_load_row(W, ld, g, c0, w, a[q]);
+    } else {
 #pragma unroll
-    for (int j = 0; j < kTsW; ++j) a[q][j] = (g >= 0 && j < w) ? W[g * ld + c0 + j] : 0.0f;
+      for (int j = 0; j < kTsW; ++j) a[q][j] = 0.0f;
+    }
   }
   bool done = false;
-  ts_all(std::make_integer_sequence<int, kTsW>{}, a, gidx, done, w, sh, out_idx);
+  ts_all(std::make_integer_sequence<int, kTsW>{}, a, gidx, done, w, srow, out_idx);
 }
 
 // LAPACK interchange sequence that brings the winners p_0..p_{w-1} to rows
@@ -768,43 +769,95 @@ __global__ void k_tslu_ipiv(const int* __restrict__ win, int w, long long c0, in
   }
 }
 
-// After the interchanges: unpivoted LU of the top w x w block (the winners
-// in pivot order) and L21 = A21 U11^-1 for the rows below.  32 lanes per
-// row; 8 rows per pass.
+// After the interchanges: k_tslu_top factors the top w x w block (the winners
+// in pivot order, no further pivoting) in place with one wave: lane r holds
+// row r and the pivot row of step k goes through LDS.  k_tslu_finish then
+// forms L21 = A21 U11^-1, one row per thread, against U11 read from LDS by
+// broadcast.  The top block is written only by k_tslu_top, so no workgroup of
+// k_tslu_finish can see it half-factored.
+template <int K>
+__device__ __forceinline__ void tf_step(float (&u)[kTsW], int w, float* __restrict__ prow) {
+  const int r = threadIdx.x;
+  if (K < w) {
+    if (r == K) {
+#pragma unroll
+      for (int j = 0; j < kTsW; ++j) prow[j] = u[j];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (r > K && r < w) {
+      const float piv = prow[K];
+      const float l = (piv != 0.0f) ? u[K] / piv : 0.0f;
+      u[K] = l;
+#pragma unroll
+      for (int j = K + 1; j < kTsW; ++j) u[j] -= l * prow[j];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
+template <int... Ks>
+__device__ __forceinline__ void tf_all(std::integer_sequence<int, Ks...>, float (&u)[kTsW], int w,
+                                       float* __restrict__ prow) {
+  (tf_step<Ks>(u, w, prow), ...);
+}
+
+template <int K>
+__device__ __forceinline__ void tf_solve_step(float (&x)[kTsW], int w, const float (*__restrict__ sU)[kTsW],
+                                              const float* __restrict__ sR) {
+  if (K < w) {
+    x[K] *= sR[K];
+#pragma unroll
+    for (int j = K + 1; j < kTsW; ++j) x[j] -= x[K] * sU[K][j];
+  }
+}
+
+template <int... Ks>
+__device__ __forceinline__ void tf_solve(std::integer_sequence<int, Ks...>, float (&x)[kTsW], int w,
+                                         const float (*__restrict__ sU)[kTsW], const float* __restrict__ sR) {
+  (tf_solve_step<Ks>(x, w, sU, sR), ...);
+}
+
+__global__ __launch_bounds__(64) void k_tslu_top(float* __restrict__ W, long long ld, long long c0, int w) {
+  __shared__ float prow[kTsW];
+  const int tid = threadIdx.x;
+  float u[kTsW];
+  if (tid < w) {
+    ts_load_row(W, ld, c0 + tid, c0, w, u);
+  } else {
+#pragma unroll
+    for (int j = 0; j < kTsW; ++j) u[j] = 0.0f;
+  }
+  tf_all(std::make_integer_sequence<int, kTsW>{}, u, w, prow);
+  if (tid < w)
+    for (int j = 0; j < w; ++j) W[(c0 + tid) * ld + c0 + j] = u[j];
+}
+
 __global__ __launch_bounds__(256) void k_tslu_finish(float* __restrict__ W, long long ld, long long N, long long c0,
                                                      int w) {
-  __shared__ float sU[kTsW][kTsW + 1];
-  const int tid = threadIdx.x, t = tid & 31, slot = tid >> 5;
+  __shared__ float sU[kTsW][kTsW];
+  __shared__ float sR[kTsW];
+  const int tid = threadIdx.x;
   for (int i = tid; i < kTsW * kTsW; i += 256) {
-    const int r = i / kTsW, cc = i % kTsW;
-    sU[r][cc] = (r < w && cc < w) ? W[(c0 + r) * ld + c0 + cc] : 0.0f;
+    const int r = i / kTsW, j = i % kTsW;
+    sU[r][j] = (r < w && j < w && j >= r) ? W[(c0 + r) * ld + c0 + j] : 0.0f;
   }
   __syncthreads();
-  // LU of the top block, all threads: multipliers of column k, then the
-  // rank-1 update of the trailing (w-k-1)^2 entries
-  for (int k = 0; k < w; ++k) {
-    if (tid > k && tid < w) sU[tid][k] = sU[tid][k] / sU[k][k];
-    __syncthreads();
-    for (int i = tid; i < kTsW * kTsW; i += 256) {
-      const int r = i / kTsW, cc = i % kTsW;
-      if (r > k && r < w && cc > k && cc < w) sU[r][cc] -= sU[r][k] * sU[k][cc];
-    }
-    __syncthreads();
+  if (tid < kTsW) {
+    const float d = sU[tid][tid];
+    sR[tid] = (tid < w && d != 0.0f) ? 1.0f / d : 0.0f;
   }
-  if (blockIdx.x == 0)
-    for (int i = tid; i < w * w; i += 256) W[(c0 + i / w) * ld + c0 + i % w] = sU[i / w][i % w];
-  const long long rb = c0 + w + (long long)blockIdx.x * 64;
-  for (int q = 0; q < 8; ++q) {
-    const long long r = rb + q * 8 + slot;
-    if (r >= N) break;
-    float x = (t < w) ? W[r * ld + c0 + t] : 0.0f;
-    for (int k = 0; k < w; ++k) {
-      const float xk = __shfl(x, k, 32) / sU[k][k];
-      if (t == k) x = xk;
-      if (t > k) x -= xk * sU[k][t];
-    }
-    if (t < w) W[r * ld + c0 + t] = x;
-  }
+  __syncthreads();
+  const long long r = c0 + w + (long long)blockIdx.x * 256 + tid;
+  if (r >= N) return;
+  float x[kTsW];
+  ts_load_row(W, ld, r, c0, w, x);
+  tf_solve(std::make_integer_sequence<int, kTsW>{}, x, w, sU, sR);
+  float* dst = W + r * ld + c0;
+  for (int j = 0; j < w; ++j) dst[j] = x[j];
 }
 
 // ---------------------------------------------------------------------------
@@ -909,12 +962,12 @@ static int tslu_leaf(LuCtx<float>& c, long long c0, int w, long long p0, long lo
   QD_ALLOC(c.h, s->ts_b, (size_t)nb * kTsW);
   int* in = s->ts_a.p;
   int* out = s->ts_b.p;
-  hipLaunchKernelGGL(k_tslu_select, dim3(nb), dim3(256), 0, c.h->stream, c.W, c.ld, c.N, c0, w, (const int*)nullptr,
+  hipLaunchKernelGGL(k_tslu_select, dim3(nb), dim3(64), 0, c.h->stream, c.W, c.ld, c.N, c0, w, (const int*)nullptr,
                      0, in);
   int n_in = nb * kTsW;
   while (nb > 1) {
     const int nb2 = (n_in + kTsRows - 1) / kTsRows;
-    hipLaunchKernelGGL(k_tslu_select, dim3(nb2), dim3(256), 0, c.h->stream, c.W, c.ld, c.N, c0, w, (const int*)in,
+    hipLaunchKernelGGL(k_tslu_select, dim3(nb2), dim3(64), 0, c.h->stream, c.W, c.ld, c.N, c0, w, (const int*)in,
                        n_in, out);
     n_in = nb2 * kTsW;
     std::swap(in, out);
@@ -923,9 +976,10 @@ static int tslu_leaf(LuCtx<float>& c, long long c0, int w, long long p0, long lo
   hipLaunchKernelGGL(k_tslu_ipiv, dim3(1), dim3(1), 0, c.h->stream, (const int*)in, w, c0, s->ipiv_dev.p);
   hipLaunchKernelGGL(k_laswp<float>, dim3((unsigned)((p1 - p0 + 255) / 256)), dim3(256), 0, c.h->stream, c.W, c.ld,
                      p0, p1, (int)c0, (int)(c0 + w), s->ipiv_dev.p);
+  hipLaunchKernelGGL(k_tslu_top, dim3(1), dim3(64), 0, c.h->stream, c.W, c.ld, c0, w);
   const long long below = std::max<long long>(m - w, 1);
-  hipLaunchKernelGGL(k_tslu_finish, dim3((unsigned)((below + 63) / 64)), dim3(256), 0, c.h->stream, c.W, c.ld, c.N,
-                     c0, w);
+  hipLaunchKernelGGL(k_tslu_finish, dim3((unsigned)((below + 255) / 256)), dim3(256), 0, c.h->stream, c.W, c.ld,
+                     c.N, c0, w);
   QD_HIP(c.h, hipGetLastError());
   return QD_OK;
 }
@@ -1369,7 +1423,7 @@ static int ensure_factors(qd_context* h, int precision, qd_solve_report* rep) {
   int rc;
   // Panel width: the FP16 trailing update streams its FP32 C once per panel,
   // so it takes a wider panel than the MFMA f64/f32 updates need.
-  int NB = precision == QD_FP16_IR ? 1024 : 512;
+  int NB = precision == QD_FP16_IR ? 2048 : 512;
   if (const char* e = std::getenv("QUADESTH_NB")) NB = std::max(64, std::atoi(e) / 32 * 32);
   if (precision == QD_FP64) {
     s->W32.release();

@@ -43,7 +43,8 @@ class Quadesth:
         h = C.c_void_p()
         rc = self.lib.qd_create(C.byref(h), device)
         if rc != 0:
-            raise _lib.QuadesthError(rc, f"qd_create(device={device}) failed (no GPU visible?)")
+            msg = self.lib.qd_last_error(None)
+            raise _lib.QuadesthError(rc, msg.decode() if msg else f"qd_create(device={device}) failed")
         self.h = h
         self.p = problem
         self.n, self.ne = problem.n, problem.ne

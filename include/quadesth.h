@@ -70,6 +70,8 @@ typedef struct {
 /* ---- lifecycle ----------------------------------------------------------- */
 int qd_create(qd_handle* out, int device);
 void qd_destroy(qd_handle h);
+/* Message of the last failed call on h; with h == NULL, of the last failed
+ * qd_create on the calling thread. */
 const char* qd_last_error(qd_handle h);
 /* Work is queued on this HIP stream (default: the null stream). */
 int qd_set_stream(qd_handle h, void* hip_stream);
