@@ -65,6 +65,11 @@ _SIGNATURES = {
     "qd_internal_points": (_i, [_vp, _i, _pd, _pd, _pd, _pd, _pd, _i, _pd, _pd, _pd, _pd, _pll]),
     "qd_internal_points_device": (_i, [_vp, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _pll]),
     "qd_boundary_stresses": (_i, [_vp, _pd, _pd, _pd, _pd, _pd, _pd]),
+    "qd_set_coordinates": (_i, [_vp, _pd, _pd, _pd]),
+    "qd_solve_transpose": (_i, [_vp, _i, _i, _pd, _i, _d, _i, C.POINTER(SolveReport)]),
+    "qd_ghmat8_vjp": (_i, [_vp, _pd, _pd, _pd, _pd, _pd, _pd, _pd]),
+    "qd_internal_points_vjp": (_i, [_vp, _i, _pd, _pd, _pd, _pd, _pd, _i, _pd, _pd, _pd, _pd, _pd, _pd, _pd,
+                                    _pd]),
     "qd_profile_enable": (_i, [_vp, C.c_uint]),
     "qd_profile_read": (_i, [_vp, _i, _pll, _pd, _pd]),
     "dgesv_": (None, [_pi, _pi, _pd, _pi, _pi, _pd, _pi, _pi]),

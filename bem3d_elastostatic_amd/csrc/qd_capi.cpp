@@ -307,6 +307,24 @@ int qd_set_mesh(qd_handle h, int n, int ne, const double* x, const double* y, co
   return QD_OK;
 }
 
+int qd_set_coordinates(qd_handle h, const double* x, const double* y, const double* z) {
+  if (!h || !x || !y || !z) return QD_ERR_ARG;
+  if (!h->have_mesh) return set_error(h, QD_ERR_STATE, "qd_set_coordinates before qd_set_mesh");
+  QD_HIP(h, hipSetDevice(h->device));
+  const int n = h->n;
+  h->hx.assign(x, x + n);
+  h->hy.assign(y, y + n);
+  h->hz.assign(z, z + n);
+  int rc;
+  if ((rc = upload_d(h, h->x, x, n))) return rc;
+  if ((rc = upload_d(h, h->y, y, n))) return rc;
+  if ((rc = upload_d(h, h->z, z, n))) return rc;
+  h->have_geom = false;
+  h->have_sys = false;
+  lu_invalidate(h);
+  return QD_OK;
+}
+
 int qd_set_material(qd_handle h, double mu, double nu) {
   if (!h) return QD_ERR_ARG;
   if (!(mu > 0.0) || !(nu < 1.0) || !(nu > -1.0))
@@ -667,6 +685,63 @@ int qd_internal_points(qd_handle h, int l, const double* cx, const double* cy, c
   if (prin) QD_HIP(h, hipMemcpy(prin, o + 9 * L, 3 * L * sizeof(double), hipMemcpyDeviceToHost));
   if (vm) QD_HIP(h, hipMemcpy(vm, o + 12 * L, L * sizeof(double), hipMemcpyDeviceToHost));
   return QD_OK;
+}
+
+int qd_solve_transpose(qd_handle h, int precision, int nrhs, double* c, int ldc, double tol, int max_iter,
+                       qd_solve_report* rep) {
+  if (!h) return QD_ERR_ARG;
+  if (!h->have_sys) return set_error(h, QD_ERR_STATE, "qd_solve_transpose before qd_ghmat8");
+  const long long N = 3LL * h->n;
+  if (!c || nrhs < 1 || ldc < N) return set_error(h, QD_ERR_ARG, "qd_solve_transpose: nrhs=%d ldc=%d", nrhs, ldc);
+  QD_HIP(h, hipSetDevice(h->device));
+  DevBuf<double> d;
+  QD_ALLOC(h, d, (size_t)N * nrhs);
+  QD_HIP(h, hipMemcpy2D(d.p, N * sizeof(double), c, (size_t)ldc * sizeof(double), N * sizeof(double), nrhs,
+                        hipMemcpyHostToDevice));
+  int rc = qd_solve_transpose_device(h, precision, nrhs, d.p, N, tol, max_iter, rep);
+  if (rc < 0) return rc;
+  QD_HIP(h, hipStreamSynchronize(h->stream));
+  QD_HIP(h, hipMemcpy2D(c, (size_t)ldc * sizeof(double), d.p, N * sizeof(double), N * sizeof(double), nrhs,
+                        hipMemcpyDeviceToHost));
+  return rc;
+}
+
+int qd_ghmat8_vjp(qd_handle h, const double* lambda, const double* fi, const double* dfi, double* gx, double* gy,
+                  double* gz, double* gnu) {
+  if (!h || !lambda || !fi || !dfi || !gx || !gy || !gz || !gnu) return QD_ERR_ARG;
+  if (!h->have_mesh || !h->have_mat || !h->have_bc)
+    return set_error(h, QD_ERR_STATE, "qd_ghmat8_vjp needs mesh, material and boundary conditions");
+  QD_HIP(h, hipSetDevice(h->device));
+  return ghmat8_vjp_run(h, lambda, fi, dfi, gx, gy, gz, gnu);
+}
+
+int qd_internal_points_vjp(qd_handle h, int l, const double* cx, const double* cy, const double* cz,
+                           const double* fi, const double* dfi, int variant, const double* g_u,
+                           const double* g_sigma, double* g_fi, double* g_dfi, double* gx, double* gy, double* gz,
+                           double* gnu) {
+  if (!h || !fi || !dfi || !g_fi || !g_dfi || !gx || !gy || !gz || !gnu) return QD_ERR_ARG;
+  if (l < 0 || (l > 0 && (!cx || !cy || !cz)) || (variant != 0 && variant != 1))
+    return set_error(h, QD_ERR_ARG, "qd_internal_points_vjp: l=%d variant=%d", l, variant);
+  if (!h->have_mesh || !h->have_mat)
+    return set_error(h, QD_ERR_STATE, "qd_internal_points_vjp before mesh/material");
+  QD_HIP(h, hipSetDevice(h->device));
+  const size_t n = (size_t)h->n, ne = (size_t)h->ne;
+  if (l == 0) {
+    std::memset(g_fi, 0, 3 * n * sizeof(double));
+    std::memset(g_dfi, 0, 27 * ne * sizeof(double));
+    std::memset(gx, 0, n * sizeof(double));
+    std::memset(gy, 0, n * sizeof(double));
+    std::memset(gz, 0, n * sizeof(double));
+    *gnu = 0.0;
+    return QD_OK;
+  }
+  std::vector<double> pts(3 * (size_t)l);
+  for (int i = 0; i < l; ++i) {
+    pts[3 * (size_t)i] = cx[i];
+    pts[3 * (size_t)i + 1] = cy[i];
+    pts[3 * (size_t)i + 2] = cz[i];
+  }
+  return somigliana_vjp_run(h, l, pts.data(), fi, dfi, variant, g_u, g_sigma, g_fi, g_dfi, gx, gy, gz, gnu);
 }
 
 int qd_profile_enable(qd_handle h, unsigned mask) {

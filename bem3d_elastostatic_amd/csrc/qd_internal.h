@@ -190,6 +190,16 @@ int assembly_run(qd_context* h, const double* bc_dev_override, double* fi_out_de
 int somigliana_run(qd_context* h, int l, const double* pts_dev, const double* fi_dev,
                    const double* dfi_dev, int variant, double* u_dev, double* sig_dev,
                    double* prin_dev, double* vm_dev, long long* counts);
+// Element centroids and near thresholds (stress_mod.f90:584-604) into h->el_centre.
+int somigliana_centres(qd_context* h);
+// 1-D Gauss-Legendre rule of stress_mod.f90:475-503 (host).
+void gauss_legendre(int npts, double* xg, double* wg);
+// Reverse-mode derivatives (qd_grad.hip).
+int ghmat8_vjp_run(qd_context* h, const double* lambda, const double* fi, const double* dfi, double* gx,
+                   double* gy, double* gz, double* gnu);
+int somigliana_vjp_run(qd_context* h, int l, const double* pts, const double* fi, const double* dfi,
+                       int variant, const double* gu, const double* gs, double* gfi, double* gdfi, double* gx,
+                       double* gy, double* gz, double* gnu);
 int boundary_stress_run(qd_context* h, const double* fi_dev, const double* dfi_dev,
                         double* se_dev, double* sn_dev, double* prin_dev, double* vm_dev);
 int lu_solve(qd_context* h, int precision, int nrhs, double* b_dev, long long ldb, double tol,

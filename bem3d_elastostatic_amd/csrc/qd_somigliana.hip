@@ -34,7 +34,7 @@ __constant__ double c_omei4[4] = {0.1739274226, 0.3260725774, 0.3260725774, 0.17
 __constant__ double c_omej4[4] = {0.1355069134, 0.2034645680, 0.1298475476, 0.0311809709};
 
 // gauss_legendre_1d (stress_mod.f90:475-503), evaluated on the host.
-static void gauss_legendre(int npts, double* xg, double* wg) {
+void gauss_legendre(int npts, double* xg, double* wg) {
   const double pi = 3.14159265358979323846;
   int m_half = (npts + 1) / 2;
   for (int i = 1; i <= m_half; ++i) {
@@ -352,6 +352,14 @@ __global__ void k_somi_final(int l, int nslice, const double* __restrict__ part,
   if (p_out)
     for (int d = 0; d < 3; ++d) p_out[3 * (size_t)ip + d] = pr[d];
   if (vm_out) vm_out[ip] = vm;
+}
+
+int somigliana_centres(qd_context* h) {
+  QD_ALLOC(h, h->el_centre, 4 * (size_t)h->ne);
+  hipLaunchKernelGGL(k_centres, dim3((h->ne + 127) / 128), dim3(128), 0, h->stream, h->ne, h->nconec.p,
+                     h->kconec0.p, h->x.p, h->y.p, h->z.p, h->el_centre.p);
+  QD_HIP(h, hipGetLastError());
+  return QD_OK;
 }
 
 int somigliana_run(qd_context* h, int l, const double* pts, const double* fi, const double* dfi,

@@ -269,25 +269,35 @@ def config3(npoints: int = 65536, h: float = 0.1) -> BemProblem:
     return p
 
 
-def config4(amplitude: float = 0.02, seed: int = 3, npoints: int = 4096) -> BemProblem:
-    """Config-1 cube with node coordinates perturbed by a seeded radial bump
-    field that keeps the supported faces (x=0, y=0, z=0) planar: the normal
-    coordinate of a node on such a face is left untouched."""
-    p = config1(npoints)
+def radial_bump(p: BemProblem, amplitude: float, seed: int) -> BemProblem:
+    """Moves the nodes of ``p`` (in place) by a seeded radial bump field:
+    displacement amplitude * exp(-|x - c|^2 / w^2) along x - c, with the
+    centre c and width w drawn from ``seed`` (c in [0.3, 0.7]^3 scaled to the
+    bounding box, w in [0.25, 0.45] times its size).  A node on one of the
+    bounding box's minimum planes (the supported faces of the configs) keeps
+    that coordinate, so those faces stay planar."""
     rng = np.random.default_rng(seed)
-    centre = rng.uniform(0.3, 0.7, 3)
-    width = rng.uniform(0.25, 0.45)
     xyz = p.coords()
+    lo, hi = xyz.min(axis=0), xyz.max(axis=0)
+    size = float(np.max(hi - lo))
+    centre = lo + rng.uniform(0.3, 0.7, 3) * (hi - lo)
+    width = rng.uniform(0.25, 0.45) * size
     d = xyz - centre[None, :]
     r2 = np.sum(d * d, axis=1)
     bump = amplitude * np.exp(-r2 / (width * width))
     rn = np.sqrt(np.maximum(r2, 1e-30))
     disp = (bump / rn)[:, None] * d
     for ax in range(3):
-        on_face = np.isclose(xyz[:, ax], 0.0)
-        disp[on_face, ax] = 0.0
+        disp[np.isclose(xyz[:, ax], lo[ax]), ax] = 0.0
     xyz = xyz + disp
     p.x, p.y, p.z = xyz[:, 0].copy(), xyz[:, 1].copy(), xyz[:, 2].copy()
+    return p
+
+
+def config4(amplitude: float = 0.02, seed: int = 3, npoints: int = 4096) -> BemProblem:
+    """Config-1 cube with node coordinates perturbed by a seeded radial bump
+    field that keeps the supported faces (x=0, y=0, z=0) planar."""
+    p = radial_bump(config1(npoints), amplitude, seed)
     p.title = "PERTURBED UNIT CUBE 16x16 QUAD9 (RADIAL BUMP 0.02)"
     return p
 

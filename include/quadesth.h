@@ -143,6 +143,33 @@ int qd_internal_points_device(qd_handle h, int l, const double* pts_dev, const d
 int qd_boundary_stresses(qd_handle h, const double* fi, const double* dfi, double* sigma_elem,
                          double* sigma_node, double* prin, double* vm);
 
+/* ---- derivatives (no reference counterpart: the reference computes no
+ * gradients).  Together with qd_solve_transpose(_device) they give the adjoint
+ * gradient of any objective of the internal-point results with respect to
+ * the node coordinates and Poisson's ratio. ------------------------------- */
+/* New node coordinates for the same connectivity, material and boundary
+ * codes (the x, y, z of qd_set_mesh); the geometry, system and factors are
+ * recomputed by the next calls. */
+int qd_set_coordinates(qd_handle h, const double* x, const double* y, const double* z);
+/* Host-buffer form of qd_solve_transpose_device: A^T y = c in place. */
+int qd_solve_transpose(qd_handle h, int precision, int nrhs, double* c, int ldc, double tol, int max_iter,
+                       qd_solve_report* rep);
+/* Derivatives of R = lambda^T (A X - FI) with X held fixed, A and FI being
+ * ghmat8's (assembly_mod.f90:53-223) at the current coordinates: gx, gy, gz
+ * (n each) = dR/d(x, y, z), *gnu = dR/dnu (mu fixed).  X enters through its
+ * reorganised form (qd_reorganize_solution): fi (3n) nodal displacements,
+ * dfi (27, ne) element-node tractions.  Outputs are overwritten. */
+int qd_ghmat8_vjp(qd_handle h, const double* lambda, const double* fi, const double* dfi, double* gx, double* gy,
+                  double* gz, double* gnu);
+/* Derivatives of sum_p g_u(:,p) . u_int(:,p) + g_sigma(:,p) . sigma_int(:,p)
+ * for compute_internal_points (stress_mod.f90:511-781) at the given fi, dfi:
+ * g_fi (3n), g_dfi (27, ne), gx, gy, gz (n), *gnu.  g_u (3, l) and
+ * g_sigma (6, l) may be NULL (zero).  Outputs are overwritten. */
+int qd_internal_points_vjp(qd_handle h, int l, const double* cx, const double* cy, const double* cz,
+                           const double* fi, const double* dfi, int variant, const double* g_u,
+                           const double* g_sigma, double* g_fi, double* g_dfi, double* gx, double* gy, double* gz,
+                           double* gnu);
+
 /* ---- instrumentation (no reference counterpart: the reference only has
  * cpu_time stopwatches, main.f90:27-71) ------------------------------------
  * With profiling on for a kernel class, every launch of that class is

@@ -98,12 +98,58 @@ def run_case(name, ref):
     print(name, "n", p.n, "ne", p.ne, "l", p.l, "written")
 
 
+# Directional derivatives of J = sum over the internal points of vm^2 (the
+# objective of the gradient configuration) by central differences of the
+# reference's own FP64 pipeline on perturbed meshes.
+GRAD_CASES = {
+    "grad_c0": lambda: mesh.radial_bump(mesh.config0(), 0.02, 3),
+    "grad_mixed": lambda: mesh.radial_bump(mixed_small(), 0.02, 5),
+}
+GRAD_DIRS = 6
+GRAD_H = 1e-6
+
+
+def _ref_J(ref, p, td):
+    path = os.path.join(td, "g.dat")
+    deck.write_deck(p, path)
+    r = ref.run(path)
+    return float(np.sum(r["vm"] ** 2))
+
+
+def run_grad_case(name, ref):
+    import dataclasses
+    p = GRAD_CASES[name]()
+    rng = np.random.default_rng(23)
+    D = rng.standard_normal((GRAD_DIRS, p.n, 3))
+    Dnu = rng.standard_normal(GRAD_DIRS) * 0.1
+    fd = np.zeros(GRAD_DIRS)
+    xyz = p.coords()
+    with tempfile.TemporaryDirectory(dir="/tmp") as td:
+        J0 = _ref_J(ref, p, td)
+        for i in range(GRAD_DIRS):
+            vals = []
+            for sgn in (1.0, -1.0):
+                c = xyz + sgn * GRAD_H * D[i]
+                q = dataclasses.replace(p, x=c[:, 0].copy(), y=c[:, 1].copy(), z=c[:, 2].copy(),
+                                        nu=p.nu + sgn * GRAD_H * Dnu[i])
+                vals.append(_ref_J(ref, q, td))
+            fd[i] = (vals[0] - vals[1]) / (2.0 * GRAD_H)
+    out = dict(x=p.x, y=p.y, z=p.z, nconec=p.nconec, kconec=p.kconec, kode=p.kode, bc=p.bc,
+               points=p.points, mu=np.float64(p.mu), nu=np.float64(p.nu), J=np.float64(J0),
+               directions=D, dnu=Dnu, h=np.float64(GRAD_H), fd=fd)
+    np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **out)
+    print(name, "n", p.n, "J", J0, "fd", fd)
+
+
 def main(argv):
     build_reference()
     ref = Reference()
-    names = argv or list(CASES)
+    names = argv or (list(CASES) + list(GRAD_CASES))
     for name in names:
-        run_case(name, ref)
+        if name in GRAD_CASES:
+            run_grad_case(name, ref)
+        else:
+            run_case(name, ref)
 
 
 if __name__ == "__main__":

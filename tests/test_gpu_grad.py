@@ -1,0 +1,150 @@
+"""Gradients with respect to node coordinates and nu (the gradient
+configuration: J = sum of internal von Mises stress squared).
+
+* qd_ghmat8_vjp against central differences of the oracle's own assembly:
+  R(theta) = lambda^T (A(theta) X - FI(theta)), random lambda and X.
+* qd_internal_points_vjp:
+  - coordinates and nu against central differences of the oracle's internal
+    points with FI, DFI fixed;
+  - FI and DFI through the adjoint identity <g, S(FI, DFI)> = <g_FI, FI> +
+    <g_DFI, DFI>, since S is linear in them.
+* The torch.autograd gradient of J against central differences (h = 1e-6)
+  of the GPU's FP64 forward and of the compiled reference (golden vectors).
+  The tolerance is 1e-6 relative.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from conftest import golden, problem_from_golden
+
+pytestmark = pytest.mark.gpu
+
+H = 1e-6
+
+
+def _shift(p, d, dnu):
+    c = p.coords() + d
+    return dataclasses.replace(p, x=c[:, 0].copy(), y=c[:, 1].copy(), z=c[:, 2].copy(), nu=p.nu + dnu)
+
+
+def _problem(name):
+    from bem3d_elastostatic_amd import mesh
+    g = golden(name)
+    return mesh.radial_bump(problem_from_golden(g), 0.02, 3)
+
+
+def _rel(a, b):
+    return abs(a - b) / max(abs(a), abs(b), 1e-300)
+
+
+@pytest.mark.parametrize("name", ["c0", "mixed_small"])
+def test_ghmat8_vjp_matches_oracle_fd(name, oracle):
+    from bem3d_elastostatic_amd.autograd import GradSession
+    p = _problem(name)
+    print(f"[grad] ghmat8_vjp {name} n={p.n}", flush=True)
+    rng = np.random.default_rng(11)
+    N = 3 * p.n
+    lam = rng.standard_normal(N)
+    X = rng.standard_normal(N)
+    FI, DFI = oracle.reorganize(p, X)
+    s = GradSession(p)
+    gc, gnu = s.ghmat8_vjp(lam, FI, DFI)
+    s.close()
+
+    def R(q):
+        A, F = oracle.ghmat8(q)
+        return float(lam @ (A @ X - F))
+
+    for trial in range(4):
+        d = rng.standard_normal((p.n, 3))
+        dnu = 0.1 * rng.standard_normal() if trial else 0.0
+        fd = (R(_shift(p, H * d, H * dnu)) - R(_shift(p, -H * d, -H * dnu))) / (2 * H)
+        an = float(np.sum(gc * d) + gnu * dnu)
+        assert _rel(an, fd) < 1e-6, (trial, an, fd)
+    # nu alone
+    fd = (R(_shift(p, 0.0, H)) - R(_shift(p, 0.0, -H))) / (2 * H)
+    assert _rel(gnu, fd) < 1e-6, (gnu, fd)
+
+
+@pytest.mark.parametrize("name,variant", [("c0", 0), ("mixed_small", 0), ("mixed_small", 1)])
+def test_internal_points_vjp(name, variant, oracle):
+    from bem3d_elastostatic_amd.autograd import GradSession
+    p = _problem(name)
+    print(f"[grad] internal_points_vjp {name} variant {variant}", flush=True)
+    rng = np.random.default_rng(5)
+    l = p.points.shape[0]
+    FI = rng.standard_normal(3 * p.n)
+    DFI = rng.standard_normal((p.ne, 27))
+    DFI[np.arange(27)[None, :] >= 3 * np.asarray(p.nconec)[:, None]] = 0.0
+    gu = rng.standard_normal((l, 3))
+    gs = rng.standard_normal((l, 6))
+    s = GradSession(p, variant=variant)
+    gfi, gdfi, gc, gnu = s.internal_points_vjp(FI, DFI, gu, gs)
+
+    # FI, DFI: adjoint identity with the GPU forward
+    FI2 = rng.standard_normal(3 * p.n)
+    DFI2 = rng.standard_normal((p.ne, 27))
+    DFI2[np.arange(27)[None, :] >= 3 * np.asarray(p.nconec)[:, None]] = 0.0
+    u2, s2 = s.internal_points(FI2, DFI2)
+    lhs = float(np.sum(gu * u2) + np.sum(gs * s2))
+    rhs = float(gfi @ FI2 + np.sum(gdfi * DFI2))
+    assert _rel(lhs, rhs) < 1e-12, (lhs, rhs)
+    s.close()
+
+    def S(q):
+        r = oracle.internal_points(q, oracle.geometry(q), FI, DFI, variant=variant)
+        return float(np.sum(gu * r["u"]) + np.sum(gs * r["sigma"]))
+
+    for trial in range(4):
+        d = rng.standard_normal((p.n, 3))
+        dnu = 0.1 * rng.standard_normal() if trial else 0.0
+        fd = (S(_shift(p, H * d, H * dnu)) - S(_shift(p, -H * d, -H * dnu))) / (2 * H)
+        an = float(np.sum(gc * d) + gnu * dnu)
+        assert _rel(an, fd) < 1e-6, (trial, an, fd)
+
+
+@pytest.mark.parametrize("name", ["grad_c0", "grad_mixed"])
+def test_objective_gradient_matches_fd(name):
+    """Adjoint gradient vs central differences of the GPU forward and vs the
+    reference's own central differences (golden)."""
+    from bem3d_elastostatic_amd.autograd import GradSession, gradient_vm2, value_vm2
+    g = golden(name)
+    p = problem_from_golden(g)
+    print(f"[grad] objective {name}", flush=True)
+    s = GradSession(p)
+    xyz = p.coords()
+    J, gc, gnu = gradient_vm2(s, xyz, p.nu)
+    assert _rel(J, float(g["J"])) < 1e-10, (J, float(g["J"]))
+    h = float(g["h"])
+    for i in range(len(g["fd"])):
+        d, dnu = g["directions"][i], float(g["dnu"][i])
+        an = float(np.sum(gc * d) + gnu * dnu)
+        assert _rel(an, float(g["fd"][i])) < 1e-6, ("reference fd", i, an, float(g["fd"][i]))
+        if i < 2:
+            fd = (value_vm2(s, xyz + h * d, p.nu + h * dnu) - value_vm2(s, xyz - h * d, p.nu - h * dnu)) / (2 * h)
+            assert _rel(an, fd) < 1e-6, ("gpu fd", i, an, fd)
+    s.close()
+
+
+def test_config4_gradient():
+    """The gradient configuration at full size (perturbed 16x16-per-face cube,
+    18,438 DOF, 4096 points): two directions against central differences of
+    the GPU's FP64 forward."""
+    from bem3d_elastostatic_amd import mesh
+    from bem3d_elastostatic_amd.autograd import GradSession, gradient_vm2, value_vm2
+    p = mesh.config4()
+    print("[grad] config4 n=6146 l=4096", flush=True)
+    s = GradSession(p)
+    xyz = p.coords()
+    J, gc, gnu = gradient_vm2(s, xyz, p.nu)
+    rng = np.random.default_rng(31)
+    for i in range(2):
+        d = rng.standard_normal((p.n, 3))
+        dnu = 0.1 * rng.standard_normal()
+        an = float(np.sum(gc * d) + gnu * dnu)
+        fd = (value_vm2(s, xyz + H * d, p.nu + H * dnu) - value_vm2(s, xyz - H * d, p.nu - H * dnu)) / (2 * H)
+        print(f"[grad] config4 direction {i}: adjoint {an:.12e} fd {fd:.12e}", flush=True)
+        assert _rel(an, fd) < 1e-6, (i, an, fd)
+    s.close()
