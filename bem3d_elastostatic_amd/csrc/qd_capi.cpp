@@ -744,6 +744,14 @@ int qd_internal_points_vjp(qd_handle h, int l, const double* cx, const double* c
   return somigliana_vjp_run(h, l, pts.data(), fi, dfi, variant, g_u, g_sigma, g_fi, g_dfi, gx, gy, gz, gnu);
 }
 
+int qd_memory_usage(long long* current_bytes, long long* peak_bytes, int reset_peak) {
+  MemStats& m = mem_stats();
+  if (current_bytes) *current_bytes = m.current.load();
+  if (peak_bytes) *peak_bytes = m.peak.load();
+  if (reset_peak) m.peak.store(m.current.load());
+  return QD_OK;
+}
+
 int qd_profile_enable(qd_handle h, unsigned mask) {
   if (!h) return QD_ERR_ARG;
   h->prof_mask = mask;

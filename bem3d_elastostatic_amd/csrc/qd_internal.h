@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -11,6 +12,22 @@
 #include "quadesth.h"
 
 namespace qd {
+
+// Device bytes held by the library's buffers (all handles): current and
+// high-water mark, for qd_memory_usage.
+struct MemStats {
+  std::atomic<long long> current{0}, peak{0};
+  void add(long long b) {
+    const long long c = current.fetch_add(b) + b;
+    long long pk = peak.load();
+    while (c > pk && !peak.compare_exchange_weak(pk, c)) {
+    }
+  }
+};
+inline MemStats& mem_stats() {
+  static MemStats s;
+  return s;
+}
 
 // Device buffer owned by the handle.
 template <typename T>
@@ -22,7 +39,10 @@ struct DevBuf {
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      (void)hipFree(p);
+      mem_stats().add(-(long long)(count * sizeof(T)));
+    }
     p = nullptr;
     count = 0;
   }
@@ -37,6 +57,7 @@ struct DevBuf {
       return e;
     }
     count = n;
+    mem_stats().add((long long)(n * sizeof(T)));
     return hipSuccess;
   }
 };

@@ -51,13 +51,14 @@ void dgesv_(const int* n, const int* nrhs, double* a, const int* lda, int* ipiv,
     return;
   }
   const size_t N = (size_t)*n;
-  double *da = nullptr, *db = nullptr;
-  if (hipSetDevice(h->device) != hipSuccess || hipMalloc((void**)&da, N * N * sizeof(double)) != hipSuccess ||
-      hipMalloc((void**)&db, N * (size_t)(*nrhs > 0 ? *nrhs : 1) * sizeof(double)) != hipSuccess) {
-    if (da) (void)hipFree(da);
+  qd::DevBuf<double> bufa, bufb;
+  if (hipSetDevice(h->device) != hipSuccess || bufa.ensure(N * N) != hipSuccess ||
+      bufb.ensure(N * (size_t)(*nrhs > 0 ? *nrhs : 1)) != hipSuccess) {
     *info = -100;
     return;
   }
+  double* da = bufa.p;
+  double* db = bufb.p;
   bool ok = hipMemcpy2D(da, N * sizeof(double), a, (size_t)*lda * sizeof(double), N * sizeof(double), N,
                         hipMemcpyHostToDevice) == hipSuccess;
   if (ok && *nrhs > 0)
@@ -74,8 +75,6 @@ void dgesv_(const int* n, const int* nrhs, double* a, const int* lda, int* ipiv,
   } else {
     *info = -100;
   }
-  (void)hipFree(da);
-  (void)hipFree(db);
 }
 
 }  // extern "C"

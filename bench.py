@@ -11,7 +11,9 @@ the mixed path against it.
 
 Prints ONE JSON line (rank 0).  Usage:
     python bench.py [--steps K] [--warmup W] [--precision fp16|fp32|fp64]
-                    [--config c3|c1] [--no-fp64] [--no-cpu]
+                    [--config c3|c1|c4] [--no-fp64] [--no-cpu] [--fd-dirs D]
+--config c4 measures the adjoint gradient on the perturbed cube instead
+(forward and backward times, peak device memory, finite-difference check).
 """
 from __future__ import annotations
 
@@ -158,13 +160,100 @@ def cpu_baseline(p, FI, DFI, budget_s=25.0):
                 phases=dict(assembly=t_asm, lu=t_lu, internal=t_som))
 
 
+def bench_gradient(args):
+    """Adjoint gradient of J = sum of internal von Mises stress squared on the
+    perturbed 16x16-per-face cube (18,438 DOF, 4096 points), FP64 forward:
+    forward and backward times, peak device memory of the library, and the
+    agreement with central differences (h = 1e-6) along seeded directions."""
+    import torch
+    from bem3d_elastostatic_amd import _lib, mesh
+    from bem3d_elastostatic_amd.autograd import BoundarySolve, GradSession, InternalPoints, von_mises_sq, value_vm2
+
+    p = mesh.config4()
+    s = GradSession(p)
+    lib = _lib.load()
+    xyz = p.coords()
+    log(f"gradient workload: n={p.n} ne={p.ne} N={3 * p.n} l={p.l}")
+
+    def run_once():
+        s._state = None                       # force the forward to recompute everything
+        c = torch.tensor(xyz, requires_grad=True)
+        v = torch.tensor(float(p.nu), dtype=torch.float64, requires_grad=True)
+        t0 = time.perf_counter()
+        FI, DFI = BoundarySolve.apply(c, v, s)
+        _, sigma = InternalPoints.apply(c, v, FI, DFI, s)
+        J = von_mises_sq(sigma).sum()
+        t1 = time.perf_counter()
+        J.backward()
+        t2 = time.perf_counter()
+        return float(J.detach()), c.grad.numpy().copy(), float(v.grad), t1 - t0, t2 - t1
+
+    for _ in range(max(args.warmup, 1)):
+        run_once()
+    cur, peak = C.c_longlong(), C.c_longlong()
+    lib.qd_memory_usage(C.byref(cur), C.byref(peak), 1)
+    fw, bw = [], []
+    for _ in range(args.steps):
+        J, gc, gnu, tf, tb = run_once()
+        fw.append(tf)
+        bw.append(tb)
+    lib.qd_memory_usage(C.byref(cur), C.byref(peak), 0)
+    log(f"J={J:.12e} forward {np.mean(fw):.3f} s backward {np.mean(bw):.3f} s peak {peak.value / 2**30:.2f} GiB")
+
+    rng = np.random.default_rng(64)
+    h = 1e-6
+
+    def central(d, dnu, hh):
+        return (value_vm2(s, xyz + hh * d, p.nu + hh * dnu) - value_vm2(s, xyz - hh * d, p.nu - hh * dnu)) / (2 * hh)
+
+    errs, abs_errs, dirs = [], [], []
+    for i in range(args.fd_dirs):
+        d = rng.standard_normal((p.n, 3))
+        dnu = 0.1 * rng.standard_normal()
+        an = float(np.sum(gc * d) + gnu * dnu)
+        fd = central(d, dnu, h)
+        errs.append(abs(an - fd) / max(abs(an), abs(fd), 1e-300))
+        abs_errs.append(abs(an - fd))
+        dirs.append((d, dnu, an))
+        if i % 8 == 0:
+            log(f"fd direction {i}: adjoint {an:.10e} fd {fd:.10e} rel {errs[-1]:.2e}")
+    # The central difference carries J's rounding noise / h.  For the worst
+    # direction, larger steps show whether the gap is that noise (it shrinks
+    # with 1/h) or a gradient error (it does not).
+    iw = int(np.argmax(errs))
+    d, dnu, an = dirs[iw]
+    worst = {"direction": iw, "adjoint": an, "rel_err_h": {}}
+    for hh in (1e-6, 1e-5, 1e-4):
+        fd = central(d, dnu, hh)
+        worst["rel_err_h"][f"{hh:g}"] = abs(an - fd) / max(abs(an), abs(fd), 1e-300)
+    log(f"worst direction {iw}: {worst}")
+    out = {
+        "metric": "adjoint gradient of sum(vm^2) w.r.t. 18438 coordinates + nu: backward/forward time",
+        "value": round(float(np.mean(bw)) / float(np.mean(fw)), 3), "unit": "ratio",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": False,
+        "forward_s": round(float(np.mean(fw)), 4), "backward_s": round(float(np.mean(bw)), 4),
+        "peak_library_hbm_bytes": int(peak.value), "J": J,
+        "fd_check": {"directions": args.fd_dirs, "h": h, "max_rel_err": float(np.max(errs)),
+                     "median_rel_err": float(np.median(errs)), "max_abs_err": float(np.max(abs_errs)),
+                     "grad_norm": float(np.sqrt(np.sum(gc * gc) + gnu * gnu)), "worst": worst},
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "unit cube 16x16 quad9 per face, radial bump 0.02 (seed 3): 6146 nodes, "
+                               "18438 DOF, 4096 internal points; J = sum vm^2", "precision": "fp64"},
+    }
+    print(json.dumps(out), flush=True)
+    s.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", default="fp16", choices=["fp16", "fp32", "fp64"])
-    ap.add_argument("--config", default="c3", choices=["c3", "c1"])
+    ap.add_argument("--config", default="c3", choices=["c3", "c1", "c4"],
+                    help="c3: the 76,806-DOF cantilever step (default); c1: the 18,438-DOF cube; "
+                         "c4: adjoint gradient on the perturbed cube")
+    ap.add_argument("--fd-dirs", type=int, default=64, help="c4: directions of the finite-difference check")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -180,8 +269,8 @@ def main():
         print(json.dumps({"copy_bytes_each_way": src.numel() * 8, "launches": 3}))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: the dense LU of this path is not distributed in this build "
-              "(the sharding plan is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the dense LU of this path runs on a single device in this "
+              "build; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -189,6 +278,9 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+    if args.config == "c4":
+        bench_gradient(args)
+        return
     if args.config == "c3":
         p = mesh.config3()
         workload = ("thick cantilever box [0,10]x[0,2]x[0,1], h=0.1: 25602 nodes, 2400 quad9 + 8000 tri6, "

@@ -187,6 +187,11 @@ enum {
   QD_PROF_KINDS = 6
 };
 int qd_profile_enable(qd_handle h, unsigned mask /* bit per QD_PROF_* */);
+/* Device memory held by the library's buffers across all handles in this
+ * process: current bytes and the high-water mark since the last reset
+ * (reset_peak != 0 resets it to the current value after reading). */
+int qd_memory_usage(long long* current_bytes, long long* peak_bytes, int reset_peak);
+
 int qd_profile_read(qd_handle h, int kind, long long* launches, double* ms, double* work);
 
 /* ---- LAPACK drop-in ------------------------------------------------------
