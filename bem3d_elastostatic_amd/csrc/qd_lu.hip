@@ -448,13 +448,32 @@ __global__ __launch_bounds__(256, 2) void k_gemm64(long long M, long long N, int
     if (kt + 1 < nk) lstore(cur ^ 1);
     __syncthreads();
   }
+  // f64 16x16x4 output rows interleave by 4.  Interior tiles update C with
+  // no per-element test, so each column group's 16 loads are issued together
+  // before one wait (a guarded element would serialise load, wait, store).
+  if (m0 + D_BM <= M && n0 + D_BN <= N) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const long long gc = n0 + wn + 16 * j + lr;
+      double cv[4][4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) cv[i][r] = C[(m0 + wm + 16 * i + lk + 4 * r) * ldc + gc];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) C[(m0 + wm + 16 * i + lk + 4 * r) * ldc + gc] = cv[i][r] - acc[i][j][r];
+    }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const long long gr = m0 + wm + 16 * i + lk + 4 * r;  // f64 16x16x4 rows interleave by 4
+        const long long gr = m0 + wm + 16 * i + lk + 4 * r;
         const long long gc = n0 + wn + 16 * j + lr;
         if (gr < M && gc < N) C[gr * ldc + gc] -= acc[i][j][r];
       }
@@ -611,6 +630,26 @@ __global__ __launch_bounds__(256, 2) void k_gemm16(long long M, long long N, int
     if (kt + 1 < nk) lstore(cur ^ 1);
     __syncthreads();
   }
+  // Interior tiles update C with no per-element test, so each column group's
+  // 32 loads are issued together before one wait (a guarded element would
+  // serialise load, wait, store).
+  if (m0 + H_BM <= M && n0 + H_BN <= N) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const long long gc = n0 + wn + 16 * j + lr;
+      const float s = colscale[gc >> 6];
+      float cv[8][4];
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) cv[i][r] = C[(m0 + wm + 16 * i + 4 * lk + r) * ldc + gc];
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) C[(m0 + wm + 16 * i + 4 * lk + r) * ldc + gc] = cv[i][r] - s * acc[i][j][r];
+    }
+    return;
+  }
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const long long gc = n0 + wn + 16 * j + lr;
@@ -642,6 +681,34 @@ __device__ __forceinline__ unsigned long long ts_key(float v, int row) {
   return ((unsigned long long)__float_as_uint(fabsf(v)) << 32) | (unsigned)(~row);
 }
 
+// Wave maximum of a 64-bit key: DPP butterflies inside each row of 16 lanes
+// (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror),
+// then the four row maxima through readlane.  Called by all 64 lanes.
+template <int kCtrl>
+__device__ __forceinline__ unsigned long long dpp_max64(unsigned long long v) {
+  const int lo = (int)(unsigned)v, hi = (int)(unsigned)(v >> 32);
+  const unsigned olo = (unsigned)__builtin_amdgcn_update_dpp(lo, lo, kCtrl, 0xf, 0xf, false);
+  const unsigned ohi = (unsigned)__builtin_amdgcn_update_dpp(hi, hi, kCtrl, 0xf, 0xf, false);
+  const unsigned long long o = ((unsigned long long)ohi << 32) | olo;
+  return o > v ? o : v;
+}
+
+__device__ __forceinline__ unsigned long long wave_max64(unsigned long long v) {
+  v = dpp_max64<0xB1>(v);
+  v = dpp_max64<0x4E>(v);
+  v = dpp_max64<0x141>(v);
+  v = dpp_max64<0x140>(v);
+  unsigned long long m = 0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, 16 * r);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), 16 * r);
+    const unsigned long long x = ((unsigned long long)hi << 32) | lo;
+    m = x > m ? x : m;
+  }
+  return m;
+}
+
 // One elimination step (column K); K is a compile-time constant so the
 // register arrays are indexed statically.
 template <int K>
@@ -653,8 +720,7 @@ __device__ __forceinline__ void ts_step(float (&a)[kTsRPT][kTsW], int (&gidx)[kT
 #pragma unroll
   for (int q = 0; q < kTsRPT; ++q)
     if (!done && gidx[q] >= 0) key = max(key, ts_key(a[q][K], gidx[q]));
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) key = max(key, (unsigned long long)__shfl_xor(key, off, 64));
+  key = wave_max64(key);
   const bool found = !done && key != 0;
   const int ib = (int)(~(unsigned)(key & 0xffffffffull));
   if (found) {
@@ -872,13 +938,17 @@ struct LuCtx {
   bool calu;  // tournament pivoting in the leaves (preconditioner factorisations)
 };
 
+constexpr double kF16MinWork = 2.0e9;
+
 template <typename T>
 static int gemm_sub(LuCtx<T>& c, long long M, long long Nn, int K, const T* A, const T* B, T* C, bool allow_f16,
                     bool trailing = false) {
   if (M <= 0 || Nn <= 0 || K <= 0) return QD_OK;
   dim3 grid((unsigned)((Nn + GBN - 1) / GBN), (unsigned)((M + GBM - 1) / GBM));
   if constexpr (std::is_same<T, float>::value) {
-    if (c.f16 && allow_f16) {
+    // FP16 conversion costs three extra launches; below kF16MinWork flops the
+    // FP32 MFMA kernel on W directly is faster.
+    if (c.f16 && allow_f16 && (trailing || (double)M * (double)Nn * (double)K >= kF16MinWork)) {
       const long long nb = (Nn + 63) / 64;
       const int Kp = (K + 31) / 32 * 32;
       QD_ALLOC(c.h, c.s->colscale, (size_t)nb);
