@@ -337,24 +337,51 @@ __global__ __launch_bounds__(256) void k_gemm_sub(long long M, long long N, int 
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = (typename MT::acc_t){0, 0, 0, 0};
+  // Interior tiles load without per-element tests, so all loads of a K step
+  // are in flight together; edge tiles take the guarded loads.
+  const bool full_mn = (m0 + GBM <= M) && (n0 + GBN <= N);
   for (int k0 = 0; k0 < K; k0 += BK) {
-    // A tile: GBM rows x BK cols
+    if (full_mn && k0 + BK <= K) {
+      T va[GBM * BK / 256], vb[GBN * BK / 256];
 #pragma unroll
-    for (int i = 0; i < GBM * BK / 256; ++i) {
-      const int idx = tid + 256 * i;
-      const int r = idx / BK, c = idx % BK;
-      const long long gr = m0 + r;
-      const int gc = k0 + c;
-      sA[c][r] = (gr < M && gc < K) ? A[gr * lda + gc] : (T)0;
-    }
-    // B tile: BK rows x GBN cols
+      for (int i = 0; i < GBM * BK / 256; ++i) {
+        const int idx = tid + 256 * i;
+        va[i] = A[(m0 + idx / BK) * lda + k0 + idx % BK];
+      }
 #pragma unroll
-    for (int i = 0; i < GBN * BK / 256; ++i) {
-      const int idx = tid + 256 * i;
-      const int r = idx / GBN, c = idx % GBN;
-      const int gr = k0 + r;
-      const long long gc = n0 + c;
-      sB[r][c] = (gr < K && gc < N) ? B[(long long)gr * ldb + gc] : (T)0;
+      for (int i = 0; i < GBN * BK / 256; ++i) {
+        const int idx = tid + 256 * i;
+        vb[i] = B[(long long)(k0 + idx / GBN) * ldb + n0 + idx % GBN];
+      }
+#pragma unroll
+      for (int i = 0; i < GBM * BK / 256; ++i) {
+        const int idx = tid + 256 * i;
+        sA[idx % BK][idx / BK] = va[i];
+      }
+#pragma unroll
+      for (int i = 0; i < GBN * BK / 256; ++i) {
+        const int idx = tid + 256 * i;
+        sB[idx / GBN][idx % GBN] = vb[i];
+      }
+    } else {
+      // A tile: GBM rows x BK cols
+#pragma unroll
+      for (int i = 0; i < GBM * BK / 256; ++i) {
+        const int idx = tid + 256 * i;
+        const int r = idx / BK, c = idx % BK;
+        const long long gr = m0 + r;
+        const int gc = k0 + c;
+        sA[c][r] = (gr < M && gc < K) ? A[gr * lda + gc] : (T)0;
+      }
+      // B tile: BK rows x GBN cols
+#pragma unroll
+      for (int i = 0; i < GBN * BK / 256; ++i) {
+        const int idx = tid + 256 * i;
+        const int r = idx / GBN, c = idx % GBN;
+        const int gr = k0 + r;
+        const long long gc = n0 + c;
+        sB[r][c] = (gr < K && gc < N) ? B[(long long)gr * ldb + gc] : (T)0;
+      }
     }
     __syncthreads();
 #pragma unroll
@@ -370,6 +397,22 @@ __global__ __launch_bounds__(256) void k_gemm_sub(long long M, long long N, int 
         for (int j = 0; j < 4; ++j) acc[i][j] = MT::mma(a[i], b[j], acc[i][j]);
     }
     __syncthreads();
+  }
+  if (full_mn) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const long long gc = n0 + wn + 16 * j + lr;
+      T cv[4][4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) cv[i][r] = C[(m0 + wm + 16 * i + MT::out_row(lk, r)) * ldc + gc];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) C[(m0 + wm + 16 * i + MT::out_row(lk, r)) * ldc + gc] = cv[i][r] - acc[i][j][r];
+    }
+    return;
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -1256,6 +1299,63 @@ __global__ __launch_bounds__(256) void k_block_update(const T* __restrict__ W, l
   }
 }
 
+// x[v][rows] -= W[rows, c0:c0+m] x[v][c0:c0+m] for rows [r0, r1), m <= TB,
+// TB * sizeof(T) = 512 bytes.  32 lanes per row, each with one 16-byte
+// piece of it and of x; the row sum is reduced across the 32 lanes.  A
+// workgroup of 256 threads covers 8 rows per pass and kBuRows rows.  Columns
+// at or beyond m (the partial last block) are masked before loading, since
+// they may lie past the end of the row.
+constexpr int kBuRows = 64, kBuMaxRhs = 8;
+template <typename T, int TB>
+__global__ __launch_bounds__(256) void k_block_update_rows(const T* __restrict__ W, long long ld, long long N,
+                                                           long long r0, long long r1, long long c0, int m, int nrhs,
+                                                           T* __restrict__ x) {
+  constexpr int E = 16 / sizeof(T);  // elements per lane
+  static_assert(E * 32 == TB, "row piece layout");
+  const int sub = threadIdx.x & 31, slot = threadIdx.x >> 5;
+  const int col = sub * E;
+  T xr[kBuMaxRhs][E];
+#pragma unroll
+  for (int v = 0; v < kBuMaxRhs; ++v)
+#pragma unroll
+    for (int e = 0; e < E; ++e) xr[v][e] = (v < nrhs && col + e < m) ? x[(long long)v * N + c0 + col + e] : (T)0;
+  const bool full = (m == TB);
+  const long long rbase = r0 + (long long)blockIdx.x * kBuRows;
+  for (int pass = 0; pass < kBuRows / 8; ++pass) {
+    const long long r = rbase + pass * 8 + slot;
+    if (r >= r1) break;
+    const T* wr = W + r * ld + c0 + col;
+    T w[E];
+    if (full) {
+      if constexpr (sizeof(T) == 4) {
+        const float4 q = *reinterpret_cast<const float4*>(wr);
+        w[0] = q.x;
+        w[1] = q.y;
+        w[2] = q.z;
+        w[3] = q.w;
+      } else {
+        const double2 q = *reinterpret_cast<const double2*>(wr);
+        w[0] = q.x;
+        w[1] = q.y;
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < E; ++e) w[e] = (col + e < m) ? wr[e] : (T)0;
+    }
+#pragma unroll
+    for (int v = 0; v < kBuMaxRhs; ++v) {
+      if (v < nrhs) {
+        T sum = 0;
+#pragma unroll
+        for (int e = 0; e < E; ++e) sum += w[e] * xr[v][e];
+#pragma unroll
+        for (int off = 16; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 32);
+        if (sub == 0) x[(long long)v * N + r] -= sum;
+      }
+    }
+  }
+}
+
 template <typename T>
 static int diag_inverses(qd_context* h, const T* W, long long ld, long long N, T* inv) {
   constexpr int TB = TriTB<T>::v;
@@ -1278,18 +1378,28 @@ static int tri_solve_inv(qd_context* h, const T* W, long long ld, long long N, i
     const int m = (int)std::min<long long>(TB, N - i0);
     hipLaunchKernelGGL((k_apply_inv<T, TB>), dim3(1), dim3(256), 0, h->stream, invL + (size_t)b * TB * TB, N, i0, m,
                        nrhs, x);
-    if (i0 + m < N)
-      hipLaunchKernelGGL((k_block_update<T, TB>), dim3((unsigned)((N - i0 - m + 63) / 64)), dim3(256), 0, h->stream,
-                         W, ld, N, i0 + m, N, i0, m, nrhs, x);
+    if (i0 + m < N) {
+      if (nrhs <= kBuMaxRhs)
+        hipLaunchKernelGGL((k_block_update_rows<T, TB>), dim3((unsigned)((N - i0 - m + kBuRows - 1) / kBuRows)),
+                           dim3(256), 0, h->stream, W, ld, N, i0 + m, N, i0, m, nrhs, x);
+      else
+        hipLaunchKernelGGL((k_block_update<T, TB>), dim3((unsigned)((N - i0 - m + 63) / 64)), dim3(256), 0,
+                           h->stream, W, ld, N, i0 + m, N, i0, m, nrhs, x);
+    }
   }
   for (long long b = nblk - 1; b >= 0; --b) {
     const long long i0 = b * TB;
     const int m = (int)std::min<long long>(TB, N - i0);
     hipLaunchKernelGGL((k_apply_inv<T, TB>), dim3(1), dim3(256), 0, h->stream, invU + (size_t)b * TB * TB, N, i0, m,
                        nrhs, x);
-    if (i0 > 0)
-      hipLaunchKernelGGL((k_block_update<T, TB>), dim3((unsigned)((i0 + 63) / 64)), dim3(256), 0, h->stream, W, ld, N,
-                         0LL, i0, i0, m, nrhs, x);
+    if (i0 > 0) {
+      if (nrhs <= kBuMaxRhs)
+        hipLaunchKernelGGL((k_block_update_rows<T, TB>), dim3((unsigned)((i0 + kBuRows - 1) / kBuRows)), dim3(256), 0,
+                           h->stream, W, ld, N, 0LL, i0, i0, m, nrhs, x);
+      else
+        hipLaunchKernelGGL((k_block_update<T, TB>), dim3((unsigned)((i0 + 63) / 64)), dim3(256), 0, h->stream, W,
+                           ld, N, 0LL, i0, i0, m, nrhs, x);
+    }
   }
   QD_HIP(h, hipGetLastError());
   return QD_OK;
