@@ -171,10 +171,12 @@ struct ProfScope {
   qd_context* h;
   int kind;
   double work;
+  hipStream_t st;
   hipEvent_t a = nullptr;
-  ProfScope(qd_context* h_, int kind_, double work_) : h(h_), kind(kind_), work(work_) {
+  ProfScope(qd_context* h_, int kind_, double work_) : ProfScope(h_, kind_, work_, h_->stream) {}
+  ProfScope(qd_context* h_, int kind_, double work_, hipStream_t st_) : h(h_), kind(kind_), work(work_), st(st_) {
     if (kind >= 0 && ((h->prof_mask >> kind) & 1)) {
-      if (hipEventCreate(&a) == hipSuccess) (void)hipEventRecord(a, h->stream);
+      if (hipEventCreate(&a) == hipSuccess) (void)hipEventRecord(a, st);
       else a = nullptr;
     }
   }
@@ -182,7 +184,7 @@ struct ProfScope {
     if (!a) return;
     hipEvent_t b;
     if (hipEventCreate(&b) != hipSuccess) return;
-    (void)hipEventRecord(b, h->stream);
+    (void)hipEventRecord(b, st);
     h->prof_pending.push_back(qd_context::ProfRec{kind, a, b, work});
   }
 };

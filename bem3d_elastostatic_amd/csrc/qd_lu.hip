@@ -51,8 +51,15 @@ struct qd_lu_state {
   qd::DevBuf<int> piv_idx;
   qd::DevBuf<int> piv_state;     // [0] counter, [1] info
   qd::DevBuf<double> rpiv;       // reciprocal of the current pivot
-  qd::DevBuf<float> colscale;    // per-64-column scales of U12 (fp16 mode)
-  qd::DevBuf<_Float16> h16a, h16b;
+  // FP16 operand copies of the GEMMs (fp16 mode): one set for the outer
+  // step's updates, one for the panel, which runs on its own stream.
+  struct F16Scratch {
+    qd::DevBuf<float> colscale;  // per-64-column scales of U12
+    qd::DevBuf<_Float16> h16a, h16b;
+  };
+  F16Scratch sc_main, sc_panel;
+  hipStream_t aux = nullptr;     // panel stream of the look-ahead (non-blocking, default priority)
+  hipEvent_t ev_cols = nullptr, ev_panel = nullptr;
   qd::DevBuf<double> vec;        // solve / refinement work vectors
   qd::DevBuf<float> vecf;
   qd::DevBuf<double> gm;         // GMRES basis
@@ -61,6 +68,11 @@ struct qd_lu_state {
   qd::DevBuf<float> inv32;       // inverted diagonal blocks of L and U
   qd::DevBuf<double> inv64;
   double anorm = -1.0;           // ||A||_inf of the FP64 matrix
+  ~qd_lu_state() {
+    if (ev_cols) (void)hipEventDestroy(ev_cols);
+    if (ev_panel) (void)hipEventDestroy(ev_panel);
+    if (aux) (void)hipStreamDestroy(aux);
+  }
 };
 
 namespace qd {
@@ -979,13 +991,17 @@ struct LuCtx {
   long long ld, N;
   bool f16;   // trailing updates in fp16
   bool calu;  // tournament pivoting in the leaves (preconditioner factorisations)
+  hipStream_t st = nullptr;                   // stream of every launch of this context
+  qd_lu_state::F16Scratch* sc = nullptr;      // FP16 operand copies of this context
 };
 
 constexpr double kF16MinWork = 2.0e9;
 
+// reuse_a16: A's FP16 copy is already in c.sc->h16a (the same A as the
+// previous call on this context's stream).
 template <typename T>
 static int gemm_sub(LuCtx<T>& c, long long M, long long Nn, int K, const T* A, const T* B, T* C, bool allow_f16,
-                    bool trailing = false) {
+                    bool trailing = false, bool reuse_a16 = false) {
   if (M <= 0 || Nn <= 0 || K <= 0) return QD_OK;
   dim3 grid((unsigned)((Nn + GBN - 1) / GBN), (unsigned)((M + GBM - 1) / GBM));
   if constexpr (std::is_same<T, float>::value) {
@@ -994,39 +1010,40 @@ static int gemm_sub(LuCtx<T>& c, long long M, long long Nn, int K, const T* A, c
     if (c.f16 && allow_f16 && (trailing || (double)M * (double)Nn * (double)K >= kF16MinWork)) {
       const long long nb = (Nn + 63) / 64;
       const int Kp = (K + 31) / 32 * 32;
-      QD_ALLOC(c.h, c.s->colscale, (size_t)nb);
-      QD_ALLOC(c.h, c.s->h16a, (size_t)M * Kp);
-      QD_ALLOC(c.h, c.s->h16b, (size_t)Nn * Kp);
-      hipLaunchKernelGGL(k_colblock_scale, dim3((unsigned)nb), dim3(256), 0, c.h->stream, K, Nn, B, c.ld,
-                         c.s->colscale.p);
+      QD_ALLOC(c.h, c.sc->colscale, (size_t)nb);
+      QD_ALLOC(c.h, c.sc->h16a, (size_t)M * Kp);
+      QD_ALLOC(c.h, c.sc->h16b, (size_t)Nn * Kp);
+      hipLaunchKernelGGL(k_colblock_scale, dim3((unsigned)nb), dim3(256), 0, c.st, K, Nn, B, c.ld,
+                         c.sc->colscale.p);
       hipLaunchKernelGGL(k_cvt_b16t, dim3((unsigned)((Nn + 31) / 32), (unsigned)(Kp / 32)), dim3(256), 0,
-                         c.h->stream, K, Nn, Kp, B, c.ld, c.s->colscale.p, c.s->h16b.p);
-      hipLaunchKernelGGL(k_cvt_a16, dim3((unsigned)((M * (Kp / 8) + 255) / 256)), dim3(256), 0, c.h->stream, M, K,
-                         Kp, A, c.ld, c.s->h16a.p);
+                         c.st, K, Nn, Kp, B, c.ld, c.sc->colscale.p, c.sc->h16b.p);
+      if (!reuse_a16)
+        hipLaunchKernelGGL(k_cvt_a16, dim3((unsigned)((M * (Kp / 8) + 255) / 256)), dim3(256), 0, c.st, M, K,
+                           Kp, A, c.ld, c.sc->h16a.p);
       dim3 g16((unsigned)((Nn + H_BN - 1) / H_BN), (unsigned)((M + H_BM - 1) / H_BM));
-      ProfScope ps(c.h, trailing ? QD_PROF_LU_TRAILING_GEMM : -1, 2.0 * (double)M * (double)Nn * K);
+      ProfScope ps(c.h, trailing ? QD_PROF_LU_TRAILING_GEMM : -1, 2.0 * (double)M * (double)Nn * K, c.st);
       if (trailing)
-        hipLaunchKernelGGL(k_gemm16<1>, g16, dim3(256), 0, c.h->stream, M, Nn, Kp, c.s->h16a.p, c.s->h16b.p,
-                           c.s->colscale.p, C, c.ld);
+        hipLaunchKernelGGL(k_gemm16<1>, g16, dim3(256), 0, c.st, M, Nn, Kp, c.sc->h16a.p, c.sc->h16b.p,
+                           c.sc->colscale.p, C, c.ld);
       else
-        hipLaunchKernelGGL(k_gemm16<0>, g16, dim3(256), 0, c.h->stream, M, Nn, Kp, c.s->h16a.p, c.s->h16b.p,
-                           c.s->colscale.p, C, c.ld);
+        hipLaunchKernelGGL(k_gemm16<0>, g16, dim3(256), 0, c.st, M, Nn, Kp, c.sc->h16a.p, c.sc->h16b.p,
+                           c.sc->colscale.p, C, c.ld);
       QD_HIP(c.h, hipGetLastError());
       return QD_OK;
     }
   }
   if constexpr (std::is_same<T, double>::value) {
     dim3 g64((unsigned)((Nn + D_BN - 1) / D_BN), (unsigned)((M + D_BM - 1) / D_BM));
-    ProfScope ps(c.h, trailing ? QD_PROF_LU_TRAILING_GEMM : -1, 2.0 * (double)M * (double)Nn * K);
+    ProfScope ps(c.h, trailing ? QD_PROF_LU_TRAILING_GEMM : -1, 2.0 * (double)M * (double)Nn * K, c.st);
     if (trailing)
-      hipLaunchKernelGGL(k_gemm64<1>, g64, dim3(256), 0, c.h->stream, M, Nn, K, A, c.ld, B, c.ld, C, c.ld);
+      hipLaunchKernelGGL(k_gemm64<1>, g64, dim3(256), 0, c.st, M, Nn, K, A, c.ld, B, c.ld, C, c.ld);
     else
-      hipLaunchKernelGGL(k_gemm64<0>, g64, dim3(256), 0, c.h->stream, M, Nn, K, A, c.ld, B, c.ld, C, c.ld);
+      hipLaunchKernelGGL(k_gemm64<0>, g64, dim3(256), 0, c.st, M, Nn, K, A, c.ld, B, c.ld, C, c.ld);
     QD_HIP(c.h, hipGetLastError());
     return QD_OK;
   }
-  ProfScope ps(c.h, trailing ? QD_PROF_LU_TRAILING_GEMM : -1, 2.0 * (double)M * (double)Nn * K);
-  hipLaunchKernelGGL(k_gemm_sub<T>, grid, dim3(256), 0, c.h->stream, M, Nn, K, A, c.ld, B, c.ld, C, c.ld);
+  ProfScope ps(c.h, trailing ? QD_PROF_LU_TRAILING_GEMM : -1, 2.0 * (double)M * (double)Nn * K, c.st);
+  hipLaunchKernelGGL(k_gemm_sub<T>, grid, dim3(256), 0, c.st, M, Nn, K, A, c.ld, B, c.ld, C, c.ld);
   QD_HIP(c.h, hipGetLastError());
   return QD_OK;
 }
@@ -1036,7 +1053,7 @@ template <typename T>
 static int trsm_lower(LuCtx<T>& c, long long r0, int m, long long c0, long long c1) {
   if (c1 <= c0 || m <= 0) return QD_OK;
   if (m <= 32) {
-    hipLaunchKernelGGL(k_trsm_small<T>, dim3((unsigned)((c1 - c0 + 255) / 256)), dim3(256), 0, c.h->stream, c.W,
+    hipLaunchKernelGGL(k_trsm_small<T>, dim3((unsigned)((c1 - c0 + 255) / 256)), dim3(256), 0, c.st, c.W,
                        c.ld, r0, m, c0, c1);
     QD_HIP(c.h, hipGetLastError());
     return QD_OK;
@@ -1057,7 +1074,7 @@ static int colstep(LuCtx<T>& c, long long col, long long prev, long long lstart,
   const long long rows = c.N - col;
   if (rows <= 0) return QD_OK;
   const unsigned nb = (unsigned)((rows + kColRowsPerBlock - 1) / kColRowsPerBlock);
-  hipLaunchKernelGGL(k_colstep<T>, dim3(nb), dim3(kColThreads), 0, c.h->stream, c.W, c.ld, c.N, col, prev, lstart,
+  hipLaunchKernelGGL(k_colstep<T>, dim3(nb), dim3(kColThreads), 0, c.st, c.W, c.ld, c.N, col, prev, lstart,
                      lend, pstart, pend, c.s->piv_part.p, c.s->piv_idx.p, c.s->piv_state.p, c.s->rpiv.p,
                      c.s->ipiv_dev.p);
   QD_HIP(c.h, hipGetLastError());
@@ -1075,23 +1092,23 @@ static int tslu_leaf(LuCtx<float>& c, long long c0, int w, long long p0, long lo
   QD_ALLOC(c.h, s->ts_b, (size_t)nb * kTsW);
   int* in = s->ts_a.p;
   int* out = s->ts_b.p;
-  hipLaunchKernelGGL(k_tslu_select, dim3(nb), dim3(64), 0, c.h->stream, c.W, c.ld, c.N, c0, w, (const int*)nullptr,
+  hipLaunchKernelGGL(k_tslu_select, dim3(nb), dim3(64), 0, c.st, c.W, c.ld, c.N, c0, w, (const int*)nullptr,
                      0, in);
   int n_in = nb * kTsW;
   while (nb > 1) {
     const int nb2 = (n_in + kTsRows - 1) / kTsRows;
-    hipLaunchKernelGGL(k_tslu_select, dim3(nb2), dim3(64), 0, c.h->stream, c.W, c.ld, c.N, c0, w, (const int*)in,
+    hipLaunchKernelGGL(k_tslu_select, dim3(nb2), dim3(64), 0, c.st, c.W, c.ld, c.N, c0, w, (const int*)in,
                        n_in, out);
     n_in = nb2 * kTsW;
     std::swap(in, out);
     nb = nb2;
   }
-  hipLaunchKernelGGL(k_tslu_ipiv, dim3(1), dim3(1), 0, c.h->stream, (const int*)in, w, c0, s->ipiv_dev.p);
-  hipLaunchKernelGGL(k_laswp<float>, dim3((unsigned)((p1 - p0 + 255) / 256)), dim3(256), 0, c.h->stream, c.W, c.ld,
+  hipLaunchKernelGGL(k_tslu_ipiv, dim3(1), dim3(1), 0, c.st, (const int*)in, w, c0, s->ipiv_dev.p);
+  hipLaunchKernelGGL(k_laswp<float>, dim3((unsigned)((p1 - p0 + 255) / 256)), dim3(256), 0, c.st, c.W, c.ld,
                      p0, p1, (int)c0, (int)(c0 + w), s->ipiv_dev.p);
-  hipLaunchKernelGGL(k_tslu_top, dim3(1), dim3(64), 0, c.h->stream, c.W, c.ld, c0, w);
+  hipLaunchKernelGGL(k_tslu_top, dim3(1), dim3(64), 0, c.st, c.W, c.ld, c0, w);
   const long long below = std::max<long long>(m - w, 1);
-  hipLaunchKernelGGL(k_tslu_finish, dim3((unsigned)((below + 255) / 256)), dim3(256), 0, c.h->stream, c.W, c.ld,
+  hipLaunchKernelGGL(k_tslu_finish, dim3((unsigned)((below + 255) / 256)), dim3(256), 0, c.st, c.W, c.ld,
                      c.N, c0, w);
   QD_HIP(c.h, hipGetLastError());
   return QD_OK;
@@ -1122,6 +1139,28 @@ static int panel(LuCtx<T>& c, long long c0, int w, long long p0, long long p1) {
   return panel(c, c0 + w1, w - w1, p0, p1);
 }
 
+// Panel of columns [j, j+jb) on the context's stream, timed as one lu_panel.
+template <typename T>
+static int panel_step(LuCtx<T>& c, long long j, int jb) {
+  const double m = (double)(c.N - j);
+  ProfScope ps(c.h, QD_PROF_LU_PANEL, m * jb * (double)jb - (double)jb * jb * jb / 3.0, c.st);
+  return panel(c, j, jb, j, j + jb);
+}
+
+// A22[:, c0:c1) -= L21 U12[:, c0:c1) for the step whose panel is [j, j+jb).
+template <typename T>
+static int update_cols(LuCtx<T>& c, long long j, int jb, long long c0, long long c1, bool reuse_a16) {
+  const long long r0 = j + jb;
+  return gemm_sub(c, c.N - r0, c1 - c0, jb, c.W + r0 * c.ld + j, c.W + j * c.ld + c0, c.W + r0 * c.ld + c0, true,
+                  true, reuse_a16);
+}
+
+// Blocked right-looking LU with a look-ahead of one panel: each step updates
+// the next panel's columns first; that panel is then factored on the aux
+// stream while the rest of the trailing matrix is updated on the main stream
+// (the two touch disjoint columns and separate FP16 scratch).  The main
+// stream waits for the panel before the next step's interchanges.
+// QUADESTH_LOOKAHEAD=0 runs the steps strictly in sequence.
 template <typename T>
 static int factor(LuCtx<T>& c, int NB) {
   qd_lu_state* s = c.s;
@@ -1132,27 +1171,55 @@ static int factor(LuCtx<T>& c, int NB) {
   QD_ALLOC(c.h, s->piv_state, 2);
   QD_ALLOC(c.h, s->rpiv, 1);
   QD_ALLOC(c.h, s->ipiv_dev, (size_t)N);
-  QD_HIP(c.h, hipMemsetAsync(s->piv_state.p, 0, 2 * sizeof(int), c.h->stream));
+  if (c.f16) {
+    // sized up front, so that nothing is reallocated while two streams run
+    const size_t Kp = (size_t)((NB + 31) / 32 * 32), Kh = (size_t)((NB / 2 + 31) / 32 * 32);
+    QD_ALLOC(c.h, s->sc_main.h16a, (size_t)N * Kp);
+    QD_ALLOC(c.h, s->sc_main.h16b, (size_t)N * Kp);
+    QD_ALLOC(c.h, s->sc_main.colscale, (size_t)(N / 64 + 2));
+    QD_ALLOC(c.h, s->sc_panel.h16a, (size_t)N * Kh);
+    QD_ALLOC(c.h, s->sc_panel.h16b, Kh * Kh);
+    QD_ALLOC(c.h, s->sc_panel.colscale, (size_t)(NB / 64 + 2));
+  }
+  const char* la = std::getenv("QUADESTH_LOOKAHEAD");
+  const bool look = !(la && la[0] == '0');
+  if (look && !s->aux) {
+    QD_HIP(c.h, hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking));
+    QD_HIP(c.h, hipEventCreateWithFlags(&s->ev_cols, hipEventDisableTiming));
+    QD_HIP(c.h, hipEventCreateWithFlags(&s->ev_panel, hipEventDisableTiming));
+  }
+  c.sc = &s->sc_main;
+  LuCtx<T> cp = c;
+  cp.st = s->aux;
+  cp.sc = &s->sc_panel;
+  QD_HIP(c.h, hipMemsetAsync(s->piv_state.p, 0, 2 * sizeof(int), c.st));
   int rc;
+  if ((rc = panel_step(c, 0, (int)std::min<long long>(NB, N)))) return rc;
   for (long long j = 0; j < N; j += NB) {
     const int jb = (int)std::min<long long>(NB, N - j);
-    {
-      const double m = (double)(N - j);
-      ProfScope ps(c.h, QD_PROF_LU_PANEL, m * jb * (double)jb - (double)jb * jb * jb / 3.0);
-      if ((rc = panel(c, j, jb, j, j + jb))) return rc;
-    }
+    const long long nj = j + jb;
+    const int njb = nj < N ? (int)std::min<long long>(NB, N - nj) : 0;
     // interchanges on the columns left and right of the panel
     if (j > 0)
-      hipLaunchKernelGGL(k_laswp<T>, dim3((unsigned)((j + 255) / 256)), dim3(256), 0, c.h->stream, c.W, c.ld, 0LL,
-                         j, (int)j, (int)(j + jb), s->ipiv_dev.p);
-    if (j + jb < N) {
-      hipLaunchKernelGGL(k_laswp<T>, dim3((unsigned)((N - j - jb + 255) / 256)), dim3(256), 0, c.h->stream, c.W,
-                         c.ld, j + jb, N, (int)j, (int)(j + jb), s->ipiv_dev.p);
+      hipLaunchKernelGGL(k_laswp<T>, dim3((unsigned)((j + 255) / 256)), dim3(256), 0, c.st, c.W, c.ld, 0LL,
+                         j, (int)j, (int)nj, s->ipiv_dev.p);
+    if (nj < N) {
+      hipLaunchKernelGGL(k_laswp<T>, dim3((unsigned)((N - nj + 255) / 256)), dim3(256), 0, c.st, c.W, c.ld, nj, N,
+                         (int)j, (int)nj, s->ipiv_dev.p);
       QD_HIP(c.h, hipGetLastError());
-      if ((rc = trsm_lower(c, j, jb, j + jb, N))) return rc;
-      if ((rc = gemm_sub(c, N - j - jb, N - j - jb, jb, c.W + (j + jb) * c.ld + j, c.W + j * c.ld + j + jb,
-                         c.W + (j + jb) * c.ld + j + jb, true, true)))
-        return rc;
+      if ((rc = trsm_lower(c, j, jb, nj, N))) return rc;
+      if (look && nj + njb < N) {
+        if ((rc = update_cols(c, j, jb, nj, nj + njb, false))) return rc;
+        QD_HIP(c.h, hipEventRecord(s->ev_cols, c.st));
+        QD_HIP(c.h, hipStreamWaitEvent(cp.st, s->ev_cols, 0));
+        if ((rc = panel_step(cp, nj, njb))) return rc;
+        QD_HIP(c.h, hipEventRecord(s->ev_panel, cp.st));
+        if ((rc = update_cols(c, j, jb, nj + njb, N, true))) return rc;
+        QD_HIP(c.h, hipStreamWaitEvent(c.st, s->ev_panel, 0));
+      } else {
+        if ((rc = update_cols(c, j, jb, nj, N, false))) return rc;
+        if (njb > 0 && (rc = panel_step(c, nj, njb))) return rc;
+      }
     }
   }
   return QD_OK;
@@ -1610,14 +1677,14 @@ static int ensure_factors(qd_context* h, int precision, qd_solve_report* rep) {
     QD_ALLOC(h, s->W64, (size_t)ld * (size_t)N);
     hipLaunchKernelGGL(k_transpose_in<double>, tg, dim3(256), 0, h->stream, N, h->A.p, h->lda, s->W64.p, ld);
     QD_HIP(h, hipGetLastError());
-    LuCtx<double> c{h, s, s->W64.p, ld, N, false, false};
+    LuCtx<double> c{h, s, s->W64.p, ld, N, false, false, h->stream, &s->sc_main};
     rc = factor(c, NB);
   } else {
     s->W64.release();
     QD_ALLOC(h, s->W32, (size_t)ld * (size_t)N);
     hipLaunchKernelGGL(k_transpose_in<float>, tg, dim3(256), 0, h->stream, N, h->A.p, h->lda, s->W32.p, ld);
     QD_HIP(h, hipGetLastError());
-    LuCtx<float> c{h, s, s->W32.p, ld, N, precision == QD_FP16_IR, true};
+    LuCtx<float> c{h, s, s->W32.p, ld, N, precision == QD_FP16_IR, true, h->stream, &s->sc_main};
     rc = factor(c, NB);
   }
   if (rc) return rc;
@@ -2007,7 +2074,7 @@ int dgesv_device(qd_context* h, int n, int nrhs, double* a, long long lda, int* 
   dim3 tg((unsigned)((N + 31) / 32), (unsigned)((N + 31) / 32));
   hipLaunchKernelGGL(k_transpose_in<double>, tg, dim3(256), 0, h->stream, N, a, lda, s->W64.p, ld);
   QD_HIP(h, hipGetLastError());
-  LuCtx<double> c{h, s, s->W64.p, ld, N, false, false};
+  LuCtx<double> c{h, s, s->W64.p, ld, N, false, false, h->stream, &s->sc_main};
   int rc = factor(c, 512);
   if (rc) return rc;
   std::vector<int> ip(N);
