@@ -376,23 +376,33 @@ __global__ __launch_bounds__(256) void k_gemm_sub(long long M, long long N, int 
         sB[idx / GBN][idx % GBN] = vb[i];
       }
     } else {
-      // A tile: GBM rows x BK cols
+      // edge tile: every load predicated, all issued before the LDS stores
+      T va[GBM * BK / 256], vb[GBN * BK / 256];
 #pragma unroll
       for (int i = 0; i < GBM * BK / 256; ++i) {
         const int idx = tid + 256 * i;
-        const int r = idx / BK, c = idx % BK;
-        const long long gr = m0 + r;
-        const int gc = k0 + c;
-        sA[c][r] = (gr < M && gc < K) ? A[gr * lda + gc] : (T)0;
+        const long long gr = m0 + idx / BK;
+        const int gc = k0 + idx % BK;
+        va[i] = (T)0;
+        if (gr < M && gc < K) va[i] = A[gr * lda + gc];
       }
-      // B tile: BK rows x GBN cols
 #pragma unroll
       for (int i = 0; i < GBN * BK / 256; ++i) {
         const int idx = tid + 256 * i;
-        const int r = idx / GBN, c = idx % GBN;
-        const int gr = k0 + r;
-        const long long gc = n0 + c;
-        sB[r][c] = (gr < K && gc < N) ? B[(long long)gr * ldb + gc] : (T)0;
+        const int gr = k0 + idx / GBN;
+        const long long gc = n0 + idx % GBN;
+        vb[i] = (T)0;
+        if (gr < K && gc < N) vb[i] = B[(long long)gr * ldb + gc];
+      }
+#pragma unroll
+      for (int i = 0; i < GBM * BK / 256; ++i) {
+        const int idx = tid + 256 * i;
+        sA[idx % BK][idx / BK] = va[i];
+      }
+#pragma unroll
+      for (int i = 0; i < GBN * BK / 256; ++i) {
+        const int idx = tid + 256 * i;
+        sB[idx / GBN][idx % GBN] = vb[i];
       }
     }
     __syncthreads();
@@ -426,16 +436,27 @@ __global__ __launch_bounds__(256) void k_gemm_sub(long long M, long long N, int 
     }
     return;
   }
+  // edge tile: predicated loads of the column group first, then the stores
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int j = 0; j < 4; ++j) {
+    const long long gc = n0 + wn + 16 * j + lr;
+    T cv[4][4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const long long gr = m0 + wm + 16 * i + MT::out_row(lk, r);
-        const long long gc = n0 + wn + 16 * j + lr;
-        if (gr < M && gc < N) C[gr * ldc + gc] -= acc[i][j][r];
+        cv[i][r] = (T)0;
+        if (gr < M && gc < N) cv[i][r] = C[gr * ldc + gc];
       }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long long gr = m0 + wm + 16 * i + MT::out_row(lk, r);
+        if (gr < M && gc < N) C[gr * ldc + gc] = cv[i][r] - acc[i][j][r];
+      }
+  }
 }
 
 // FP64 C[M x N] -= A[M x K] B[K x N] (row-major, inside W) with MFMA
@@ -522,16 +543,27 @@ __global__ __launch_bounds__(256, 2) void k_gemm64(long long M, long long N, int
     }
     return;
   }
+  // edge tile: predicated loads of the column group first, then the stores
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int j = 0; j < 4; ++j) {
+    const long long gc = n0 + wn + 16 * j + lr;
+    double cv[4][4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const long long gr = m0 + wm + 16 * i + lk + 4 * r;
-        const long long gc = n0 + wn + 16 * j + lr;
-        if (gr < M && gc < N) C[gr * ldc + gc] -= acc[i][j][r];
+        cv[i][r] = 0.0;
+        if (gr < M && gc < N) cv[i][r] = C[gr * ldc + gc];
       }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long long gr = m0 + wm + 16 * i + lk + 4 * r;
+        if (gr < M && gc < N) C[gr * ldc + gc] = cv[i][r] - acc[i][j][r];
+      }
+  }
 }
 
 // Per-64-column maxima of |U12| (fp16 mode): colscale[b] = max |B[:, 64b..64b+64)|.
@@ -592,20 +624,22 @@ __global__ __launch_bounds__(256) void k_cvt_b16t(int K, long long N, int Kp, co
 // C[M x N] (fp32, row-major) -= colscale[n/64] * (A16 B16^T).  256 x 128 tile
 // per 256-thread workgroup, 4 waves of 128 x 64 (8 x 4 MFMA 16x16x32 tiles),
 // K steps of 32 through double-buffered LDS with register prefetch.
-// Workgroups are remapped so that each XCD receives a contiguous band of
-// tiles (its L2 then serves the shared A/B rows).
+// Each workgroup loops over tiles.  The tiles are split into 8 contiguous
+// bands, one per XCD (workgroups are dispatched to the XCDs round-robin), so
+// an XCD's L2 serves the A/B rows its tiles share.  A grid smaller than the
+// tile count leaves CU capacity to work on another stream.
 constexpr int H_BM = 256, H_BN = 128, H_BK = 32, H_PAD = 8;
-template <int kRole>
-__global__ __launch_bounds__(256, 2) void k_gemm16(long long M, long long N, int Kp, const _Float16* __restrict__ A16,
-                                                const _Float16* __restrict__ B16, const float* __restrict__ colscale,
-                                                float* __restrict__ C, long long ldc) {
-  __shared__ __attribute__((aligned(16))) _Float16 sA[2][H_BM][H_BK + H_PAD];
-  __shared__ __attribute__((aligned(16))) _Float16 sB[2][H_BN][H_BK + H_PAD];
-  const unsigned total = gridDim.x * gridDim.y;
-  const unsigned bid = blockIdx.y * gridDim.x + blockIdx.x;
-  const unsigned per_xcd = total / 8;
-  const unsigned vid = (bid < per_xcd * 8) ? (bid % 8) * per_xcd + bid / 8 : bid;
-  const long long m0 = (long long)(vid / gridDim.x) * H_BM, n0 = (long long)(vid % gridDim.x) * H_BN;
+struct G16Smem {
+  _Float16 sA[2][H_BM][H_BK + H_PAD];
+  _Float16 sB[2][H_BN][H_BK + H_PAD];
+};
+
+__device__ __forceinline__ void gemm16_tile(long long m0, long long n0, long long M, long long N, int Kp,
+                                            const _Float16* __restrict__ A16, const _Float16* __restrict__ B16,
+                                            const float* __restrict__ colscale, float* __restrict__ C, long long ldc,
+                                            G16Smem& sm) {
+  auto& sA = sm.sA;
+  auto& sB = sm.sB;
   const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
   const int wm = (wave / 2) * 128, wn = (wave % 2) * 64;
   const int lr = lane % 16, lk = lane / 16;
@@ -615,7 +649,6 @@ __global__ __launch_bounds__(256, 2) void k_gemm16(long long M, long long N, int
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = (f4){0, 0, 0, 0};
   uint4 ra[4], rb[2];
-#ifndef QD_GEMM16_ROWPERLANE
   // four lanes per 64-byte row segment: one instruction covers 16 rows
   const int lrow = tid >> 2, lchunk = (tid & 3) * 8;
   auto gload = [&](int k0) {
@@ -636,36 +669,6 @@ __global__ __launch_bounds__(256, 2) void k_gemm16(long long M, long long N, int
 #pragma unroll
     for (int q = 0; q < 2; ++q) *(uint4*)&sB[buf][lrow + 64 * q][lchunk] = rb[q];
   };
-#else
-  const long long ga = m0 + tid;            // A row loaded by this thread
-  const long long gb = n0 + tid / 2;        // B row
-  const int boff = (tid % 2) * 16;
-  auto gload = [&](int k0) {
-    if (ga < M) {
-      const uint4* p = (const uint4*)(A16 + ga * Kp + k0);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) ra[q] = p[q];
-    } else {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) ra[q] = make_uint4(0, 0, 0, 0);
-    }
-    if (gb < N) {
-      const uint4* p = (const uint4*)(B16 + gb * Kp + k0 + boff);
-      rb[0] = p[0];
-      rb[1] = p[1];
-    } else {
-      rb[0] = rb[1] = make_uint4(0, 0, 0, 0);
-    }
-  };
-  auto lstore = [&](int buf) {
-    uint4* da = (uint4*)&sA[buf][tid][0];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) da[q] = ra[q];
-    uint4* db = (uint4*)&sB[buf][tid / 2][boff];
-    db[0] = rb[0];
-    db[1] = rb[1];
-  };
-#endif
   gload(0);
   lstore(0);
   __syncthreads();
@@ -705,17 +708,44 @@ __global__ __launch_bounds__(256, 2) void k_gemm16(long long M, long long N, int
     }
     return;
   }
+  // edge tile: predicated loads of the column group first, then the stores
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const long long gc = n0 + wn + 16 * j + lr;
     const float s = (gc < N) ? colscale[gc >> 6] : 0.0f;
+    float cv[8][4];
 #pragma unroll
     for (int i = 0; i < 8; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const long long gr = m0 + wm + 16 * i + 4 * lk + r;
-        if (gr < M && gc < N) C[gr * ldc + gc] -= s * acc[i][j][r];
+        cv[i][r] = 0.0f;
+        if (gr < M && gc < N) cv[i][r] = C[gr * ldc + gc];
       }
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long long gr = m0 + wm + 16 * i + 4 * lk + r;
+        if (gr < M && gc < N) C[gr * ldc + gc] = cv[i][r] - s * acc[i][j][r];
+      }
+  }
+}
+
+template <int kRole>
+__global__ __launch_bounds__(256, 2) void k_gemm16(long long M, long long N, int Kp, const _Float16* __restrict__ A16,
+                                                const _Float16* __restrict__ B16, const float* __restrict__ colscale,
+                                                float* __restrict__ C, long long ldc) {
+  __shared__ __attribute__((aligned(16))) G16Smem sm;
+  const unsigned tiles_n = (unsigned)((N + H_BN - 1) / H_BN);
+  const unsigned total = (unsigned)((M + H_BM - 1) / H_BM) * tiles_n;
+  const unsigned per_xcd = (total + 7) / 8;
+  const unsigned xcd = blockIdx.x % 8, lw = blockIdx.x / 8, wx = (gridDim.x + 7 - xcd) / 8;
+  for (unsigned i = lw; i < per_xcd; i += wx) {
+    const unsigned vid = xcd * per_xcd + i;
+    if (vid >= total) break;
+    gemm16_tile((long long)(vid / tiles_n) * H_BM, (long long)(vid % tiles_n) * H_BN, M, N, Kp, A16, B16, colscale,
+                C, ldc, sm);
   }
 }
 
@@ -999,9 +1029,11 @@ constexpr double kF16MinWork = 2.0e9;
 
 // reuse_a16: A's FP16 copy is already in c.sc->h16a (the same A as the
 // previous call on this context's stream).
+// max_wgs > 0 caps the workgroups of the FP16 kernel (each then loops over
+// several tiles).
 template <typename T>
 static int gemm_sub(LuCtx<T>& c, long long M, long long Nn, int K, const T* A, const T* B, T* C, bool allow_f16,
-                    bool trailing = false, bool reuse_a16 = false) {
+                    bool trailing = false, bool reuse_a16 = false, int max_wgs = 0) {
   if (M <= 0 || Nn <= 0 || K <= 0) return QD_OK;
   dim3 grid((unsigned)((Nn + GBN - 1) / GBN), (unsigned)((M + GBM - 1) / GBM));
   if constexpr (std::is_same<T, float>::value) {
@@ -1020,7 +1052,8 @@ static int gemm_sub(LuCtx<T>& c, long long M, long long Nn, int K, const T* A, c
       if (!reuse_a16)
         hipLaunchKernelGGL(k_cvt_a16, dim3((unsigned)((M * (Kp / 8) + 255) / 256)), dim3(256), 0, c.st, M, K,
                            Kp, A, c.ld, c.sc->h16a.p);
-      dim3 g16((unsigned)((Nn + H_BN - 1) / H_BN), (unsigned)((M + H_BM - 1) / H_BM));
+      const long long tiles = ((Nn + H_BN - 1) / H_BN) * ((M + H_BM - 1) / H_BM);
+      const dim3 g16((unsigned)(max_wgs > 0 ? std::min<long long>(tiles, max_wgs) : tiles));
       ProfScope ps(c.h, trailing ? QD_PROF_LU_TRAILING_GEMM : -1, 2.0 * (double)M * (double)Nn * K, c.st);
       if (trailing)
         hipLaunchKernelGGL(k_gemm16<1>, g16, dim3(256), 0, c.st, M, Nn, Kp, c.sc->h16a.p, c.sc->h16b.p,
@@ -1149,10 +1182,20 @@ static int panel_step(LuCtx<T>& c, long long j, int jb) {
 
 // A22[:, c0:c1) -= L21 U12[:, c0:c1) for the step whose panel is [j, j+jb).
 template <typename T>
-static int update_cols(LuCtx<T>& c, long long j, int jb, long long c0, long long c1, bool reuse_a16) {
+static int update_cols(LuCtx<T>& c, long long j, int jb, long long c0, long long c1, bool reuse_a16,
+                       int max_wgs = 0) {
   const long long r0 = j + jb;
   return gemm_sub(c, c.N - r0, c1 - c0, jb, c.W + r0 * c.ld + j, c.W + j * c.ld + c0, c.W + r0 * c.ld + c0, true,
-                  true, reuse_a16);
+                  true, reuse_a16, max_wgs);
+}
+
+// Workgroup cap of the trailing update that runs beside a panel
+// (QUADESTH_GEMM_WGS; 0 = one workgroup per tile).  Measured on MI355X at
+// 76,806 DOF, caps of 320-512 did not shorten the factorisation: the panel
+// beside the update is slowed by the memory system, not by CU slots.
+static int lookahead_gemm_wgs() {
+  if (const char* e = std::getenv("QUADESTH_GEMM_WGS")) return std::max(0, std::atoi(e));
+  return 0;
 }
 
 // Blocked right-looking LU with a look-ahead of one panel: each step updates
@@ -1214,7 +1257,7 @@ static int factor(LuCtx<T>& c, int NB) {
         QD_HIP(c.h, hipStreamWaitEvent(cp.st, s->ev_cols, 0));
         if ((rc = panel_step(cp, nj, njb))) return rc;
         QD_HIP(c.h, hipEventRecord(s->ev_panel, cp.st));
-        if ((rc = update_cols(c, j, jb, nj + njb, N, true))) return rc;
+        if ((rc = update_cols(c, j, jb, nj + njb, N, true, lookahead_gemm_wgs()))) return rc;
         QD_HIP(c.h, hipStreamWaitEvent(c.st, s->ev_panel, 0));
       } else {
         if ((rc = update_cols(c, j, jb, nj, N, false))) return rc;
