@@ -276,9 +276,20 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_trsm_small(T* __restrict__ W, long long ld, long long r0, int m,
                                                     long long c0, long long c1) {
   __shared__ T sL[32][33];
-  for (int i = threadIdx.x; i < m * m; i += 256) {
-    const int a = i / m, b = i % m;
-    sL[a][b] = W[(r0 + a) * ld + r0 + b];
+  {
+    // the (at most 4) loads of this thread first, then the LDS stores
+    T v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = threadIdx.x + 256 * q;
+      v[q] = (T)0;
+      if (i < m * m) v[q] = W[(r0 + i / m) * ld + r0 + i % m];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = threadIdx.x + 256 * q;
+      if (i < m * m) sL[i / m][i % m] = v[q];
+    }
   }
   __syncthreads();
   const long long c = c0 + (long long)blockIdx.x * 256 + threadIdx.x;
@@ -1431,36 +1442,47 @@ __global__ __launch_bounds__(256) void k_block_update_rows(const T* __restrict__
     for (int e = 0; e < E; ++e) xr[v][e] = (v < nrhs && col + e < m) ? x[(long long)v * N + c0 + col + e] : (T)0;
   const bool full = (m == TB);
   const long long rbase = r0 + (long long)blockIdx.x * kBuRows;
-  for (int pass = 0; pass < kBuRows / 8; ++pass) {
+  constexpr int kPass = kBuRows / 8;
+  // all rows' loads first, then the reductions
+  T w[kPass][E];
+#pragma unroll
+  for (int pass = 0; pass < kPass; ++pass) {
     const long long r = rbase + pass * 8 + slot;
-    if (r >= r1) break;
     const T* wr = W + r * ld + c0 + col;
-    T w[E];
-    if (full) {
-      if constexpr (sizeof(T) == 4) {
-        const float4 q = *reinterpret_cast<const float4*>(wr);
-        w[0] = q.x;
-        w[1] = q.y;
-        w[2] = q.z;
-        w[3] = q.w;
+#pragma unroll
+    for (int e = 0; e < E; ++e) w[pass][e] = (T)0;
+    if (r < r1) {
+      if (full) {
+        if constexpr (sizeof(T) == 4) {
+          const float4 q = *reinterpret_cast<const float4*>(wr);
+          w[pass][0] = q.x;
+          w[pass][1] = q.y;
+          w[pass][2] = q.z;
+          w[pass][3] = q.w;
+        } else {
+          const double2 q = *reinterpret_cast<const double2*>(wr);
+          w[pass][0] = q.x;
+          w[pass][1] = q.y;
+        }
       } else {
-        const double2 q = *reinterpret_cast<const double2*>(wr);
-        w[0] = q.x;
-        w[1] = q.y;
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+          if (col + e < m) w[pass][e] = wr[e];
       }
-    } else {
-#pragma unroll
-      for (int e = 0; e < E; ++e) w[e] = (col + e < m) ? wr[e] : (T)0;
     }
+  }
 #pragma unroll
-    for (int v = 0; v < kBuMaxRhs; ++v) {
-      if (v < nrhs) {
+  for (int v = 0; v < kBuMaxRhs; ++v) {
+    if (v < nrhs) {
+#pragma unroll
+      for (int pass = 0; pass < kPass; ++pass) {
         T sum = 0;
 #pragma unroll
-        for (int e = 0; e < E; ++e) sum += w[e] * xr[v][e];
+        for (int e = 0; e < E; ++e) sum += w[pass][e] * xr[v][e];
 #pragma unroll
         for (int off = 16; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 32);
-        if (sub == 0) x[(long long)v * N + r] -= sum;
+        const long long r = rbase + pass * 8 + slot;
+        if (sub == 0 && r < r1) x[(long long)v * N + r] -= sum;
       }
     }
   }
