@@ -874,20 +874,28 @@ __global__ __launch_bounds__(64) void k_tslu_select(const float* __restrict__ W,
   const int lane = threadIdx.x;
   float a[kTsRPT][kTsW];
   int gidx[kTsRPT];
+  // row numbers first, then all row pieces (an absent row reads row c0 and
+  // is zeroed), so the loads are in flight together
 #pragma unroll
   for (int q = 0; q < kTsRPT; ++q) {
     const long long li = (long long)blockIdx.x * kTsRows + lane + 64 * q;
-    long long g;
-    if (in_idx) g = (li < n_in) ? in_idx[li] : -1;
-    else g = (c0 + li < N) ? c0 + li : -1;
-    gidx[q] = (int)g;
-    if (g >= 0) {
-      ts_load_row(W, ld, g, c0, w, a[q]);
-    } else {
-#pragma unroll
-      for (int j = 0; j < kTsW; ++j) a[q][j] = 0.0f;
+    long long g = -1;
+    if (in_idx) {
+      if (li < n_in) g = in_idx[li];
+    } else if (c0 + li < N) {
+      g = c0 + li;
     }
+    gidx[q] = (int)g;
   }
+#pragma unroll
+  for (int q = 0; q < kTsRPT; ++q) {
+    const long long g = gidx[q] >= 0 ? gidx[q] : c0;
+    ts_load_row(W, ld, g, c0, w, a[q]);
+  }
+#pragma unroll
+  for (int q = 0; q < kTsRPT; ++q)
+#pragma unroll
+    for (int j = 0; j < kTsW; ++j) a[q][j] = (gidx[q] >= 0) ? a[q][j] : 0.0f;
   bool done = false;
   ts_all(std::make_integer_sequence<int, kTsW>{}, a, gidx, done, w, srow, out_idx);
 }
@@ -1003,9 +1011,19 @@ __global__ __launch_bounds__(256) void k_tslu_finish(float* __restrict__ W, long
   __shared__ float sU[kTsW][kTsW];
   __shared__ float sR[kTsW];
   const int tid = threadIdx.x;
-  for (int i = tid; i < kTsW * kTsW; i += 256) {
-    const int r = i / kTsW, j = i % kTsW;
-    sU[r][j] = (r < w && j < w && j >= r) ? W[(c0 + r) * ld + c0 + j] : 0.0f;
+  {
+    float v[kTsW * kTsW / 256];
+#pragma unroll
+    for (int q = 0; q < kTsW * kTsW / 256; ++q) {
+      const int i = tid + 256 * q, r = i / kTsW, j = i % kTsW;
+      v[q] = 0.0f;
+      if (r < w && j < w && j >= r) v[q] = W[(c0 + r) * ld + c0 + j];
+    }
+#pragma unroll
+    for (int q = 0; q < kTsW * kTsW / 256; ++q) {
+      const int i = tid + 256 * q;
+      sU[i / kTsW][i % kTsW] = v[q];
+    }
   }
   __syncthreads();
   if (tid < kTsW) {
