@@ -65,6 +65,7 @@ struct qd_lu_state {
   qd::DevBuf<double> gm;         // GMRES basis
   qd::DevBuf<double> rowpart;    // residual slice partials
   qd::DevBuf<int> ts_a, ts_b;    // tournament candidate lists
+  qd::DevBuf<float> ts_lu;       // LU of a leaf's winners block (32 x 32)
   qd::DevBuf<float> inv32;       // inverted diagonal blocks of L and U
   qd::DevBuf<double> inv64;
   double anorm = -1.0;           // ||A||_inf of the FP64 matrix
@@ -809,7 +810,8 @@ __device__ __forceinline__ unsigned long long wave_max64(unsigned long long v) {
 // register arrays are indexed statically.
 template <int K>
 __device__ __forceinline__ void ts_step(float (&a)[kTsRPT][kTsW], int (&gidx)[kTsRPT], bool& done, int w,
-                                        float* __restrict__ srow, int* __restrict__ out_idx) {
+                                        float* __restrict__ srow, int* __restrict__ out_idx,
+                                        float* __restrict__ lu_out) {
   const int lane = threadIdx.x;
   done = done || (K >= w);
   unsigned long long key = 0;
@@ -832,11 +834,15 @@ __device__ __forceinline__ void ts_step(float (&a)[kTsRPT][kTsW], int (&gidx)[kT
   done = !found;
   __syncthreads();
   if (found) {
+    // the winner of step K, eliminated by the K earlier winners with its
+    // multipliers kept in place, is row K of the winners' LU
+    if (lu_out && lane < kTsW) lu_out[K * kTsW + lane] = srow[lane];
     const float piv = srow[K];
     const float rp = (piv != 0.0f) ? 1.0f / piv : 0.0f;
 #pragma unroll
     for (int q = 0; q < kTsRPT; ++q) {
       const float l = (gidx[q] >= 0) ? a[q][K] * rp : 0.0f;
+      if (gidx[q] >= 0) a[q][K] = l;
 #pragma unroll
       for (int j = K + 1; j < kTsW; ++j) a[q][j] -= l * srow[j];
     }
@@ -847,8 +853,8 @@ __device__ __forceinline__ void ts_step(float (&a)[kTsRPT][kTsW], int (&gidx)[kT
 template <int... Ks>
 __device__ __forceinline__ void ts_all(std::integer_sequence<int, Ks...>, float (&a)[kTsRPT][kTsW],
                                        int (&gidx)[kTsRPT], bool& done, int w, float* __restrict__ srow,
-                                       int* __restrict__ out_idx) {
-  (ts_step<Ks>(a, gidx, done, w, srow, out_idx), ...);
+                                       int* __restrict__ out_idx, float* __restrict__ lu_out) {
+  (ts_step<Ks>(a, gidx, done, w, srow, out_idx, lu_out), ...);
 }
 
 // Loads the 32-column segment [c0, c0+32) of row g (zero beyond column w).
@@ -867,9 +873,11 @@ __device__ __forceinline__ void ts_load_row(const float* __restrict__ W, long lo
   }
 }
 
+// lu_out (final level only, one workgroup): the winners' LU, w x w, row-major
+// with stride 32.
 __global__ __launch_bounds__(64) void k_tslu_select(const float* __restrict__ W, long long ld, long long N,
                                                     long long c0, int w, const int* __restrict__ in_idx, int n_in,
-                                                    int* __restrict__ out_idx) {
+                                                    int* __restrict__ out_idx, float* __restrict__ lu_out) {
   __shared__ float srow[kTsW];
   const int lane = threadIdx.x;
   float a[kTsRPT][kTsW];
@@ -897,84 +905,70 @@ __global__ __launch_bounds__(64) void k_tslu_select(const float* __restrict__ W,
 #pragma unroll
     for (int j = 0; j < kTsW; ++j) a[q][j] = (gidx[q] >= 0) ? a[q][j] : 0.0f;
   bool done = false;
-  ts_all(std::make_integer_sequence<int, kTsW>{}, a, gidx, done, w, srow, out_idx);
+  ts_all(std::make_integer_sequence<int, kTsW>{}, a, gidx, done, w, srow, out_idx, lu_out);
 }
 
-// LAPACK interchange sequence that brings the winners p_0..p_{w-1} to rows
-// c0..c0+w-1 in order: ipiv[c0+k] = current position of p_k.
-__global__ void k_tslu_ipiv(const int* __restrict__ win, int w, long long c0, int* __restrict__ ipiv) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  int prow[2 * kTsW], ppos[2 * kTsW];  // row -> position, for rows that moved
-  int np = 0;
-  auto pos_of = [&](int row) {
-    for (int i = 0; i < np; ++i)
-      if (prow[i] == row) return ppos[i];
-    return row;
-  };
-  auto set_pos = [&](int row, int pos) {
-    for (int i = 0; i < np; ++i)
-      if (prow[i] == row) {
-        ppos[i] = pos;
-        return;
+// The LAPACK interchange sequence that brings the winners p_0..p_{w-1} to
+// rows c0..c0+w-1 in order (ipiv[c0+k] = current position of p_k), then the
+// interchanges on columns [p0, p1).  Thread 0 of every workgroup forms the
+// sequence in LDS (about w^2 steps); workgroup 0 also stores it in ipiv.
+__global__ __launch_bounds__(256) void k_tslu_swap(float* __restrict__ W, long long ld, long long p0, long long p1,
+                                                   const int* __restrict__ win, int w, long long c0,
+                                                   int* __restrict__ ipiv) {
+  __shared__ int seq[kTsW];
+  if (threadIdx.x == 0) {
+    int prow[2 * kTsW], ppos[2 * kTsW];  // row -> position, for rows that moved
+    int np = 0;
+    auto pos_of = [&](int row) {
+      for (int i = 0; i < np; ++i)
+        if (prow[i] == row) return ppos[i];
+      return row;
+    };
+    auto set_pos = [&](int row, int pos) {
+      for (int i = 0; i < np; ++i)
+        if (prow[i] == row) {
+          ppos[i] = pos;
+          return;
+        }
+      prow[np] = row;
+      ppos[np] = pos;
+      ++np;
+    };
+    auto row_at = [&](int pos) {
+      for (int i = 0; i < np; ++i)
+        if (ppos[i] == pos) return prow[i];
+      return pos;  // an unmoved row sits at its own index
+    };
+    for (int k = 0; k < w; ++k) {
+      const int t = (int)c0 + k;
+      const int p = win[k];
+      const int cur = pos_of(p);
+      seq[k] = cur;
+      if (blockIdx.x == 0) ipiv[t] = cur;
+      if (cur != t) {
+        const int r_t = row_at(t);
+        set_pos(p, t);
+        set_pos(r_t, cur);
       }
-    prow[np] = row;
-    ppos[np] = pos;
-    ++np;
-  };
-  auto row_at = [&](int pos) {
-    for (int i = 0; i < np; ++i)
-      if (ppos[i] == pos) return prow[i];
-    return pos;  // an unmoved row sits at its own index
-  };
+    }
+  }
+  __syncthreads();
+  const long long c = p0 + (long long)blockIdx.x * 256 + threadIdx.x;
+  if (c >= p1) return;
   for (int k = 0; k < w; ++k) {
-    const int t = (int)c0 + k;
-    const int p = win[k];
-    const int cur = pos_of(p);
-    ipiv[t] = cur;
-    if (cur != t) {
-      const int r_t = row_at(t);
-      set_pos(p, t);
-      set_pos(r_t, cur);
+    const long long t = c0 + k, p = seq[k];
+    if (p != t) {
+      const float v = W[t * ld + c];
+      W[t * ld + c] = W[p * ld + c];
+      W[p * ld + c] = v;
     }
   }
 }
 
-// After the interchanges: k_tslu_top factors the top w x w block (the winners
-// in pivot order, no further pivoting) in place with one wave: lane r holds
-// row r and the pivot row of step k goes through LDS.  k_tslu_finish then
-// forms L21 = A21 U11^-1, one row per thread, against U11 read from LDS by
-// broadcast.  The top block is written only by k_tslu_top, so no workgroup of
-// k_tslu_finish can see it half-factored.
-template <int K>
-__device__ __forceinline__ void tf_step(float (&u)[kTsW], int w, float* __restrict__ prow) {
-  const int r = threadIdx.x;
-  if (K < w) {
-    if (r == K) {
-#pragma unroll
-      for (int j = 0; j < kTsW; ++j) prow[j] = u[j];
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (r > K && r < w) {
-      const float piv = prow[K];
-      const float l = (piv != 0.0f) ? u[K] / piv : 0.0f;
-      u[K] = l;
-#pragma unroll
-      for (int j = K + 1; j < kTsW; ++j) u[j] -= l * prow[j];
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  }
-}
-
-template <int... Ks>
-__device__ __forceinline__ void tf_all(std::integer_sequence<int, Ks...>, float (&u)[kTsW], int w,
-                                       float* __restrict__ prow) {
-  (tf_step<Ks>(u, w, prow), ...);
-}
-
+// After the interchanges: L21 = A21 U11^-1, one row per thread, against U11
+// read from LDS by broadcast.  U11 (with L11 below its diagonal) comes from
+// the final tournament level's side buffer, not from W, and workgroup 0 copies
+// that block into W's top rows: no workgroup reads the block from W here.
 template <int K>
 __device__ __forceinline__ void tf_solve_step(float (&x)[kTsW], int w, const float (*__restrict__ sU)[kTsW],
                                               const float* __restrict__ sR) {
@@ -991,23 +985,8 @@ __device__ __forceinline__ void tf_solve(std::integer_sequence<int, Ks...>, floa
   (tf_solve_step<Ks>(x, w, sU, sR), ...);
 }
 
-__global__ __launch_bounds__(64) void k_tslu_top(float* __restrict__ W, long long ld, long long c0, int w) {
-  __shared__ float prow[kTsW];
-  const int tid = threadIdx.x;
-  float u[kTsW];
-  if (tid < w) {
-    ts_load_row(W, ld, c0 + tid, c0, w, u);
-  } else {
-#pragma unroll
-    for (int j = 0; j < kTsW; ++j) u[j] = 0.0f;
-  }
-  tf_all(std::make_integer_sequence<int, kTsW>{}, u, w, prow);
-  if (tid < w)
-    for (int j = 0; j < w; ++j) W[(c0 + tid) * ld + c0 + j] = u[j];
-}
-
 __global__ __launch_bounds__(256) void k_tslu_finish(float* __restrict__ W, long long ld, long long N, long long c0,
-                                                     int w) {
+                                                     int w, const float* __restrict__ lu_in) {
   __shared__ float sU[kTsW][kTsW];
   __shared__ float sR[kTsW];
   const int tid = threadIdx.x;
@@ -1017,12 +996,13 @@ __global__ __launch_bounds__(256) void k_tslu_finish(float* __restrict__ W, long
     for (int q = 0; q < kTsW * kTsW / 256; ++q) {
       const int i = tid + 256 * q, r = i / kTsW, j = i % kTsW;
       v[q] = 0.0f;
-      if (r < w && j < w && j >= r) v[q] = W[(c0 + r) * ld + c0 + j];
+      if (r < w && j < w) v[q] = lu_in[i];
     }
 #pragma unroll
     for (int q = 0; q < kTsW * kTsW / 256; ++q) {
-      const int i = tid + 256 * q;
-      sU[i / kTsW][i % kTsW] = v[q];
+      const int i = tid + 256 * q, r = i / kTsW, j = i % kTsW;
+      sU[r][j] = (j >= r) ? v[q] : 0.0f;
+      if (blockIdx.x == 0 && r < w && j < w) W[(c0 + r) * ld + c0 + j] = v[q];
     }
   }
   __syncthreads();
@@ -1152,26 +1132,25 @@ static int tslu_leaf(LuCtx<float>& c, long long c0, int w, long long p0, long lo
   int nb = (int)((m + kTsRows - 1) / kTsRows);
   QD_ALLOC(c.h, s->ts_a, (size_t)nb * kTsW);
   QD_ALLOC(c.h, s->ts_b, (size_t)nb * kTsW);
+  QD_ALLOC(c.h, s->ts_lu, (size_t)kTsW * kTsW);
   int* in = s->ts_a.p;
   int* out = s->ts_b.p;
-  hipLaunchKernelGGL(k_tslu_select, dim3(nb), dim3(64), 0, c.st, c.W, c.ld, c.N, c0, w, (const int*)nullptr,
-                     0, in);
+  hipLaunchKernelGGL(k_tslu_select, dim3(nb), dim3(64), 0, c.st, c.W, c.ld, c.N, c0, w, (const int*)nullptr, 0, in,
+                     nb == 1 ? s->ts_lu.p : (float*)nullptr);
   int n_in = nb * kTsW;
   while (nb > 1) {
     const int nb2 = (n_in + kTsRows - 1) / kTsRows;
-    hipLaunchKernelGGL(k_tslu_select, dim3(nb2), dim3(64), 0, c.st, c.W, c.ld, c.N, c0, w, (const int*)in,
-                       n_in, out);
+    hipLaunchKernelGGL(k_tslu_select, dim3(nb2), dim3(64), 0, c.st, c.W, c.ld, c.N, c0, w, (const int*)in, n_in,
+                       out, nb2 == 1 ? s->ts_lu.p : (float*)nullptr);
     n_in = nb2 * kTsW;
     std::swap(in, out);
     nb = nb2;
   }
-  hipLaunchKernelGGL(k_tslu_ipiv, dim3(1), dim3(1), 0, c.st, (const int*)in, w, c0, s->ipiv_dev.p);
-  hipLaunchKernelGGL(k_laswp<float>, dim3((unsigned)((p1 - p0 + 255) / 256)), dim3(256), 0, c.st, c.W, c.ld,
-                     p0, p1, (int)c0, (int)(c0 + w), s->ipiv_dev.p);
-  hipLaunchKernelGGL(k_tslu_top, dim3(1), dim3(64), 0, c.st, c.W, c.ld, c0, w);
+  hipLaunchKernelGGL(k_tslu_swap, dim3((unsigned)((p1 - p0 + 255) / 256)), dim3(256), 0, c.st, c.W, c.ld, p0, p1,
+                     (const int*)in, w, c0, s->ipiv_dev.p);
   const long long below = std::max<long long>(m - w, 1);
-  hipLaunchKernelGGL(k_tslu_finish, dim3((unsigned)((below + 255) / 256)), dim3(256), 0, c.st, c.W, c.ld,
-                     c.N, c0, w);
+  hipLaunchKernelGGL(k_tslu_finish, dim3((unsigned)((below + 255) / 256)), dim3(256), 0, c.st, c.W, c.ld, c.N, c0,
+                     w, (const float*)s->ts_lu.p);
   QD_HIP(c.h, hipGetLastError());
   return QD_OK;
 }
