@@ -160,6 +160,60 @@ def cpu_baseline(p, FI, DFI, budget_s=25.0):
                 phases=dict(assembly=t_asm, lu=t_lu, internal=t_som))
 
 
+HBM_PEAK_GBS = 8000.0
+
+
+def trailing_shapes(N, precision):
+    """(M, Nn, K) of every outer trailing-update launch of one factorisation:
+    the library's panel width (FP16 1024, else 512; QUADESTH_NB overrides) and
+    its look-ahead split (the next panel's columns, then the rest)."""
+    nb = int(os.environ.get("QUADESTH_NB", "1024" if precision == "fp16" else "512"))
+    nb = max(64, nb // 32 * 32)
+    look = os.environ.get("QUADESTH_LOOKAHEAD", "1")[:1] != "0"
+    out = []
+    for j in range(0, N, nb):
+        jb = min(nb, N - j)
+        nj = j + jb
+        if nj >= N:
+            break
+        njb = min(nb, N - nj)
+        if look and nj + njb < N:
+            out.append((N - nj, njb, jb))
+            out.append((N - nj, N - nj - njb, jb))
+        else:
+            out.append((N - nj, N - nj, jb))
+    return out
+
+
+def trailing_roofline(N, precision, d):
+    """Roofline of the outer trailing update from its algorithmic work: 2 M Nn K
+    flops and, in bytes, the C read and write (8 per FP32 element) plus one
+    pass over the operands in the precision the kernel reads them.  The bound
+    is whichever floor (peak MFMA rate or 8 TB/s) is higher for these shapes."""
+    shapes = trailing_shapes(N, precision)
+    ob = {"fp16": 2, "fp32": 4, "fp64": 8}[precision]   # operand bytes
+    cb = {"fp16": 8, "fp32": 8, "fp64": 16}[precision]  # C read + write bytes per element
+    flops = sum(2.0 * m * n * k for m, n, k in shapes)
+    nbytes = sum(cb * m * n + ob * k * (m + n) for m, n, k in shapes)
+    runs = max(1, round(d["launches"] / max(len(shapes), 1)))
+    if abs(d["work"] - runs * flops) > 0.01 * d["work"]:
+        log(f"trailing-update shapes do not match the profiled work: {runs * flops:.4e} vs {d['work']:.4e}")
+    t = d["ms"] * 1e-3 / runs   # device seconds of one factorisation's launches
+    peak_f = PEAK_TFLOPS[precision]
+    t_mfma = flops / (peak_f * 1e12)
+    t_hbm = nbytes / (HBM_PEAK_GBS * 1e9)
+    common = dict(kernel="lu_trailing_gemm", traffic=None, launches=d["launches"],
+                  avg_launch_ms=d["ms"] / max(d["launches"], 1), flops_per_launch=flops / len(shapes),
+                  bytes_per_launch=nbytes / len(shapes), floor_s=dict(mfma=t_mfma, hbm=t_hbm))
+    if t_hbm >= t_mfma:
+        ach = nbytes / t / 1e9
+        return dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(ach / HBM_PEAK_GBS, 4), mfma_frac=round(flops / t / 1e12 / peak_f, 4), **common)
+    ach = flops / t / 1e12
+    return dict(bound="mfma", achieved=round(ach, 3), peak=peak_f, unit="TFLOP/s", frac=round(ach / peak_f, 4),
+                hbm_frac=round(nbytes / t / 1e9 / HBM_PEAK_GBS, 4), **common)
+
+
 def bench_gradient(args):
     """Adjoint gradient of J = sum of internal von Mises stress squared on the
     perturbed 16x16-per-face cube (18,438 DOF, 4096 points), FP64 forward:
@@ -342,12 +396,15 @@ def main():
     # launches per panel), whose time is reported in "kernels" and "phases_s".
     dom = max((k for k in prof_out if k != "lu_panel"), key=lambda k: prof_out[k]["ms"])
     d = prof_out[dom]
-    peak = PEAK_TFLOPS[args.precision] if dom.startswith("lu") else PEAK_TFLOPS["fp64_vector"]
-    achieved = d["tflops"] or 0.0
-    roofline = dict(bound="mfma" if dom.startswith("lu") else "valu", kernel=dom, achieved=round(achieved, 3),
-                    peak=peak, unit="TFLOP/s", frac=round(achieved / peak, 4), traffic=None,
-                    launches=d["launches"], avg_launch_ms=d["ms"] / max(d["launches"], 1),
-                    work_per_launch=d["work"] / max(d["launches"], 1))
+    if dom == "lu_trailing_gemm":
+        roofline = trailing_roofline(N, args.precision, d)
+    else:
+        peak = PEAK_TFLOPS["fp64_vector"]
+        achieved = d["tflops"] or 0.0
+        roofline = dict(bound="valu", kernel=dom, achieved=round(achieved, 3), peak=peak, unit="TFLOP/s",
+                        frac=round(achieved / peak, 4), traffic=None, launches=d["launches"],
+                        avg_launch_ms=d["ms"] / max(d["launches"], 1),
+                        work_per_launch=d["work"] / max(d["launches"], 1))
 
     if not args.no_fp64 and args.precision != "fp64":
         log("FP64 LU path, one step")
