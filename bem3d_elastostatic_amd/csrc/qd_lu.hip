@@ -1394,29 +1394,6 @@ __global__ __launch_bounds__(256) void k_apply_inv(const T* __restrict__ invb, l
   }
 }
 
-// x[rows] -= W[rows, c0:c0+m] x[c0:c0+m], rows [r0, r1), m <= TB.
-template <typename T, int TB>
-__global__ __launch_bounds__(256) void k_block_update(const T* __restrict__ W, long long ld, long long N, long long r0,
-                                                      long long r1, long long c0, int m, int nrhs,
-                                                      T* __restrict__ x) {
-  __shared__ T sW[64][TB + 1];
-  __shared__ T sx[8][TB];
-  const long long rb = r0 + (long long)blockIdx.x * 64;
-  for (int i = threadIdx.x; i < m * nrhs; i += 256) sx[i / m][i % m] = x[(long long)(i / m) * N + c0 + i % m];
-  for (int i = threadIdx.x; i < 64 * m; i += 256) {
-    const int a = i / m, b = i % m;
-    sW[a][b] = (rb + a < r1) ? W[(rb + a) * ld + c0 + b] : (T)0;
-  }
-  __syncthreads();
-  const int a = threadIdx.x % 64, q = threadIdx.x / 64;
-  if (rb + a >= r1) return;
-  for (int v = q; v < nrhs; v += 4) {
-    T s = 0;
-    for (int b = 0; b < m; ++b) s += sW[a][b] * sx[v][b];
-    x[(long long)v * N + rb + a] -= s;
-  }
-}
-
 // x[v][rows] -= W[rows, c0:c0+m] x[v][c0:c0+m] for rows [r0, r1), m <= TB,
 // TB * sizeof(T) = 512 bytes.  32 lanes per row, each with one 16-byte
 // piece of it and of x; the row sum is reduced across the 32 lanes.  A
@@ -1496,9 +1473,11 @@ static int diag_inverses(qd_context* h, const T* W, long long ld, long long N, T
   return QD_OK;
 }
 
+// nrhs <= kBuMaxRhs (lu_solve solves in groups of 8).
 template <typename T>
 static int tri_solve_inv(qd_context* h, const T* W, long long ld, long long N, int nrhs, T* x, const T* inv) {
   constexpr int TB = TriTB<T>::v;
+  if (nrhs < 1 || nrhs > kBuMaxRhs) return set_error(h, QD_ERR_ARG, "tri_solve_inv: nrhs=%d", nrhs);
   const long long nblk = (N + TB - 1) / TB;
   const T* invL = inv;
   const T* invU = inv + (size_t)nblk * TB * TB;
@@ -1507,28 +1486,18 @@ static int tri_solve_inv(qd_context* h, const T* W, long long ld, long long N, i
     const int m = (int)std::min<long long>(TB, N - i0);
     hipLaunchKernelGGL((k_apply_inv<T, TB>), dim3(1), dim3(256), 0, h->stream, invL + (size_t)b * TB * TB, N, i0, m,
                        nrhs, x);
-    if (i0 + m < N) {
-      if (nrhs <= kBuMaxRhs)
-        hipLaunchKernelGGL((k_block_update_rows<T, TB>), dim3((unsigned)((N - i0 - m + kBuRows - 1) / kBuRows)),
-                           dim3(256), 0, h->stream, W, ld, N, i0 + m, N, i0, m, nrhs, x);
-      else
-        hipLaunchKernelGGL((k_block_update<T, TB>), dim3((unsigned)((N - i0 - m + 63) / 64)), dim3(256), 0,
-                           h->stream, W, ld, N, i0 + m, N, i0, m, nrhs, x);
-    }
+    if (i0 + m < N)
+      hipLaunchKernelGGL((k_block_update_rows<T, TB>), dim3((unsigned)((N - i0 - m + kBuRows - 1) / kBuRows)),
+                         dim3(256), 0, h->stream, W, ld, N, i0 + m, N, i0, m, nrhs, x);
   }
   for (long long b = nblk - 1; b >= 0; --b) {
     const long long i0 = b * TB;
     const int m = (int)std::min<long long>(TB, N - i0);
     hipLaunchKernelGGL((k_apply_inv<T, TB>), dim3(1), dim3(256), 0, h->stream, invU + (size_t)b * TB * TB, N, i0, m,
                        nrhs, x);
-    if (i0 > 0) {
-      if (nrhs <= kBuMaxRhs)
-        hipLaunchKernelGGL((k_block_update_rows<T, TB>), dim3((unsigned)((i0 + kBuRows - 1) / kBuRows)), dim3(256), 0,
-                           h->stream, W, ld, N, 0LL, i0, i0, m, nrhs, x);
-      else
-        hipLaunchKernelGGL((k_block_update<T, TB>), dim3((unsigned)((i0 + 63) / 64)), dim3(256), 0, h->stream, W,
-                           ld, N, 0LL, i0, i0, m, nrhs, x);
-    }
+    if (i0 > 0)
+      hipLaunchKernelGGL((k_block_update_rows<T, TB>), dim3((unsigned)((i0 + kBuRows - 1) / kBuRows)), dim3(256), 0,
+                         h->stream, W, ld, N, 0LL, i0, i0, m, nrhs, x);
   }
   QD_HIP(h, hipGetLastError());
   return QD_OK;
