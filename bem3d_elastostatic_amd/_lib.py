@@ -71,6 +71,7 @@ _SIGNATURES = {
     "qd_internal_points_vjp": (_i, [_vp, _i, _pd, _pd, _pd, _pd, _pd, _i, _pd, _pd, _pd, _pd, _pd, _pd, _pd,
                                     _pd]),
     "qd_memory_usage": (_i, [_pll, _pll, _i]),
+    "qd_fp16_update": (_i, [_vp, _ll, _ll, _i, _vp, _vp, _vp, _vp, _ll]),
     "qd_profile_enable": (_i, [_vp, C.c_uint]),
     "qd_profile_read": (_i, [_vp, _i, _pll, _pd, _pd]),
     "dgesv_": (None, [_pi, _pi, _pd, _pi, _pi, _pd, _pi, _pi]),

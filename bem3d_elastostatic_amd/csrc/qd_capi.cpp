@@ -744,6 +744,15 @@ int qd_internal_points_vjp(qd_handle h, int l, const double* cx, const double* c
   return somigliana_vjp_run(h, l, pts.data(), fi, dfi, variant, g_u, g_sigma, g_fi, g_dfi, gx, gy, gz, gnu);
 }
 
+int qd_fp16_update(qd_handle h, long long M, long long N, int K, const void* A16, const void* B16,
+                   const float* colscale, float* C, long long ldc) {
+  if (!h) return QD_ERR_ARG;
+  if (M < 0 || N < 0 || K <= 0 || K % 32 != 0 || ldc < N || (M > 0 && N > 0 && (!A16 || !B16 || !colscale || !C)))
+    return set_error(h, QD_ERR_ARG, "qd_fp16_update: M=%lld N=%lld K=%d ldc=%lld", M, N, K, ldc);
+  QD_HIP(h, hipSetDevice(h->device));
+  return fp16_update(h, M, N, K, A16, B16, colscale, C, ldc);
+}
+
 int qd_memory_usage(long long* current_bytes, long long* peak_bytes, int reset_peak) {
   MemStats& m = mem_stats();
   if (current_bytes) *current_bytes = m.current.load();

@@ -228,6 +228,8 @@ int boundary_stress_run(qd_context* h, const double* fi_dev, const double* dfi_d
 int lu_solve(qd_context* h, int precision, int nrhs, double* b_dev, long long ldb, double tol,
              int max_iter, bool transpose, qd_solve_report* rep);
 void lu_free(qd_context* h);
+int fp16_update(qd_context* h, long long M, long long N, int Kp, const void* A16, const void* B16,
+                const float* colscale, float* C, long long ldc);
 int lu_invalidate(qd_context* h);
 int dgesv_device(qd_context* h, int n, int nrhs, double* a_dev, long long lda, int* ipiv_host,
                  double* b_dev, long long ldb);

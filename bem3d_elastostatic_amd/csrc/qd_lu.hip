@@ -28,6 +28,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <chrono>
 #include <cstdlib>
 #include <cmath>
@@ -641,6 +642,10 @@ __global__ __launch_bounds__(256) void k_cvt_b16t(int K, long long N, int Kp, co
 // an XCD's L2 serves the A/B rows its tiles share.  A grid smaller than the
 // tile count leaves CU capacity to work on another stream.
 constexpr int H_BM = 256, H_BN = 128, H_BK = 32, H_PAD = 8;
+#ifndef QD_G16_GROUP
+#define QD_G16_GROUP 4
+#endif
+constexpr unsigned kG16Group = QD_G16_GROUP;
 struct G16Smem {
   _Float16 sA[2][H_BM][H_BK + H_PAD];
   _Float16 sB[2][H_BN][H_BK + H_PAD];
@@ -649,7 +654,7 @@ struct G16Smem {
 __device__ __forceinline__ void gemm16_tile(long long m0, long long n0, long long M, long long N, int Kp,
                                             const _Float16* __restrict__ A16, const _Float16* __restrict__ B16,
                                             const float* __restrict__ colscale, float* __restrict__ C, long long ldc,
-                                            G16Smem& sm) {
+                                            bool rows16, G16Smem& sm) {
   auto& sA = sm.sA;
   auto& sB = sm.sB;
   const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
@@ -700,9 +705,60 @@ __device__ __forceinline__ void gemm16_tile(long long m0, long long n0, long lon
     if (kt + 1 < nk) lstore(cur ^ 1);
     __syncthreads();
   }
-  // Interior tiles update C with no per-element test, so each column group's
-  // 32 loads are issued together before one wait (a guarded element would
-  // serialise load, wait, store).
+  // Interior tiles with 16-byte aligned C rows: each wave stages its
+  // 128 x 64 result through LDS (the operand buffers are free now) in slices
+  // of 32 rows, then reads it back row-contiguously, so that C is read and
+  // written in float4s covering 256 contiguous bytes per 16 lanes (the MFMA
+  // layout alone gives 64-byte pieces).  The wave's 64 columns share one
+  // column scale.
+  if (rows16 && m0 + H_BM <= M && n0 + H_BN <= N) {
+    constexpr int kPitch = 68;  // floats per staged row
+    float* stage = reinterpret_cast<float*>(&sm) + wave * (32 * kPitch);
+    const float s = colscale[(n0 + wn) >> 6];
+    // An opaque copy of ldc keeps the row-offset products inside the epilogue:
+    // hoisted out of the tile loop they would stay live across the MFMA loop.
+    long long ld = ldc;
+    asm volatile("" : "+s"(ld));
+    const int c4 = lane & 15;
+    float* cbase = C + (m0 + wm + (lane >> 4)) * ld + n0 + wn + 4 * c4;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+#pragma unroll
+      for (int ii = 0; ii < 2; ++ii)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) stage[(16 * ii + 4 * lk + r) * kPitch + 16 * j + lr] = acc[2 * q + ii][j][r];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        float4 cv[4], av[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int row = (lane >> 4) + 4 * (4 * h + t);
+          cv[t] = *reinterpret_cast<const float4*>(cbase + (32 * q + 4 * (4 * h + t)) * ld);
+          av[t] = *reinterpret_cast<const float4*>(stage + row * kPitch + 4 * c4);
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          float4 o;
+          o.x = cv[t].x - s * av[t].x;
+          o.y = cv[t].y - s * av[t].y;
+          o.z = cv[t].z - s * av[t].z;
+          o.w = cv[t].w - s * av[t].w;
+          *reinterpret_cast<float4*>(cbase + (32 * q + 4 * (4 * h + t)) * ld) = o;
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    return;
+  }
+  // Interior tiles otherwise: no per-element test, so each column group's 32
+  // loads are issued together before one wait.
   if (m0 + H_BM <= M && n0 + H_BN <= N) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -744,20 +800,29 @@ __device__ __forceinline__ void gemm16_tile(long long m0, long long n0, long lon
   }
 }
 
+// rows16: C and ldc keep every row start 16-byte aligned.
 template <int kRole>
 __global__ __launch_bounds__(256, 2) void k_gemm16(long long M, long long N, int Kp, const _Float16* __restrict__ A16,
                                                 const _Float16* __restrict__ B16, const float* __restrict__ colscale,
-                                                float* __restrict__ C, long long ldc) {
+                                                float* __restrict__ C, long long ldc, int rows16) {
   __shared__ __attribute__((aligned(16))) G16Smem sm;
   const unsigned tiles_n = (unsigned)((N + H_BN - 1) / H_BN);
-  const unsigned total = (unsigned)((M + H_BM - 1) / H_BM) * tiles_n;
+  const unsigned tiles_m = (unsigned)((M + H_BM - 1) / H_BM);
+  const unsigned total = tiles_m * tiles_n;
   const unsigned per_xcd = (total + 7) / 8;
   const unsigned xcd = blockIdx.x % 8, lw = blockIdx.x / 8, wx = (gridDim.x + 7 - xcd) / 8;
   for (unsigned i = lw; i < per_xcd; i += wx) {
     const unsigned vid = xcd * per_xcd + i;
     if (vid >= total) break;
-    gemm16_tile((long long)(vid / tiles_n) * H_BM, (long long)(vid % tiles_n) * H_BN, M, N, Kp, A16, B16, colscale,
-                C, ldc, sm);
+    // grouped order: kG16Group row bands sweep the column tiles together, so
+    // a B tile is fetched once for the group while the group's A rows stay
+    // in L2 (row-band order re-streams all of B for every row band)
+    const unsigned grp = vid / (kG16Group * tiles_n);
+    const unsigned first_m = grp * kG16Group;
+    const unsigned gsz = min(tiles_m - first_m, (unsigned)kG16Group);
+    const unsigned in_grp = vid - grp * kG16Group * tiles_n;
+    const unsigned tm = first_m + in_grp % gsz, tn = in_grp / gsz;
+    gemm16_tile((long long)tm * H_BM, (long long)tn * H_BN, M, N, Kp, A16, B16, colscale, C, ldc, rows16 != 0, sm);
   }
 }
 
@@ -1064,12 +1129,13 @@ static int gemm_sub(LuCtx<T>& c, long long M, long long Nn, int K, const T* A, c
       const long long tiles = ((Nn + H_BN - 1) / H_BN) * ((M + H_BM - 1) / H_BM);
       const dim3 g16((unsigned)(max_wgs > 0 ? std::min<long long>(tiles, max_wgs) : tiles));
       ProfScope ps(c.h, trailing ? QD_PROF_LU_TRAILING_GEMM : -1, 2.0 * (double)M * (double)Nn * K, c.st);
+      const int rows16 = (c.ld % 4 == 0 && (reinterpret_cast<uintptr_t>(C) & 15) == 0) ? 1 : 0;
       if (trailing)
         hipLaunchKernelGGL(k_gemm16<1>, g16, dim3(256), 0, c.st, M, Nn, Kp, c.sc->h16a.p, c.sc->h16b.p,
-                           c.sc->colscale.p, C, c.ld);
+                           c.sc->colscale.p, C, c.ld, rows16);
       else
         hipLaunchKernelGGL(k_gemm16<0>, g16, dim3(256), 0, c.st, M, Nn, Kp, c.sc->h16a.p, c.sc->h16b.p,
-                           c.sc->colscale.p, C, c.ld);
+                           c.sc->colscale.p, C, c.ld, rows16);
       QD_HIP(c.h, hipGetLastError());
       return QD_OK;
     }
@@ -2082,6 +2148,17 @@ static int gmres_ir(qd_context* h, bool trans, double* x, double* b, long long /
 void lu_free(qd_context* h) {
   delete h->lu;
   h->lu = nullptr;
+}
+
+int fp16_update(qd_context* h, long long M, long long N, int Kp, const void* A16, const void* B16,
+                const float* colscale, float* C, long long ldc) {
+  if (M <= 0 || N <= 0) return QD_OK;
+  const long long tiles = ((N + H_BN - 1) / H_BN) * ((M + H_BM - 1) / H_BM);
+  const int rows16 = (ldc % 4 == 0 && (reinterpret_cast<uintptr_t>(C) & 15) == 0) ? 1 : 0;
+  hipLaunchKernelGGL(k_gemm16<1>, dim3((unsigned)tiles), dim3(256), 0, h->stream, M, N, Kp,
+                     (const _Float16*)A16, (const _Float16*)B16, colscale, C, ldc, rows16);
+  QD_HIP(h, hipGetLastError());
+  return QD_OK;
 }
 
 int lu_invalidate(qd_context* h) {

@@ -214,6 +214,40 @@ def trailing_roofline(N, precision, d):
                 hbm_frac=round(nbytes / t / 1e9 / HBM_PEAK_GBS, 4), **common)
 
 
+def bench_trailing_update():
+    """The outer trailing updates of one FP16 factorisation at 76,806 DOF, in
+    its launch order and shapes, through qd_fp16_update on synthetic operands
+    (one FP32 C of the largest shape, 23 GB).  Used under rocprofv3 counter
+    passes, whose launch interception does not survive the full solve."""
+    import torch
+    from bem3d_elastostatic_amd import _lib
+    N = 76806
+    shapes = trailing_shapes(N, "fp16")
+    K = max(k for _, _, k in shapes)
+    mx = max(m for m, _, _ in shapes)
+    nx = max(n for _, n, _ in shapes)
+    dev = torch.device("cuda:0")
+    a16 = (0.1 * torch.randn(mx, K, device=dev)).half()
+    b16 = (0.1 * torch.randn(nx, K, device=dev)).half()
+    sc = torch.ones((nx + 63) // 64, device=dev)
+    ldc = (nx + 63) // 64 * 64    # the factorisation's row pitch rule
+    c = torch.zeros(mx, ldc, device=dev)
+    lib = _lib.load()
+    h = C.c_void_p()
+    _lib.check(None, lib.qd_create(C.byref(h), 0), "create")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for m, n, k in shapes:
+        _lib.check(h, lib.qd_fp16_update(h, m, n, k, C.c_void_p(a16.data_ptr()), C.c_void_p(b16.data_ptr()),
+                                         C.c_void_p(sc.data_ptr()), C.c_void_p(c.data_ptr()), ldc), "fp16_update")
+    _lib.check(h, lib.qd_synchronize(h), "sync")
+    t = time.perf_counter() - t0
+    flops = sum(2.0 * m * n * k for m, n, k in shapes)
+    print(json.dumps({"mode": "trailing-update-only", "launches": len(shapes), "seconds": round(t, 4),
+                      "tflops": round(flops / t / 1e12, 1)}), flush=True)
+    lib.qd_destroy(h)
+
+
 def bench_gradient(args):
     """Adjoint gradient of J = sum of internal von Mises stress squared on the
     perturbed 16x16-per-face cube (18,438 DOF, 4096 points), FP64 forward:
@@ -308,6 +342,9 @@ def main():
                     help="c3: the 76,806-DOF cantilever step (default); c1: the 18,438-DOF cube; "
                          "c4: adjoint gradient on the perturbed cube")
     ap.add_argument("--fd-dirs", type=int, default=64, help="c4: directions of the finite-difference check")
+    ap.add_argument("--trailing-update-only", action="store_true",
+                    help="only the 149 FP16 trailing-update launches of one 76,806-DOF factorisation "
+                         "(qd_fp16_update on synthetic operands), for hardware counter passes")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -334,6 +371,9 @@ def main():
     torch.cuda.set_device(0)
     if args.config == "c4":
         bench_gradient(args)
+        return
+    if args.trailing_update_only:
+        bench_trailing_update()
         return
     if args.config == "c3":
         p = mesh.config3()

@@ -187,6 +187,14 @@ enum {
   QD_PROF_KINDS = 6
 };
 int qd_profile_enable(qd_handle h, unsigned mask /* bit per QD_PROF_* */);
+/* The outer trailing update of the FP16 factorisation as an operator (no
+ * reference counterpart), on device buffers and the handle's stream:
+ *   C[i][j] -= colscale[j / 64] * sum_k A16[i][k] * B16[j][k]
+ * A16 (M x K) and B16 (N x K) row-major _Float16 with K a multiple of 32 (pad
+ * with zeros), C (M x N) row-major float with leading dimension ldc, colscale
+ * ceil(N / 64) floats.  MFMA f16 with FP32 accumulation. */
+int qd_fp16_update(qd_handle h, long long M, long long N, int K, const void* A16, const void* B16,
+                   const float* colscale, float* C, long long ldc);
 /* Device memory held by the library's buffers across all handles in this
  * process: current bytes and the high-water mark since the last reset
  * (reset_peak != 0 resets it to the current value after reading). */
