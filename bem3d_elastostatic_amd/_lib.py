@@ -90,6 +90,14 @@ def load(path: str = LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise QuadesthError(-100, f"{path} is not built (python -m bem3d_elastostatic_amd.build)")
+    # PyTorch ships its own HIP runtime under the same sonames as /opt/rocm's.
+    # Whichever loads first serves the whole process, and torch cannot
+    # initialise its GPU state on /opt/rocm's newer runtime, so torch (when
+    # installed) is loaded before the library's own dependencies.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(path)
     for name, (res, args) in _SIGNATURES.items():
         fn = getattr(lib, name)

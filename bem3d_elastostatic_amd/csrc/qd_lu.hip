@@ -65,8 +65,11 @@ struct qd_lu_state {
   qd::DevBuf<float> vecf;
   qd::DevBuf<double> gm;         // GMRES basis
   qd::DevBuf<double> rowpart;    // residual slice partials
-  qd::DevBuf<int> ts_a, ts_b;    // tournament candidate lists
+  qd::DevBuf<int> ts_cand;       // tournament candidate lists, all levels
+  qd::DevBuf<unsigned> ts_cnt;   // tournament group counters (zero between leaves)
+  qd::DevBuf<int> ts_meta;       // interchanges of a leaf
   qd::DevBuf<float> ts_lu;       // LU of a leaf's winners block (32 x 32)
+  qd::DevBuf<float> ts_top;      // a leaf's original top rows (32 x 32)
   qd::DevBuf<float> inv32;       // inverted diagonal blocks of L and U
   qd::DevBuf<double> inv64;
   double anorm = -1.0;           // ||A||_inf of the FP64 matrix
@@ -826,264 +829,11 @@ __global__ __launch_bounds__(256, 2) void k_gemm16(long long M, long long N, int
   }
 }
 
-// ---------------------------------------------------------------------------
-// Tournament pivoting for a leaf of <= 32 columns (the factorisation that
-// only preconditions the FP64 refinement).  k_tslu_select runs Gaussian
-// elimination with partial pivoting on up to 256 rows per workgroup of one
-// wave (4 rows per lane, held in registers) and emits the 32 winning row
-// numbers in pivot order; applied first to the leaf's rows and then, level by
-// level, to the winners of groups of 8 workgroups, until one workgroup's
-// winners remain.  The rows themselves are read from W, which is not modified
-// before the winners are known, so every level works on the original values.
-// The pivot search reduces one 64-bit key per lane, (bits of |a| << 32) |
-// ~row: the largest |a| wins and, among equal values, the lowest row.
-constexpr int kTsW = 32, kTsRPT = 4, kTsRows = 64 * kTsRPT;
+}  // namespace qd
 
-__device__ __forceinline__ unsigned long long ts_key(float v, int row) {
-  return ((unsigned long long)__float_as_uint(fabsf(v)) << 32) | (unsigned)(~row);
-}
+#include "qd_tslu.h"
 
-// Wave maximum of a 64-bit key: DPP butterflies inside each row of 16 lanes
-// (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror),
-// then the four row maxima through readlane.  Called by all 64 lanes.
-template <int kCtrl>
-__device__ __forceinline__ unsigned long long dpp_max64(unsigned long long v) {
-  const int lo = (int)(unsigned)v, hi = (int)(unsigned)(v >> 32);
-  const unsigned olo = (unsigned)__builtin_amdgcn_update_dpp(lo, lo, kCtrl, 0xf, 0xf, false);
-  const unsigned ohi = (unsigned)__builtin_amdgcn_update_dpp(hi, hi, kCtrl, 0xf, 0xf, false);
-  const unsigned long long o = ((unsigned long long)ohi << 32) | olo;
-  return o > v ? o : v;
-}
-
-__device__ __forceinline__ unsigned long long wave_max64(unsigned long long v) {
-  v = dpp_max64<0xB1>(v);
-  v = dpp_max64<0x4E>(v);
-  v = dpp_max64<0x141>(v);
-  v = dpp_max64<0x140>(v);
-  unsigned long long m = 0;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, 16 * r);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), 16 * r);
-    const unsigned long long x = ((unsigned long long)hi << 32) | lo;
-    m = x > m ? x : m;
-  }
-  return m;
-}
-
-// One elimination step (column K); K is a compile-time constant so the
-// register arrays are indexed statically.
-template <int K>
-__device__ __forceinline__ void ts_step(float (&a)[kTsRPT][kTsW], int (&gidx)[kTsRPT], bool& done, int w,
-                                        float* __restrict__ srow, int* __restrict__ out_idx,
-                                        float* __restrict__ lu_out) {
-  const int lane = threadIdx.x;
-  done = done || (K >= w);
-  unsigned long long key = 0;
-#pragma unroll
-  for (int q = 0; q < kTsRPT; ++q)
-    if (!done && gidx[q] >= 0) key = max(key, ts_key(a[q][K], gidx[q]));
-  key = wave_max64(key);
-  const bool found = !done && key != 0;
-  const int ib = (int)(~(unsigned)(key & 0xffffffffull));
-  if (found) {
-#pragma unroll
-    for (int q = 0; q < kTsRPT; ++q)
-      if (gidx[q] == ib) {
-#pragma unroll
-        for (int j = 0; j < kTsW; ++j) srow[j] = a[q][j];
-        gidx[q] = -1;
-      }
-  }
-  if (lane == 0) out_idx[(long long)blockIdx.x * kTsW + K] = found ? ib : -1;
-  done = !found;
-  __syncthreads();
-  if (found) {
-    // the winner of step K, eliminated by the K earlier winners with its
-    // multipliers kept in place, is row K of the winners' LU
-    if (lu_out && lane < kTsW) lu_out[K * kTsW + lane] = srow[lane];
-    const float piv = srow[K];
-    const float rp = (piv != 0.0f) ? 1.0f / piv : 0.0f;
-#pragma unroll
-    for (int q = 0; q < kTsRPT; ++q) {
-      const float l = (gidx[q] >= 0) ? a[q][K] * rp : 0.0f;
-      if (gidx[q] >= 0) a[q][K] = l;
-#pragma unroll
-      for (int j = K + 1; j < kTsW; ++j) a[q][j] -= l * srow[j];
-    }
-  }
-  __syncthreads();
-}
-
-template <int... Ks>
-__device__ __forceinline__ void ts_all(std::integer_sequence<int, Ks...>, float (&a)[kTsRPT][kTsW],
-                                       int (&gidx)[kTsRPT], bool& done, int w, float* __restrict__ srow,
-                                       int* __restrict__ out_idx, float* __restrict__ lu_out) {
-  (ts_step<Ks>(a, gidx, done, w, srow, out_idx, lu_out), ...);
-}
-
-// Loads the 32-column segment [c0, c0+32) of row g (zero beyond column w).
-// c0 is a multiple of 32 and rows are ld >= roundup(N, 64) floats long, so the
-// segment is 128-byte aligned and inside the row.
-__device__ __forceinline__ void ts_load_row(const float* __restrict__ W, long long ld, long long g, long long c0,
-                                            int w, float (&r)[kTsW]) {
-  const float4* src = reinterpret_cast<const float4*>(W + g * ld + c0);
-#pragma unroll
-  for (int j4 = 0; j4 < kTsW / 4; ++j4) {
-    const float4 v = src[j4];
-    r[4 * j4 + 0] = (4 * j4 + 0 < w) ? v.x : 0.0f;
-    r[4 * j4 + 1] = (4 * j4 + 1 < w) ? v.y : 0.0f;
-    r[4 * j4 + 2] = (4 * j4 + 2 < w) ? v.z : 0.0f;
-    r[4 * j4 + 3] = (4 * j4 + 3 < w) ? v.w : 0.0f;
-  }
-}
-
-// lu_out (final level only, one workgroup): the winners' LU, w x w, row-major
-// with stride 32.
-__global__ __launch_bounds__(64) void k_tslu_select(const float* __restrict__ W, long long ld, long long N,
-                                                    long long c0, int w, const int* __restrict__ in_idx, int n_in,
-                                                    int* __restrict__ out_idx, float* __restrict__ lu_out) {
-  __shared__ float srow[kTsW];
-  const int lane = threadIdx.x;
-  float a[kTsRPT][kTsW];
-  int gidx[kTsRPT];
-  // row numbers first, then all row pieces (an absent row reads row c0 and
-  // is zeroed), so the loads are in flight together
-#pragma unroll
-  for (int q = 0; q < kTsRPT; ++q) {
-    const long long li = (long long)blockIdx.x * kTsRows + lane + 64 * q;
-    long long g = -1;
-    if (in_idx) {
-      if (li < n_in) g = in_idx[li];
-    } else if (c0 + li < N) {
-      g = c0 + li;
-    }
-    gidx[q] = (int)g;
-  }
-#pragma unroll
-  for (int q = 0; q < kTsRPT; ++q) {
-    const long long g = gidx[q] >= 0 ? gidx[q] : c0;
-    ts_load_row(W, ld, g, c0, w, a[q]);
-  }
-#pragma unroll
-  for (int q = 0; q < kTsRPT; ++q)
-#pragma unroll
-    for (int j = 0; j < kTsW; ++j) a[q][j] = (gidx[q] >= 0) ? a[q][j] : 0.0f;
-  bool done = false;
-  ts_all(std::make_integer_sequence<int, kTsW>{}, a, gidx, done, w, srow, out_idx, lu_out);
-}
-
-// The LAPACK interchange sequence that brings the winners p_0..p_{w-1} to
-// rows c0..c0+w-1 in order (ipiv[c0+k] = current position of p_k), then the
-// interchanges on columns [p0, p1).  Thread 0 of every workgroup forms the
-// sequence in LDS (about w^2 steps); workgroup 0 also stores it in ipiv.
-__global__ __launch_bounds__(256) void k_tslu_swap(float* __restrict__ W, long long ld, long long p0, long long p1,
-                                                   const int* __restrict__ win, int w, long long c0,
-                                                   int* __restrict__ ipiv) {
-  __shared__ int seq[kTsW];
-  if (threadIdx.x == 0) {
-    int prow[2 * kTsW], ppos[2 * kTsW];  // row -> position, for rows that moved
-    int np = 0;
-    auto pos_of = [&](int row) {
-      for (int i = 0; i < np; ++i)
-        if (prow[i] == row) return ppos[i];
-      return row;
-    };
-    auto set_pos = [&](int row, int pos) {
-      for (int i = 0; i < np; ++i)
-        if (prow[i] == row) {
-          ppos[i] = pos;
-          return;
-        }
-      prow[np] = row;
-      ppos[np] = pos;
-      ++np;
-    };
-    auto row_at = [&](int pos) {
-      for (int i = 0; i < np; ++i)
-        if (ppos[i] == pos) return prow[i];
-      return pos;  // an unmoved row sits at its own index
-    };
-    for (int k = 0; k < w; ++k) {
-      const int t = (int)c0 + k;
-      const int p = win[k];
-      const int cur = pos_of(p);
-      seq[k] = cur;
-      if (blockIdx.x == 0) ipiv[t] = cur;
-      if (cur != t) {
-        const int r_t = row_at(t);
-        set_pos(p, t);
-        set_pos(r_t, cur);
-      }
-    }
-  }
-  __syncthreads();
-  const long long c = p0 + (long long)blockIdx.x * 256 + threadIdx.x;
-  if (c >= p1) return;
-  for (int k = 0; k < w; ++k) {
-    const long long t = c0 + k, p = seq[k];
-    if (p != t) {
-      const float v = W[t * ld + c];
-      W[t * ld + c] = W[p * ld + c];
-      W[p * ld + c] = v;
-    }
-  }
-}
-
-// After the interchanges: L21 = A21 U11^-1, one row per thread, against U11
-// read from LDS by broadcast.  U11 (with L11 below its diagonal) comes from
-// the final tournament level's side buffer, not from W, and workgroup 0 copies
-// that block into W's top rows: no workgroup reads the block from W here.
-template <int K>
-__device__ __forceinline__ void tf_solve_step(float (&x)[kTsW], int w, const float (*__restrict__ sU)[kTsW],
-                                              const float* __restrict__ sR) {
-  if (K < w) {
-    x[K] *= sR[K];
-#pragma unroll
-    for (int j = K + 1; j < kTsW; ++j) x[j] -= x[K] * sU[K][j];
-  }
-}
-
-template <int... Ks>
-__device__ __forceinline__ void tf_solve(std::integer_sequence<int, Ks...>, float (&x)[kTsW], int w,
-                                         const float (*__restrict__ sU)[kTsW], const float* __restrict__ sR) {
-  (tf_solve_step<Ks>(x, w, sU, sR), ...);
-}
-
-__global__ __launch_bounds__(256) void k_tslu_finish(float* __restrict__ W, long long ld, long long N, long long c0,
-                                                     int w, const float* __restrict__ lu_in) {
-  __shared__ float sU[kTsW][kTsW];
-  __shared__ float sR[kTsW];
-  const int tid = threadIdx.x;
-  {
-    float v[kTsW * kTsW / 256];
-#pragma unroll
-    for (int q = 0; q < kTsW * kTsW / 256; ++q) {
-      const int i = tid + 256 * q, r = i / kTsW, j = i % kTsW;
-      v[q] = 0.0f;
-      if (r < w && j < w) v[q] = lu_in[i];
-    }
-#pragma unroll
-    for (int q = 0; q < kTsW * kTsW / 256; ++q) {
-      const int i = tid + 256 * q, r = i / kTsW, j = i % kTsW;
-      sU[r][j] = (j >= r) ? v[q] : 0.0f;
-      if (blockIdx.x == 0 && r < w && j < w) W[(c0 + r) * ld + c0 + j] = v[q];
-    }
-  }
-  __syncthreads();
-  if (tid < kTsW) {
-    const float d = sU[tid][tid];
-    sR[tid] = (tid < w && d != 0.0f) ? 1.0f / d : 0.0f;
-  }
-  __syncthreads();
-  const long long r = c0 + w + (long long)blockIdx.x * 256 + tid;
-  if (r >= N) return;
-  float x[kTsW];
-  ts_load_row(W, ld, r, c0, w, x);
-  tf_solve(std::make_integer_sequence<int, kTsW>{}, x, w, sU, sR);
-  float* dst = W + r * ld + c0;
-  for (int j = 0; j < w; ++j) dst[j] = x[j];
-}
+namespace qd {
 
 // ---------------------------------------------------------------------------
 // host-side LU driver
@@ -1195,28 +945,19 @@ static int colstep(LuCtx<T>& c, long long col, long long prev, long long lstart,
 static int tslu_leaf(LuCtx<float>& c, long long c0, int w, long long p0, long long p1) {
   qd_lu_state* s = c.s;
   const long long m = c.N - c0;
-  int nb = (int)((m + kTsRows - 1) / kTsRows);
-  QD_ALLOC(c.h, s->ts_a, (size_t)nb * kTsW);
-  QD_ALLOC(c.h, s->ts_b, (size_t)nb * kTsW);
-  QD_ALLOC(c.h, s->ts_lu, (size_t)kTsW * kTsW);
-  int* in = s->ts_a.p;
-  int* out = s->ts_b.p;
-  hipLaunchKernelGGL(k_tslu_select, dim3(nb), dim3(64), 0, c.st, c.W, c.ld, c.N, c0, w, (const int*)nullptr, 0, in,
-                     nb == 1 ? s->ts_lu.p : (float*)nullptr);
-  int n_in = nb * kTsW;
-  while (nb > 1) {
-    const int nb2 = (n_in + kTsRows - 1) / kTsRows;
-    hipLaunchKernelGGL(k_tslu_select, dim3(nb2), dim3(64), 0, c.st, c.W, c.ld, c.N, c0, w, (const int*)in, n_in,
-                       out, nb2 == 1 ? s->ts_lu.p : (float*)nullptr);
-    n_in = nb2 * kTsW;
-    std::swap(in, out);
-    nb = nb2;
-  }
-  hipLaunchKernelGGL(k_tslu_swap, dim3((unsigned)((p1 - p0 + 255) / 256)), dim3(256), 0, c.st, c.W, c.ld, p0, p1,
-                     (const int*)in, w, c0, s->ipiv_dev.p);
+  long long ncand = 0, ncnt = 0;
+  ts_tree_sizes(m, ncand, ncnt);
+  if ((long long)s->ts_cand.count < ncand || (long long)s->ts_cnt.count < ncnt)
+    return set_error(c.h, QD_ERR_STATE, "tournament buffers sized for fewer rows than %lld", m);
+  const unsigned nb = (unsigned)((m + kTsRows - 1) / kTsRows);
+  hipLaunchKernelGGL(k_tslu_tree, dim3(nb), dim3(64), 0, c.st, c.W, c.ld, c.N, c0, w, s->ts_cand.p, s->ts_cnt.p,
+                     s->ts_lu.p, s->ts_top.p, s->ts_meta.p, s->ipiv_dev.p);
   const long long below = std::max<long long>(m - w, 1);
-  hipLaunchKernelGGL(k_tslu_finish, dim3((unsigned)((below + 255) / 256)), dim3(256), 0, c.st, c.W, c.ld, c.N, c0,
-                     w, (const float*)s->ts_lu.p);
+  const int row_blocks = (int)((below + 255) / 256);
+  const int col_blocks = (int)((p1 - p0 + 255) / 256);
+  hipLaunchKernelGGL(k_tslu_finish, dim3((unsigned)(row_blocks + col_blocks)), dim3(256), 0, c.st, c.W, c.ld, c.N,
+                     c0, w, p0, p1, (const float*)s->ts_lu.p, (const float*)s->ts_top.p, (const int*)s->ts_meta.p,
+                     row_blocks);
   QD_HIP(c.h, hipGetLastError());
   return QD_OK;
 }
@@ -1272,11 +1013,12 @@ static int lookahead_gemm_wgs() {
   return 0;
 }
 
-// Blocked right-looking LU with a look-ahead of one panel: each step updates
-// the next panel's columns first; that panel is then factored on the aux
-// stream while the rest of the trailing matrix is updated on the main stream
-// (the two touch disjoint columns and separate FP16 scratch).  The main
-// stream waits for the panel before the next step's interchanges.
+// Blocked right-looking LU with a look-ahead of one panel: each step solves
+// for and updates the next panel's columns first; that panel is then factored
+// on the aux stream while the rest of U12 and of the trailing matrix is
+// updated on the main stream (the two touch disjoint columns and separate
+// FP16 scratch).  The main stream waits for the panel before the next step's
+// interchanges.
 // QUADESTH_LOOKAHEAD=0 runs the steps strictly in sequence.
 template <typename T>
 static int factor(LuCtx<T>& c, int NB) {
@@ -1288,6 +1030,17 @@ static int factor(LuCtx<T>& c, int NB) {
   QD_ALLOC(c.h, s->piv_state, 2);
   QD_ALLOC(c.h, s->rpiv, 1);
   QD_ALLOC(c.h, s->ipiv_dev, (size_t)N);
+  if (c.calu) {
+    long long ncand = 0, ncnt = 0;
+    ts_tree_sizes(N, ncand, ncnt);
+    const bool fresh = (long long)s->ts_cnt.count < ncnt + 1 || !s->ts_cnt.p;
+    QD_ALLOC(c.h, s->ts_cand, (size_t)ncand);
+    QD_ALLOC(c.h, s->ts_cnt, (size_t)ncnt + 1);
+    QD_ALLOC(c.h, s->ts_meta, (size_t)kTsMeta);
+    QD_ALLOC(c.h, s->ts_lu, (size_t)kTsW * kTsW);
+    QD_ALLOC(c.h, s->ts_top, (size_t)kTsW * kTsW);
+    if (fresh) QD_HIP(c.h, hipMemsetAsync(s->ts_cnt.p, 0, s->ts_cnt.count * sizeof(unsigned), c.st));
+  }
   if (c.f16) {
     // sized up front, so that nothing is reallocated while two streams run
     const size_t Kp = (size_t)((NB + 31) / 32 * 32), Kh = (size_t)((NB / 2 + 31) / 32 * 32);
@@ -1324,16 +1077,23 @@ static int factor(LuCtx<T>& c, int NB) {
       hipLaunchKernelGGL(k_laswp<T>, dim3((unsigned)((N - nj + 255) / 256)), dim3(256), 0, c.st, c.W, c.ld, nj, N,
                          (int)j, (int)nj, s->ipiv_dev.p);
       QD_HIP(c.h, hipGetLastError());
-      if ((rc = trsm_lower(c, j, jb, nj, N))) return rc;
       if (look && nj + njb < N) {
+        // the next panel's columns first: U12 and its update, then the panel
+        // can start on the aux stream
+        if ((rc = trsm_lower(c, j, jb, nj, nj + njb))) return rc;
         if ((rc = update_cols(c, j, jb, nj, nj + njb, false))) return rc;
         QD_HIP(c.h, hipEventRecord(s->ev_cols, c.st));
+        // The rest of the step is enqueued before the panel: the host can
+        // be held up enqueuing the panel's many small launches, and the
+        // main stream would idle meanwhile.
+        if ((rc = trsm_lower(c, j, jb, nj + njb, N))) return rc;
+        if ((rc = update_cols(c, j, jb, nj + njb, N, false, lookahead_gemm_wgs()))) return rc;
         QD_HIP(c.h, hipStreamWaitEvent(cp.st, s->ev_cols, 0));
         if ((rc = panel_step(cp, nj, njb))) return rc;
         QD_HIP(c.h, hipEventRecord(s->ev_panel, cp.st));
-        if ((rc = update_cols(c, j, jb, nj + njb, N, true, lookahead_gemm_wgs()))) return rc;
         QD_HIP(c.h, hipStreamWaitEvent(c.st, s->ev_panel, 0));
       } else {
+        if ((rc = trsm_lower(c, j, jb, nj, N))) return rc;
         if ((rc = update_cols(c, j, jb, nj, N, false))) return rc;
         if (njb > 0 && (rc = panel_step(c, nj, njb))) return rc;
       }

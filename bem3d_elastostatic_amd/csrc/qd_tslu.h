@@ -1,0 +1,406 @@
+// Tournament pivoting kernels of the FP32 preconditioner LU (a leaf of <= 32
+// columns): the selection tree and the finish.  Device code only; included
+// by qd_lu.hip inside namespace qd.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <utility>
+
+namespace qd {
+
+// Tournament pivoting for a leaf of <= 32 columns (the factorisation that
+// only preconditions the FP64 refinement), in two launches per leaf.
+//
+// k_tslu_tree runs the whole selection tree.  A node is Gaussian elimination
+// with partial pivoting on up to 256 rows, four per lane of one wave, and
+// yields its 32 winning row numbers in pivot order.  A step needs no LDS and
+// no barrier: the pivot search is a DPP wave maximum, and the pivot row
+// reaches every lane through readlane (into SGPRs).  Level 0 has one node per
+// 256 rows of the leaf; each level above has one node per 8 nodes below, over
+// their 8 x 32 winners.  No workgroup waits for another: the last workgroup of
+// a group to finish (an agent-scope counter) goes on to run the parent node,
+// and the workgroup that runs the root writes the winners' LU, the LAPACK
+// interchange sequence (ipiv) and the permutation it amounts to.  The rows are
+// read from W, which the tree does not modify, so every node works on the
+// original values.
+//
+// k_tslu_finish then writes the winners' LU into the leaf's top rows, forms
+// L21 = A21 U11^-1 for every row below (a row whose content the interchanges
+// replace reads that content from the copy of the original top rows the root
+// made) and, in its remaining workgroups, permutes the panel's other columns.
+//
+// Both raise their waves' issue priority (s_setprio): they run beside the
+// trailing update on the same SIMDs and sit on the factorisation's critical
+// path.
+constexpr int kTsW = 32, kTsR = 4;
+constexpr int kTsRows = 64 * kTsR;     // rows of a node
+constexpr int kTsFan = kTsRows / kTsW;  // nodes per parent
+// meta layout: [0, 32) ipiv values, [32, 64) displaced positions, [64, 96)
+// the original top row each of them receives, [96] their count, [128, 160)
+// the original row that ends at top position c0 + k.
+constexpr int kTsMeta = 160;
+
+// Candidate-list entries and counters the tree needs for a leaf of m rows.
+static inline void ts_tree_sizes(long long m, long long& cand, long long& counters) {
+  long long nl = (m + kTsRows - 1) / kTsRows;
+  cand = 0;
+  counters = 0;
+  while (true) {
+    cand += nl * kTsW;
+    if (nl == 1) break;
+    const long long up = (nl + kTsFan - 1) / kTsFan;
+    counters += up;
+    nl = up;
+  }
+}
+
+// Wave maximum of an unsigned value: DPP butterflies inside each row of 16
+// lanes (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror,
+// row_mirror), then the four row maxima through readlane.  Called by all 64
+// lanes; the result is uniform.
+template <int kCtrl>
+__device__ __forceinline__ unsigned dpp_max_u32(unsigned v) {
+  const unsigned o = (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, kCtrl, 0xf, 0xf, false);
+  return o > v ? o : v;
+}
+
+__device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
+  v = dpp_max_u32<0xB1>(v);
+  v = dpp_max_u32<0x4E>(v);
+  v = dpp_max_u32<0x141>(v);
+  v = dpp_max_u32<0x140>(v);
+  unsigned m = (unsigned)__builtin_amdgcn_readlane((int)v, 0);
+  m = max(m, (unsigned)__builtin_amdgcn_readlane((int)v, 16));
+  m = max(m, (unsigned)__builtin_amdgcn_readlane((int)v, 32));
+  m = max(m, (unsigned)__builtin_amdgcn_readlane((int)v, 48));
+  return m;
+}
+
+__device__ __forceinline__ float readlane_f(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// One elimination step (column K) of a node.  A row's key is bits(|a|) + 1
+// (0 = no row), so the largest |a| wins; ties go to the lowest lane, then the
+// lane's first row.  The winning lane stores its row in LDS (under a uniform
+// branch on which of its rows won) and every lane reads it back by broadcast.
+// The step has no other branch, so the row registers need no copies between
+// steps: a node out of rows elects no row (its winner index reads -1) and
+// changes nothing, and columns K >= w are zero, so their steps change nothing
+// either (their winners are not used).  A scheduling barrier closes the step.
+template <int K>
+__device__ __forceinline__ void tn_step(float (&a)[kTsR][kTsW], int (&gi)[kTsR], int* __restrict__ swin,
+                                        float* __restrict__ srow, int* __restrict__ sidx,
+                                        float* __restrict__ lu_out) {
+  const int lane = threadIdx.x;
+  unsigned key = 0;
+  int qb = 0;
+#pragma unroll
+  for (int q = 0; q < kTsR; ++q) {
+    const unsigned kq = (gi[q] >= 0) ? __float_as_uint(fabsf(a[q][K])) + 1u : 0u;
+    qb = (kq > key) ? q : qb;
+    key = (kq > key) ? kq : key;
+  }
+  const unsigned wmax = wave_max_u32(key);
+  const int wl = __ffsll((long long)__ballot(key == wmax)) - 1;
+  const int qw = __builtin_amdgcn_readlane(qb, wl);
+#pragma unroll
+  for (int q = 0; q < kTsR; ++q)
+    if (q == qw && lane == wl) {
+#pragma unroll
+      for (int j = 0; j < kTsW; ++j) srow[j] = a[q][j];
+      *sidx = gi[q];
+    }
+#pragma unroll
+  for (int q = 0; q < kTsR; ++q) gi[q] = (q == qw && lane == wl) ? -1 : gi[q];
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the row is in LDS
+  __builtin_amdgcn_wave_barrier();
+  const float4* s4 = reinterpret_cast<const float4*>(srow);
+  float r[kTsW];
+#pragma unroll
+  for (int j4 = K / 4; j4 < kTsW / 4; ++j4) {
+    const float4 v = s4[j4];
+    r[4 * j4] = v.x;
+    r[4 * j4 + 1] = v.y;
+    r[4 * j4 + 2] = v.z;
+    r[4 * j4 + 3] = v.w;
+  }
+  const int ib = *sidx;
+  if (lane == 0) swin[K] = ib;
+  // the winner of step K, eliminated by the K earlier winners with its
+  // multipliers kept in place, is row K of the winners' LU
+  if (lu_out && lane < kTsW) lu_out[K * kTsW + lane] = srow[lane];
+  const float piv = r[K];
+  const float rp = (piv != 0.0f) ? __builtin_amdgcn_rcpf(piv) : 0.0f;
+#pragma unroll
+  for (int q = 0; q < kTsR; ++q) {
+    const float l = (gi[q] >= 0) ? a[q][K] * rp : 0.0f;
+    a[q][K] = l;
+#pragma unroll
+    for (int j = K + 1; j < kTsW; ++j) a[q][j] -= l * r[j];
+  }
+  // every lane has read the row before the next step overwrites it
+  __builtin_amdgcn_s_waitcnt(0xc07f);
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+template <int... Ks>
+__device__ __forceinline__ void tn_all(std::integer_sequence<int, Ks...>, float (&a)[kTsR][kTsW], int (&gi)[kTsR],
+                                       int* __restrict__ swin, float* __restrict__ srow, int* __restrict__ sidx,
+                                       float* __restrict__ lu_out) {
+  (tn_step<Ks>(a, gi, swin, srow, sidx, lu_out), ...);
+}
+
+// Loads the 32-column segment [c0, c0+32) of row g (zero beyond column w).
+// c0 is a multiple of 32 and rows are ld >= roundup(N, 64) floats long, so the
+// segment is 128-byte aligned and inside the row.
+__device__ __forceinline__ void ts_load_row(const float* __restrict__ W, long long ld, long long g, long long c0,
+                                            int w, float (&r)[kTsW]) {
+  const float4* src = reinterpret_cast<const float4*>(W + g * ld + c0);
+#pragma unroll
+  for (int j4 = 0; j4 < kTsW / 4; ++j4) {
+    const float4 v = src[j4];
+    r[4 * j4 + 0] = (4 * j4 + 0 < w) ? v.x : 0.0f;
+    r[4 * j4 + 1] = (4 * j4 + 1 < w) ? v.y : 0.0f;
+    r[4 * j4 + 2] = (4 * j4 + 2 < w) ? v.z : 0.0f;
+    r[4 * j4 + 3] = (4 * j4 + 3 < w) ? v.w : 0.0f;
+  }
+}
+
+// cand: the candidate lists of every level (ts_tree_sizes entries); cnt: the
+// group counters, zero between leaves (the last arriver resets its own).
+// Outputs of the root: lu_out (w x w winners' LU, stride 32), top_out (the
+// original top rows' leaf columns, 32 x 32), meta, ipiv[c0 .. c0+w).
+// One wave per workgroup.
+__global__ __launch_bounds__(64) void k_tslu_tree(const float* __restrict__ W, long long ld, long long N,
+                                                  long long c0, int w, int* cand, unsigned* cnt,
+                                                  float* __restrict__ lu_out, float* __restrict__ top_out,
+                                                  int* __restrict__ meta, int* __restrict__ ipiv) {
+  __shared__ int swin[kTsW];
+  __shared__ __attribute__((aligned(16))) float srow[kTsW];
+  __shared__ int sidx;
+  __shared__ int slast;
+  __builtin_amdgcn_s_setprio(3);
+  const int tid = threadIdx.x;
+  const long long m = N - c0;
+  long long nl = (m + kTsRows - 1) / kTsRows;  // nodes of the current level
+  long long node = blockIdx.x;
+  long long off = 0;   // first candidate entry of the current level
+  long long coff = 0;  // first counter of the current level's groups
+  int gi[kTsR];
+#pragma unroll
+  for (int q = 0; q < kTsR; ++q) {
+    const long long g = c0 + node * kTsRows + tid + 64 * q;
+    gi[q] = (g < N) ? (int)g : -1;
+  }
+  while (true) {
+    float a[kTsR][kTsW];
+#pragma unroll
+    for (int q = 0; q < kTsR; ++q) ts_load_row(W, ld, gi[q] >= 0 ? gi[q] : c0, c0, w, a[q]);
+#pragma unroll
+    for (int q = 0; q < kTsR; ++q)
+#pragma unroll
+      for (int j = 0; j < kTsW; ++j) a[q][j] = (gi[q] >= 0) ? a[q][j] : 0.0f;
+    const bool root = (nl == 1);
+    tn_all(std::make_integer_sequence<int, kTsW>{}, a, gi, swin, srow, &sidx, root ? lu_out : nullptr);
+    __syncthreads();
+    if (root) break;
+    if (tid < kTsW) cand[off + node * kTsW + tid] = (tid < w) ? swin[tid] : -1;
+    // hand the winners on: the group's last node to finish runs the parent
+    const long long grp = node / kTsFan;
+    const unsigned gsz = (unsigned)min((long long)kTsFan, nl - kTsFan * grp);
+    __syncthreads();
+    if (tid == 0) {
+      __threadfence();
+      const unsigned old = atomicAdd(&cnt[coff + grp], 1u);
+      const int last = (old == gsz - 1) ? 1 : 0;
+      if (last) atomicExch(&cnt[coff + grp], 0u);
+      slast = last;
+    }
+    __syncthreads();
+    if (!slast) return;
+    __threadfence();
+#pragma unroll
+    for (int q = 0; q < kTsR; ++q) {
+      const int t = tid + 64 * q;
+      int ng = -1;
+      if (t < (int)(kTsW * gsz)) {
+        ng = __hip_atomic_load(&cand[off + kTsFan * grp * kTsW + t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (ng < c0 || ng >= N) ng = -1;
+      }
+      gi[q] = ng;
+    }
+    off += nl * kTsW;
+    coff += (nl + kTsFan - 1) / kTsFan;
+    nl = (nl + kTsFan - 1) / kTsFan;
+    node = grp;
+  }
+  // root: the original top rows' leaf columns, then the interchange sequence
+  // that brings the winners p_0..p_{w-1} to rows c0..c0+w-1 in order
+  // (ipiv[c0+k] = current position of p_k), formed over the 64 positions it
+  // can touch: lanes 0..31 the top rows, lane 32+k the original position of
+  // p_k when that lies below the top rows.
+#pragma unroll
+  for (int i = tid; i < kTsW * kTsW / 4; i += 64) {
+    const int r = i >> 3, j4 = (i & 7) * 4;
+    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (r < w) v = *reinterpret_cast<const float4*>(W + (c0 + r) * ld + c0 + j4);
+    *reinterpret_cast<float4*>(top_out + r * kTsW + j4) = v;
+  }
+  {
+    const int lane = tid;
+    int pos = -1;
+    if (lane < w) {
+      pos = (int)c0 + lane;
+    } else if (lane >= 32 && lane - 32 < w) {
+      const int p = swin[lane - 32];
+      if (p >= c0 + w && p < N) pos = p;
+    }
+    int content = pos;
+    for (int k = 0; k < w; ++k) {
+      const int t = (int)c0 + k, p = swin[k];
+      const unsigned long long bp = __ballot(p >= 0 && content == p);
+      int cur = t;
+      if (bp) {
+        const int lp = __ffsll((long long)bp) - 1;
+        cur = __builtin_amdgcn_readlane(pos, lp);
+        if (cur != t) {
+          const int ct = __builtin_amdgcn_readlane(content, k);
+          if (lane == lp) content = ct;
+          if (lane == k) content = p;
+        }
+      }
+      if (lane == 0) {
+        meta[k] = cur;
+        ipiv[t] = cur;
+      }
+    }
+    const unsigned long long bd = __ballot(lane >= 32 && pos >= 0 && content != pos);
+    const int rank = __popcll(bd & ((1ull << lane) - 1ull));
+    if (lane >= 32 && pos >= 0 && content != pos) {
+      meta[32 + rank] = pos;
+      meta[64 + rank] = content;
+    }
+    if (lane == 0) meta[96] = __popcll(bd);
+    if (lane < kTsW) meta[128 + lane] = (lane < w) ? content : -1;
+  }
+}
+
+// L21 = A21 U11^-1 (one row per thread, U11 read from LDS by broadcast),
+// against the root's winners' LU, which workgroup 0 also writes into the top
+// rows; then, in workgroups row_blocks.., the permutation on the panel's
+// columns [p0, p1) outside the leaf.  No workgroup reads what another writes:
+// rows replaced by the interchanges read the root's copy of the top rows.
+template <int K>
+__device__ __forceinline__ void tf_solve_step(float (&x)[kTsW], int w, const float (*__restrict__ sU)[kTsW],
+                                              const float* __restrict__ sR) {
+  if (K < w) {
+    const float4* u4 = reinterpret_cast<const float4*>(sU[K]);
+    float u[kTsW];
+#pragma unroll
+    for (int j4 = K / 4; j4 < kTsW / 4; ++j4) {
+      const float4 v = u4[j4];
+      u[4 * j4] = v.x;
+      u[4 * j4 + 1] = v.y;
+      u[4 * j4 + 2] = v.z;
+      u[4 * j4 + 3] = v.w;
+    }
+    x[K] *= sR[K];
+#pragma unroll
+    for (int j = K + 1; j < kTsW; ++j) x[j] -= x[K] * u[j];
+  }
+}
+
+template <int... Ks>
+__device__ __forceinline__ void tf_solve(std::integer_sequence<int, Ks...>, float (&x)[kTsW], int w,
+                                         const float (*__restrict__ sU)[kTsW], const float* __restrict__ sR) {
+  (tf_solve_step<Ks>(x, w, sU, sR), ...);
+}
+
+__global__ __launch_bounds__(256) void k_tslu_finish(float* __restrict__ W, long long ld, long long N, long long c0,
+                                                     int w, long long p0, long long p1,
+                                                     const float* __restrict__ lu_in,
+                                                     const float* __restrict__ top_in, const int* __restrict__ meta,
+                                                     int row_blocks) {
+  __shared__ __attribute__((aligned(16))) float sU[kTsW][kTsW];
+  __builtin_amdgcn_s_setprio(3);
+  __shared__ float sR[kTsW];
+  __shared__ int sdpos[kTsW], sdsrc[kTsW], stop[kTsW];
+  __shared__ int snd;
+  const int tid = threadIdx.x;
+  if (tid < kTsW) {
+    const int dp = meta[32 + tid], ds = meta[64 + tid], ts = meta[128 + tid];
+    sdpos[tid] = dp;
+    sdsrc[tid] = (ds >= c0 && ds < c0 + w && dp >= c0 + w && dp < N) ? ds : -1;
+    stop[tid] = (tid < w && ts >= c0 && ts < N) ? ts : (int)c0 + tid;
+  }
+  if (tid == 0) snd = min(max(meta[96], 0), kTsW);
+  if ((int)blockIdx.x >= row_blocks) {
+    __syncthreads();
+    const long long c = p0 + (long long)(blockIdx.x - row_blocks) * 256 + tid;
+    if (c >= p1 || (c >= c0 && c < c0 + w)) return;
+    const int nd = snd;
+    float vt[kTsW], vd[kTsW];
+#pragma unroll
+    for (int k = 0; k < kTsW; ++k) {
+      vt[k] = (k < w) ? W[(long long)stop[k] * ld + c] : 0.0f;
+      vd[k] = (k < nd && sdsrc[k] >= 0) ? W[(long long)sdsrc[k] * ld + c] : 0.0f;
+    }
+    // every read of the column lands before its first write
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+#pragma unroll
+    for (int k = 0; k < kTsW; ++k) {
+      if (k < w) W[(c0 + k) * ld + c] = vt[k];
+      if (k < nd && sdsrc[k] >= 0) W[(long long)sdpos[k] * ld + c] = vd[k];
+    }
+    return;
+  }
+  {
+    float v[kTsW * kTsW / 256];
+#pragma unroll
+    for (int q = 0; q < kTsW * kTsW / 256; ++q) {
+      const int i = tid + 256 * q, r = i / kTsW, j = i % kTsW;
+      v[q] = 0.0f;
+      if (r < w && j < w) v[q] = lu_in[i];
+    }
+#pragma unroll
+    for (int q = 0; q < kTsW * kTsW / 256; ++q) {
+      const int i = tid + 256 * q, r = i / kTsW, j = i % kTsW;
+      sU[r][j] = (j >= r) ? v[q] : 0.0f;
+      if (blockIdx.x == 0 && r < w && j < w) W[(c0 + r) * ld + c0 + j] = v[q];
+    }
+  }
+  __syncthreads();
+  if (tid < kTsW) {
+    const float d = sU[tid][tid];
+    sR[tid] = (tid < w && d != 0.0f) ? 1.0f / d : 0.0f;
+  }
+  __syncthreads();
+  const long long r = c0 + w + (long long)blockIdx.x * 256 + tid;
+  if (r >= N) return;
+  int src = -1;
+  const int nd = snd;
+  for (int i = 0; i < nd; ++i)
+    if (sdpos[i] == r) src = sdsrc[i];
+  float x[kTsW];
+  if (src >= 0) {
+    const float4* t4 = reinterpret_cast<const float4*>(top_in + (src - c0) * kTsW);
+#pragma unroll
+    for (int j4 = 0; j4 < kTsW / 4; ++j4) {
+      const float4 t = t4[j4];
+      x[4 * j4] = t.x;
+      x[4 * j4 + 1] = t.y;
+      x[4 * j4 + 2] = t.z;
+      x[4 * j4 + 3] = t.w;
+    }
+  } else {
+    ts_load_row(W, ld, r, c0, w, x);
+  }
+  tf_solve(std::make_integer_sequence<int, kTsW>{}, x, w, sU, sR);
+  float* dst = W + r * ld + c0;
+  for (int j = 0; j < w; ++j) dst[j] = x[j];
+}
+
+}  // namespace qd
