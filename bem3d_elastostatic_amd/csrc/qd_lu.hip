@@ -70,6 +70,7 @@ struct qd_lu_state {
   qd::DevBuf<int> ts_meta;       // interchanges of a leaf
   qd::DevBuf<float> ts_lu;       // LU of a leaf's winners block (32 x 32)
   qd::DevBuf<float> ts_top;      // a leaf's original top rows (32 x 32)
+  qd::DevBuf<float> ts_linv;     // inverses of the leaves' unit lower blocks (32 x 32 each)
   qd::DevBuf<float> inv32;       // inverted diagonal blocks of L and U
   qd::DevBuf<double> inv64;
   double anorm = -1.0;           // ||A||_inf of the FP64 matrix
@@ -316,6 +317,149 @@ __global__ __launch_bounds__(256) void k_trsm_small(T* __restrict__ W, long long
 #pragma unroll
   for (int i = 0; i < 32; ++i)
     if (i < m) W[(r0 + i) * ld + c] = x[i];
+}
+
+// X = L^-1 B in one launch for the FP32 factorisation: B = W[r0:r0+m, c0:c1),
+// L unit lower at W[r0:r0+m, r0:r0+m] (m a multiple of 32) whose 32 x 32
+// diagonal blocks are the tournament leaves, with their inverses in linv
+// (block at rows r0+32b: linv + (r0/32 + b) * 1024, row-major).  One workgroup
+// per strip of kSW columns, held in LDS (m x kSW floats).  Block by block, the
+// strip's 32 rows are solved with the block's inverse plus one correction
+// (x += inv (b - L x), which brings the explicit inverse to the accuracy of
+// substitution), and the rows below subtract L[rows, block] times them.  Those
+// rows go in chunks of kStripChunk whose L segments are staged through LDS by
+// coalesced loads, the next chunk's loads in flight while one is used.
+constexpr int kStripChunk = 128;
+
+template <int kSW>
+__global__ __launch_bounds__(256) void k_trsm_strip(float* __restrict__ W, long long ld, long long r0, int m,
+                                                    long long c0, long long c1, const float* __restrict__ linv) {
+  extern __shared__ float xs[];  // m x kSW
+  __shared__ float sinv[32][33], sl[32][33];
+  __shared__ float sb[32][kSW + 1];
+  __shared__ float lbuf[2][kStripChunk][33];
+  constexpr int kSlots = 256 / kSW, kPer = 32 * kSW / 256;
+  constexpr int kRowsPer = kStripChunk / kSlots;  // rows of a chunk per thread
+  const int tid = threadIdx.x, c = tid % kSW, slot = tid / kSW;
+  const long long col = c0 + (long long)blockIdx.x * kSW + c;
+  const bool cv = col < c1;
+  for (int r = slot; r < m; r += kSlots) xs[r * kSW + c] = cv ? W[(r0 + r) * ld + col] : 0.0f;
+  const float* inv0 = linv + (r0 / 32) * 1024;
+  // a chunk's L segments: 128 rows x 32 floats, four float4 per thread
+  auto load_chunk = [&](long long d0, int first, float4 (&v)[4]) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = tid + 256 * u, rr = e >> 3, k4 = e & 7;
+      const int row = min(first + rr, m - 1);
+      v[u] = *reinterpret_cast<const float4*>(W + (r0 + row) * ld + d0 + 4 * k4);
+    }
+  };
+  auto stage_chunk = [&](int buf, const float4 (&v)[4]) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = tid + 256 * u, rr = e >> 3, k4 = e & 7;
+      lbuf[buf][rr][4 * k4] = v[u].x;
+      lbuf[buf][rr][4 * k4 + 1] = v[u].y;
+      lbuf[buf][rr][4 * k4 + 2] = v[u].z;
+      lbuf[buf][rr][4 * k4 + 3] = v[u].w;
+    }
+  };
+  // a diagonal block's inverse and L, four elements of each per thread
+  float pinv[4], pl[4];
+  auto load_diag = [&](int i) {
+    const long long d0 = r0 + 32 * i;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = tid + 256 * u, rr = e >> 5, kk = e & 31;
+      pinv[u] = inv0[(long long)i * 1024 + e];
+      pl[u] = (kk < rr) ? W[(d0 + rr) * ld + d0 + kk] : (kk == rr ? 1.0f : 0.0f);
+    }
+  };
+  load_diag(0);
+  for (int i = 0; i < m / 32; ++i) {
+    const long long d0 = r0 + 32 * i;  // the diagonal block's first row and column
+    const int below = 32 * (i + 1), nch = (m - below + kStripChunk - 1) / kStripChunk;
+    float4 pf[4];
+    if (nch > 0) load_chunk(d0, below, pf);  // in flight during the block solve
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = tid + 256 * u, rr = e >> 5, kk = e & 31;
+      sinv[rr][kk] = pinv[u];
+      sl[rr][kk] = pl[u];
+    }
+    if (i + 1 < m / 32) load_diag(i + 1);  // in flight until the next block
+    __syncthreads();
+    // x = inv b, then x += inv (b - L x)
+    float bv[kPer], xv[kPer];
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int e = tid + 256 * q, k = e / kSW, cc = e % kSW;
+      bv[q] = xs[(32 * i + k) * kSW + cc];
+      float acc = 0.0f;
+#pragma unroll
+      for (int j = 0; j < 32; ++j) acc += (j <= k) ? sinv[k][j] * xs[(32 * i + j) * kSW + cc] : 0.0f;
+      xv[q] = acc;
+    }
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int e = tid + 256 * q, k = e / kSW, cc = e % kSW;
+      sb[k][cc] = xv[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int e = tid + 256 * q, k = e / kSW, cc = e % kSW;
+      float res = bv[q];
+#pragma unroll
+      for (int j = 0; j < 32; ++j) res -= (j <= k) ? sl[k][j] * sb[j][cc] : 0.0f;
+      bv[q] = res;  // the residual of row k
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int e = tid + 256 * q, k = e / kSW, cc = e % kSW;
+      xs[(32 * i + k) * kSW + cc] = bv[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int e = tid + 256 * q, k = e / kSW, cc = e % kSW;
+      float acc = xv[q];
+#pragma unroll
+      for (int j = 0; j < 32; ++j) acc += (j <= k) ? sinv[k][j] * xs[(32 * i + j) * kSW + cc] : 0.0f;
+      bv[q] = acc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int e = tid + 256 * q, k = e / kSW, cc = e % kSW;
+      xs[(32 * i + k) * kSW + cc] = bv[q];
+    }
+    __syncthreads();
+    // the rows below, chunk by chunk: X_r -= L[r, block i] X_i
+    float xi[32];
+#pragma unroll
+    for (int k = 0; k < 32; ++k) xi[k] = xs[(32 * i + k) * kSW + c];
+    if (nch > 0) stage_chunk(0, pf);
+    __syncthreads();
+    for (int ch = 0; ch < nch; ++ch) {
+      const int first = below + ch * kStripChunk;
+      if (ch + 1 < nch) load_chunk(d0, first + kStripChunk, pf);
+      const int buf = ch & 1;
+#pragma unroll
+      for (int t = 0; t < kRowsPer; ++t) {
+        const int rr = slot + kSlots * t, r = first + rr;
+        float sum = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 32; ++k) sum += lbuf[buf][rr][k] * xi[k];
+        if (r < m) xs[r * kSW + c] -= sum;
+      }
+      if (ch + 1 < nch) stage_chunk(buf ^ 1, pf);
+      __syncthreads();
+    }
+  }
+  if (cv)
+    for (int r = slot; r < m; r += kSlots) W[(r0 + r) * ld + col] = xs[r * kSW + c];
 }
 
 // ---------------------------------------------------------------------------
@@ -850,6 +994,8 @@ struct LuCtx {
 };
 
 constexpr double kF16MinWork = 2.0e9;
+constexpr int kStripMaxRows = 1024;      // LDS of k_trsm_strip<8>: 32 KiB
+constexpr long long kStripNarrowCols = 8192;
 
 // reuse_a16: A's FP16 copy is already in c.sc->h16a (the same A as the
 // previous call on this context's stream).
@@ -910,6 +1056,25 @@ static int gemm_sub(LuCtx<T>& c, long long M, long long Nn, int K, const T* A, c
 template <typename T>
 static int trsm_lower(LuCtx<T>& c, long long r0, int m, long long c0, long long c1) {
   if (c1 <= c0 || m <= 0) return QD_OK;
+  if constexpr (std::is_same<T, float>::value) {
+    // One strip solve when there are few columns (narrow strips: more
+    // workgroups); for many columns the recursion below stops at 256 rows,
+    // where wide strips re-read little of L, and the GEMMs between carry the
+    // bulk of the work.
+    const long long n = c1 - c0;
+    if (c.calu && m % 32 == 0 && r0 % 32 == 0 && (n <= kStripNarrowCols ? m <= kStripMaxRows : m <= 256)) {
+      if (n <= kStripNarrowCols) {
+        hipLaunchKernelGGL(k_trsm_strip<8>, dim3((unsigned)((n + 7) / 8)), dim3(256), (size_t)m * 8 * sizeof(float),
+                           c.st, c.W, c.ld, r0, m, c0, c1, (const float*)c.s->ts_linv.p);
+      } else {
+        hipLaunchKernelGGL(k_trsm_strip<32>, dim3((unsigned)((n + 31) / 32)), dim3(256),
+                           (size_t)m * 32 * sizeof(float), c.st, c.W, c.ld, r0, m, c0, c1,
+                           (const float*)c.s->ts_linv.p);
+      }
+      QD_HIP(c.h, hipGetLastError());
+      return QD_OK;
+    }
+  }
   if (m <= 32) {
     hipLaunchKernelGGL(k_trsm_small<T>, dim3((unsigned)((c1 - c0 + 255) / 256)), dim3(256), 0, c.st, c.W,
                        c.ld, r0, m, c0, c1);
@@ -955,9 +1120,9 @@ static int tslu_leaf(LuCtx<float>& c, long long c0, int w, long long p0, long lo
   const long long below = std::max<long long>(m - w, 1);
   const int row_blocks = (int)((below + 255) / 256);
   const int col_blocks = (int)((p1 - p0 + 255) / 256);
-  hipLaunchKernelGGL(k_tslu_finish, dim3((unsigned)(row_blocks + col_blocks)), dim3(256), 0, c.st, c.W, c.ld, c.N,
+  hipLaunchKernelGGL(k_tslu_finish, dim3((unsigned)(row_blocks + col_blocks + 1)), dim3(256), 0, c.st, c.W, c.ld, c.N,
                      c0, w, p0, p1, (const float*)s->ts_lu.p, (const float*)s->ts_top.p, (const int*)s->ts_meta.p,
-                     row_blocks);
+                     row_blocks, s->ts_linv.p + (c0 / kTsW) * kTsW * kTsW);
   QD_HIP(c.h, hipGetLastError());
   return QD_OK;
 }
@@ -1039,6 +1204,13 @@ static int factor(LuCtx<T>& c, int NB) {
     QD_ALLOC(c.h, s->ts_meta, (size_t)kTsMeta);
     QD_ALLOC(c.h, s->ts_lu, (size_t)kTsW * kTsW);
     QD_ALLOC(c.h, s->ts_top, (size_t)kTsW * kTsW);
+    QD_ALLOC(c.h, s->ts_linv, (size_t)((N + kTsW - 1) / kTsW + 1) * kTsW * kTsW);
+    // the strip solves keep their strips in dynamic LDS
+    QD_HIP(c.h, hipFuncSetAttribute(reinterpret_cast<const void*>(&k_trsm_strip<32>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, 256 * 32 * (int)sizeof(float)));
+    QD_HIP(c.h, hipFuncSetAttribute(reinterpret_cast<const void*>(&k_trsm_strip<8>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    kStripMaxRows * 8 * (int)sizeof(float)));
     if (fresh) QD_HIP(c.h, hipMemsetAsync(s->ts_cnt.p, 0, s->ts_cnt.count * sizeof(unsigned), c.st));
   }
   if (c.f16) {

@@ -290,7 +290,8 @@ __global__ __launch_bounds__(64) void k_tslu_tree(const float* __restrict__ W, l
 // L21 = A21 U11^-1 (one row per thread, U11 read from LDS by broadcast),
 // against the root's winners' LU, which workgroup 0 also writes into the top
 // rows; then, in workgroups row_blocks.., the permutation on the panel's
-// columns [p0, p1) outside the leaf.  No workgroup reads what another writes:
+// columns [p0, p1) outside the leaf; the last workgroup inverts the leaf's
+// unit lower block into linv_out.  No workgroup reads what another writes:
 // rows replaced by the interchanges read the root's copy of the top rows.
 template <int K>
 __device__ __forceinline__ void tf_solve_step(float (&x)[kTsW], int w, const float (*__restrict__ sU)[kTsW],
@@ -322,7 +323,7 @@ __global__ __launch_bounds__(256) void k_tslu_finish(float* __restrict__ W, long
                                                      int w, long long p0, long long p1,
                                                      const float* __restrict__ lu_in,
                                                      const float* __restrict__ top_in, const int* __restrict__ meta,
-                                                     int row_blocks) {
+                                                     int row_blocks, float* __restrict__ linv_out) {
   __shared__ __attribute__((aligned(16))) float sU[kTsW][kTsW];
   __builtin_amdgcn_s_setprio(3);
   __shared__ float sR[kTsW];
@@ -336,6 +337,27 @@ __global__ __launch_bounds__(256) void k_tslu_finish(float* __restrict__ W, long
     stop[tid] = (tid < w && ts >= c0 && ts < N) ? ts : (int)c0 + tid;
   }
   if (tid == 0) snd = min(max(meta[96], 0), kTsW);
+  if (blockIdx.x == gridDim.x - 1) {
+    // the last workgroup: inverse of the winners' unit lower factor
+    // (identity beyond w), one column per thread by forward substitution,
+    // for the later triangular solves with this block
+    __shared__ float sL[kTsW][kTsW + 1], sI[kTsW][kTsW + 1];
+    for (int i = tid; i < kTsW * kTsW; i += 256) {
+      const int r = i / kTsW, j = i % kTsW;
+      sL[r][j] = (j < r && r < w) ? lu_in[i] : 0.0f;
+    }
+    __syncthreads();
+    if (tid < kTsW) {
+      for (int r = 0; r < kTsW; ++r) {
+        float acc = (r == tid) ? 1.0f : 0.0f;
+        for (int k = tid; k < r; ++k) acc -= sL[r][k] * sI[k][tid];
+        sI[r][tid] = acc;
+      }
+    }
+    __syncthreads();
+    for (int i = tid; i < kTsW * kTsW; i += 256) linv_out[i] = sI[i / kTsW][i % kTsW];
+    return;
+  }
   if ((int)blockIdx.x >= row_blocks) {
     __syncthreads();
     const long long c = p0 + (long long)(blockIdx.x - row_blocks) * 256 + tid;
