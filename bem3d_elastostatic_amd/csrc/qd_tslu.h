@@ -341,21 +341,26 @@ __global__ __launch_bounds__(256) void k_tslu_finish(float* __restrict__ W, long
     // the last workgroup: inverse of the winners' unit lower factor
     // (identity beyond w), one column per thread by forward substitution,
     // for the later triangular solves with this block
-    __shared__ float sL[kTsW][kTsW + 1], sI[kTsW][kTsW + 1];
+    __shared__ float sL[kTsW][kTsW + 1];
     for (int i = tid; i < kTsW * kTsW; i += 256) {
       const int r = i / kTsW, j = i % kTsW;
       sL[r][j] = (j < r && r < w) ? lu_in[i] : 0.0f;
     }
     __syncthreads();
     if (tid < kTsW) {
+      // column tid in registers (entries above the diagonal come out zero);
+      // the LDS reads of a row do not depend on the column and go together
+      float y[kTsW];
+#pragma unroll
       for (int r = 0; r < kTsW; ++r) {
         float acc = (r == tid) ? 1.0f : 0.0f;
-        for (int k = tid; k < r; ++k) acc -= sL[r][k] * sI[k][tid];
-        sI[r][tid] = acc;
+#pragma unroll
+        for (int k = 0; k < r; ++k) acc -= sL[r][k] * y[k];
+        y[r] = acc;
       }
+#pragma unroll
+      for (int r = 0; r < kTsW; ++r) linv_out[r * kTsW + tid] = y[r];
     }
-    __syncthreads();
-    for (int i = tid; i < kTsW * kTsW; i += 256) linv_out[i] = sI[i / kTsW][i % kTsW];
     return;
   }
   if ((int)blockIdx.x >= row_blocks) {

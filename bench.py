@@ -185,6 +185,24 @@ def trailing_shapes(N, precision):
     return out
 
 
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                            "lu_trailing_gemm16_traffic.txt")
+
+
+def measured_traffic(N, precision):
+    """HBM bytes per trailing-update launch from the committed counter passes
+    (FETCH_SIZE and WRITE_SIZE over the replayed launches, each scaled by the
+    same counter's reading for a copy of known size), for the workload they
+    were taken on (FP16 update, 76,806 DOF); None otherwise."""
+    if precision != "fp16" or N != 76806 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        f = line.split()
+        if len(f) == 2 and f[0] == "traffic_bytes_per_launch":
+            return float(f[1])
+    return None
+
+
 def trailing_roofline(N, precision, d):
     """Roofline of the outer trailing update from its algorithmic work: 2 M Nn K
     flops and, in bytes, the C read and write (8 per FP32 element) plus one
@@ -202,7 +220,7 @@ def trailing_roofline(N, precision, d):
     peak_f = PEAK_TFLOPS[precision]
     t_mfma = flops / (peak_f * 1e12)
     t_hbm = nbytes / (HBM_PEAK_GBS * 1e9)
-    common = dict(kernel="lu_trailing_gemm", traffic=None, launches=d["launches"],
+    common = dict(kernel="lu_trailing_gemm", traffic=measured_traffic(N, precision), launches=d["launches"],
                   avg_launch_ms=d["ms"] / max(d["launches"], 1), flops_per_launch=flops / len(shapes),
                   bytes_per_launch=nbytes / len(shapes), floor_s=dict(mfma=t_mfma, hbm=t_hbm))
     if t_hbm >= t_mfma:
@@ -432,7 +450,7 @@ def main():
                            tflops=(work / (ms * 1e-3) / 1e12) if ms > 0 else None)
     extra["kernels"] = prof_out
     # Dominant single kernel by device time.  lu_panel is excluded: it is the
-    # panel factorisation's whole sequence (about a thousand small dependent
+    # panel factorisation's whole sequence (about 150 small dependent
     # launches per panel), whose time is reported in "kernels" and "phases_s".
     dom = max((k for k in prof_out if k != "lu_panel"), key=lambda k: prof_out[k]["ms"])
     d = prof_out[dom]
