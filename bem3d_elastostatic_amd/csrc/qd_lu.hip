@@ -788,76 +788,132 @@ __global__ __launch_bounds__(256) void k_cvt_b16t(int K, long long N, int Kp, co
 // bands, one per XCD (workgroups are dispatched to the XCDs round-robin), so
 // an XCD's L2 serves the A/B rows its tiles share.  A grid smaller than the
 // tile count leaves CU capacity to work on another stream.
-constexpr int H_BM = 256, H_BN = 128, H_BK = 32, H_PAD = 8;
+#ifndef QD_G16_BM
+#define QD_G16_BM 256
+#endif
+#ifndef QD_G16_OCC
+#define QD_G16_OCC 2
+#endif
+// tile of H_BM x 128: four waves of (H_BM / 2) x 64, kG16Mi MFMA tiles of 16
+// rows each per wave
+constexpr int H_BM = QD_G16_BM, H_BN = 128, H_BK = 32;
+constexpr int kG16Mi = H_BM / 32;
 #ifndef QD_G16_GROUP
 #define QD_G16_GROUP 4
 #endif
 constexpr unsigned kG16Group = QD_G16_GROUP;
+#ifndef QD_G16_STAGES
+#define QD_G16_STAGES 2
+#endif
+// Operand tiles go straight from global memory to LDS (global_load_lds_dwordx4:
+// lane l of an instruction writes 16 bytes at the instruction's LDS address +
+// 16 l), kG16Stages K steps in flight.  A stage holds A (256 rows) and B (128
+// rows) as 64-byte rows of four 16-byte chunks; chunk c of row r sits in slot
+// c ^ g16_swz(r), which keeps the 16-lane fragment reads of one MFMA operand
+// on distinct banks.
+constexpr int kG16Stages = QD_G16_STAGES;
+constexpr int kG16StageBytes = (H_BM + H_BN) * H_BK * 2;
+constexpr int kG16Loads = H_BM / 64 + 2;  // load instructions per stage and thread
+static_assert(2 * kG16Loads < 16, "vmcnt immediates below use the low four bits");
 struct G16Smem {
-  _Float16 sA[2][H_BM][H_BK + H_PAD];
-  _Float16 sB[2][H_BN][H_BK + H_PAD];
+  unsigned char stage[kG16Stages][kG16StageBytes];
 };
+static_assert(4 * 32 * 68 * sizeof(float) <= sizeof(G16Smem), "epilogue staging must fit in the operand stages");
+static_assert(H_BM == 128 || H_BM == 256, "tile height");
+
+__device__ __forceinline__ int g16_swz(int row) { return ((row >> 1) ^ (row >> 3)) & 3; }
+
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
+
+// Issues one stage's loads: A rows m0.., B rows n0.., K columns [k0, k0+32).
+// Rows past M (N) read row M-1 (N-1): they feed only outputs that are not
+// stored.  H_BM/64 + 2 instructions per thread, each covering 16 rows of A or B.
+__device__ __forceinline__ void g16_issue(unsigned char* __restrict__ st, long long m0, long long n0, long long M,
+                                          long long N, int Kp, int k0, const _Float16* __restrict__ A16,
+                                          const _Float16* __restrict__ B16, int wave, int lane) {
+  const int rl = lane >> 2, cs = lane & 3;  // row within the instruction, chunk slot
+#pragma unroll
+  for (int q = 0; q < H_BM / 64; ++q) {
+    const int row = (q * 4 + wave) * 16 + rl;
+    const long long g = min(m0 + row, M - 1);
+    const int kc = cs ^ g16_swz(row);
+    __builtin_amdgcn_global_load_lds(A16 + g * Kp + k0 + 8 * kc, (lds_ptr_t)(st + (q * 4 + wave) * 1024), 16, 0, 0);
+  }
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int row = (q * 4 + wave) * 16 + rl;
+    const long long g = min(n0 + row, N - 1);
+    const int kc = cs ^ g16_swz(row);
+    __builtin_amdgcn_global_load_lds(B16 + g * Kp + k0 + 8 * kc,
+                                     (lds_ptr_t)(st + H_BM * 64 + (q * 4 + wave) * 1024), 16, 0, 0);
+  }
+}
 
 __device__ __forceinline__ void gemm16_tile(long long m0, long long n0, long long M, long long N, int Kp,
                                             const _Float16* __restrict__ A16, const _Float16* __restrict__ B16,
                                             const float* __restrict__ colscale, float* __restrict__ C, long long ldc,
                                             bool rows16, G16Smem& sm) {
-  auto& sA = sm.sA;
-  auto& sB = sm.sB;
   const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
-  const int wm = (wave / 2) * 128, wn = (wave % 2) * 64;
+  const int wm = (wave / 2) * (H_BM / 2), wn = (wave % 2) * 64;
   const int lr = lane % 16, lk = lane / 16;
-  f4 acc[8][4];
+  f4 acc[kG16Mi][4];
 #pragma unroll
-  for (int i = 0; i < 8; ++i)
+  for (int i = 0; i < kG16Mi; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = (f4){0, 0, 0, 0};
-  uint4 ra[4], rb[2];
-  // four lanes per 64-byte row segment: one instruction covers 16 rows
-  const int lrow = tid >> 2, lchunk = (tid & 3) * 8;
-  auto gload = [&](int k0) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const long long g = m0 + lrow + 64 * q;
-      ra[q] = (g < M) ? *(const uint4*)(A16 + g * Kp + k0 + lchunk) : make_uint4(0, 0, 0, 0);
-    }
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const long long g = n0 + lrow + 64 * q;
-      rb[q] = (g < N) ? *(const uint4*)(B16 + g * Kp + k0 + lchunk) : make_uint4(0, 0, 0, 0);
-    }
-  };
-  auto lstore = [&](int buf) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) *(uint4*)&sA[buf][lrow + 64 * q][lchunk] = ra[q];
-#pragma unroll
-    for (int q = 0; q < 2; ++q) *(uint4*)&sB[buf][lrow + 64 * q][lchunk] = rb[q];
-  };
-  gload(0);
-  lstore(0);
-  __syncthreads();
   const int nk = Kp / H_BK;
+  // fragment offsets within a stage (bytes): A row wm+16i+lr, B row wn+16j+lr, chunk lk
+  int offa[kG16Mi], offb[4];
+#pragma unroll
+  for (int i = 0; i < kG16Mi; ++i) {
+    const int row = wm + 16 * i + lr;
+    offa[i] = row * 64 + 16 * (lk ^ g16_swz(row));
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int row = wn + 16 * j + lr;
+    offb[j] = H_BM * 64 + row * 64 + 16 * (lk ^ g16_swz(row));
+  }
+  // every wave is done with the stages (previous tile, epilogue staging)
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < kG16Stages - 1; ++s)
+    if (s < nk) g16_issue(sm.stage[s], m0, n0, M, N, Kp, s * H_BK, A16, B16, wave, lane);
   for (int kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
-    if (kt + 1 < nk) gload((kt + 1) * H_BK);
+    // this step's stage has landed once at most the later stages' loads are
+    // outstanding (kG16Loads per stage and thread)
+    const int later = min(nk - 1 - kt, kG16Stages - 2);
+    if (later >= 2) __builtin_amdgcn_s_waitcnt(0x0f70 | (2 * kG16Loads));
+    else if (later == 1) __builtin_amdgcn_s_waitcnt(0x0f70 | kG16Loads);
+    else __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0)
+    // A plain barrier: __syncthreads() would also wait for every load in
+    // flight (its release fence), which is the prefetch itself.  The previous
+    // step's LDS reads have completed (the MFMAs consumed them), so the stage
+    // the loads below overwrite is free.
+    asm volatile("s_barrier" ::: "memory");
+    if (kt + kG16Stages - 1 < nk)
+      g16_issue(sm.stage[(kt + kG16Stages - 1) % kG16Stages], m0, n0, M, N, Kp, (kt + kG16Stages - 1) * H_BK, A16,
+                B16, wave, lane);
+    const unsigned char* st = sm.stage[kt % kG16Stages];
     h8 b[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) b[j] = *(const h8*)&sB[cur][wn + 16 * j + lr][8 * lk];
+    for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const h8*>(st + offb[j]);
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const h8 a = *(const h8*)&sA[cur][wm + 16 * i + lr][8 * lk];
+    for (int i = 0; i < kG16Mi; ++i) {
+      const h8 a = *reinterpret_cast<const h8*>(st + offa[i]);
 #pragma unroll
       for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b[j], acc[i][j], 0, 0, 0);
     }
-    if (kt + 1 < nk) lstore(cur ^ 1);
-    __syncthreads();
   }
+  __syncthreads();
   // Interior tiles with 16-byte aligned C rows: each wave stages its
-  // 128 x 64 result through LDS (the operand buffers are free now) in slices
-  // of 32 rows, then reads it back row-contiguously, so that C is read and
+  // (H_BM/2) x 64 result through LDS (the operand buffers are free now) in
+  // slices of 32 rows, then reads it back row-contiguously, so that C is read and
   // written in float4s covering 256 contiguous bytes per 16 lanes (the MFMA
-  // layout alone gives 64-byte pieces).  The wave's 64 columns share one
-  // column scale.
+  // layout alone gives 64-byte pieces).  The C accesses are non-temporal: C
+  // is not touched again in the launch, and kept out of L2 it leaves room for
+  // the operand tiles that neighbouring workgroups share.  The wave's 64
+  // columns share one column scale.
   if (rows16 && m0 + H_BM <= M && n0 + H_BN <= N) {
     constexpr int kPitch = 68;  // floats per staged row
     float* stage = reinterpret_cast<float*>(&sm) + wave * (32 * kPitch);
@@ -869,7 +925,7 @@ __device__ __forceinline__ void gemm16_tile(long long m0, long long n0, long lon
     const int c4 = lane & 15;
     float* cbase = C + (m0 + wm + (lane >> 4)) * ld + n0 + wn + 4 * c4;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < kG16Mi / 2; ++q) {
 #pragma unroll
       for (int ii = 0; ii < 2; ++ii)
 #pragma unroll
@@ -885,7 +941,10 @@ __device__ __forceinline__ void gemm16_tile(long long m0, long long n0, long lon
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
           const int row = (lane >> 4) + 4 * (4 * h + t);
-          cv[t] = *reinterpret_cast<const float4*>(cbase + (32 * q + 4 * (4 * h + t)) * ld);
+          {
+            const f4 v = __builtin_nontemporal_load(reinterpret_cast<const f4*>(cbase + (32 * q + 4 * (4 * h + t)) * ld));
+            cv[t] = make_float4(v[0], v[1], v[2], v[3]);
+          }
           av[t] = *reinterpret_cast<const float4*>(stage + row * kPitch + 4 * c4);
         }
 #pragma unroll
@@ -895,7 +954,8 @@ __device__ __forceinline__ void gemm16_tile(long long m0, long long n0, long lon
           o.y = cv[t].y - s * av[t].y;
           o.z = cv[t].z - s * av[t].z;
           o.w = cv[t].w - s * av[t].w;
-          *reinterpret_cast<float4*>(cbase + (32 * q + 4 * (4 * h + t)) * ld) = o;
+          __builtin_nontemporal_store((f4){o.x, o.y, o.z, o.w},
+                                      reinterpret_cast<f4*>(cbase + (32 * q + 4 * (4 * h + t)) * ld));
         }
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -911,13 +971,13 @@ __device__ __forceinline__ void gemm16_tile(long long m0, long long n0, long lon
     for (int j = 0; j < 4; ++j) {
       const long long gc = n0 + wn + 16 * j + lr;
       const float s = colscale[gc >> 6];
-      float cv[8][4];
+      float cv[kG16Mi][4];
 #pragma unroll
-      for (int i = 0; i < 8; ++i)
+      for (int i = 0; i < kG16Mi; ++i)
 #pragma unroll
         for (int r = 0; r < 4; ++r) cv[i][r] = C[(m0 + wm + 16 * i + 4 * lk + r) * ldc + gc];
 #pragma unroll
-      for (int i = 0; i < 8; ++i)
+      for (int i = 0; i < kG16Mi; ++i)
 #pragma unroll
         for (int r = 0; r < 4; ++r) C[(m0 + wm + 16 * i + 4 * lk + r) * ldc + gc] = cv[i][r] - s * acc[i][j][r];
     }
@@ -928,9 +988,9 @@ __device__ __forceinline__ void gemm16_tile(long long m0, long long n0, long lon
   for (int j = 0; j < 4; ++j) {
     const long long gc = n0 + wn + 16 * j + lr;
     const float s = (gc < N) ? colscale[gc >> 6] : 0.0f;
-    float cv[8][4];
+    float cv[kG16Mi][4];
 #pragma unroll
-    for (int i = 0; i < 8; ++i)
+    for (int i = 0; i < kG16Mi; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const long long gr = m0 + wm + 16 * i + 4 * lk + r;
@@ -938,7 +998,7 @@ __device__ __forceinline__ void gemm16_tile(long long m0, long long n0, long lon
         if (gr < M && gc < N) cv[i][r] = C[gr * ldc + gc];
       }
 #pragma unroll
-    for (int i = 0; i < 8; ++i)
+    for (int i = 0; i < kG16Mi; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const long long gr = m0 + wm + 16 * i + 4 * lk + r;
@@ -949,7 +1009,8 @@ __device__ __forceinline__ void gemm16_tile(long long m0, long long n0, long lon
 
 // rows16: C and ldc keep every row start 16-byte aligned.
 template <int kRole>
-__global__ __launch_bounds__(256, 2) void k_gemm16(long long M, long long N, int Kp, const _Float16* __restrict__ A16,
+__global__ __launch_bounds__(256, QD_G16_OCC) void k_gemm16(long long M, long long N, int Kp,
+                                                            const _Float16* __restrict__ A16,
                                                 const _Float16* __restrict__ B16, const float* __restrict__ colscale,
                                                 float* __restrict__ C, long long ldc, int rows16) {
   __shared__ __attribute__((aligned(16))) G16Smem sm;
@@ -993,6 +1054,21 @@ struct LuCtx {
   qd_lu_state::F16Scratch* sc = nullptr;      // FP16 operand copies of this context
 };
 
+// The FP16 update C -= (A16 B16^T) * colscale on stream st; max_wgs > 0
+// caps the workgroups (each then loops over several tiles).
+static int launch_f16_update(qd_context* h, bool trailing, long long M, long long N, int Kp, const _Float16* A16,
+                             const _Float16* B16, const float* colscale, float* C, long long ldc, int rows16,
+                             int max_wgs, hipStream_t st) {
+  const long long tiles = ((N + H_BN - 1) / H_BN) * ((M + H_BM - 1) / H_BM);
+  const dim3 g((unsigned)(max_wgs > 0 ? std::min<long long>(tiles, max_wgs) : tiles));
+  if (trailing)
+    hipLaunchKernelGGL(k_gemm16<1>, g, dim3(256), 0, st, M, N, Kp, A16, B16, colscale, C, ldc, rows16);
+  else
+    hipLaunchKernelGGL(k_gemm16<0>, g, dim3(256), 0, st, M, N, Kp, A16, B16, colscale, C, ldc, rows16);
+  QD_HIP(h, hipGetLastError());
+  return QD_OK;
+}
+
 constexpr double kF16MinWork = 2.0e9;
 constexpr int kStripMaxRows = 1024;      // LDS of k_trsm_strip<8>: 32 KiB
 constexpr long long kStripNarrowCols = 8192;
@@ -1005,6 +1081,7 @@ template <typename T>
 static int gemm_sub(LuCtx<T>& c, long long M, long long Nn, int K, const T* A, const T* B, T* C, bool allow_f16,
                     bool trailing = false, bool reuse_a16 = false, int max_wgs = 0) {
   if (M <= 0 || Nn <= 0 || K <= 0) return QD_OK;
+  int rc_ = QD_OK;
   dim3 grid((unsigned)((Nn + GBN - 1) / GBN), (unsigned)((M + GBM - 1) / GBM));
   if constexpr (std::is_same<T, float>::value) {
     // FP16 conversion costs three extra launches; below kF16MinWork flops the
@@ -1022,16 +1099,11 @@ static int gemm_sub(LuCtx<T>& c, long long M, long long Nn, int K, const T* A, c
       if (!reuse_a16)
         hipLaunchKernelGGL(k_cvt_a16, dim3((unsigned)((M * (Kp / 8) + 255) / 256)), dim3(256), 0, c.st, M, K,
                            Kp, A, c.ld, c.sc->h16a.p);
-      const long long tiles = ((Nn + H_BN - 1) / H_BN) * ((M + H_BM - 1) / H_BM);
-      const dim3 g16((unsigned)(max_wgs > 0 ? std::min<long long>(tiles, max_wgs) : tiles));
       ProfScope ps(c.h, trailing ? QD_PROF_LU_TRAILING_GEMM : -1, 2.0 * (double)M * (double)Nn * K, c.st);
       const int rows16 = (c.ld % 4 == 0 && (reinterpret_cast<uintptr_t>(C) & 15) == 0) ? 1 : 0;
-      if (trailing)
-        hipLaunchKernelGGL(k_gemm16<1>, g16, dim3(256), 0, c.st, M, Nn, Kp, c.sc->h16a.p, c.sc->h16b.p,
-                           c.sc->colscale.p, C, c.ld, rows16);
-      else
-        hipLaunchKernelGGL(k_gemm16<0>, g16, dim3(256), 0, c.st, M, Nn, Kp, c.sc->h16a.p, c.sc->h16b.p,
-                           c.sc->colscale.p, C, c.ld, rows16);
+      if ((rc_ = launch_f16_update(c.h, trailing, M, Nn, Kp, c.sc->h16a.p, c.sc->h16b.p, c.sc->colscale.p, C, c.ld,
+                                   rows16, max_wgs, c.st)))
+        return rc_;
       QD_HIP(c.h, hipGetLastError());
       return QD_OK;
     }
@@ -2085,12 +2157,9 @@ void lu_free(qd_context* h) {
 int fp16_update(qd_context* h, long long M, long long N, int Kp, const void* A16, const void* B16,
                 const float* colscale, float* C, long long ldc) {
   if (M <= 0 || N <= 0) return QD_OK;
-  const long long tiles = ((N + H_BN - 1) / H_BN) * ((M + H_BM - 1) / H_BM);
   const int rows16 = (ldc % 4 == 0 && (reinterpret_cast<uintptr_t>(C) & 15) == 0) ? 1 : 0;
-  hipLaunchKernelGGL(k_gemm16<1>, dim3((unsigned)tiles), dim3(256), 0, h->stream, M, N, Kp,
-                     (const _Float16*)A16, (const _Float16*)B16, colscale, C, ldc, rows16);
-  QD_HIP(h, hipGetLastError());
-  return QD_OK;
+  return launch_f16_update(h, true, M, N, Kp, (const _Float16*)A16, (const _Float16*)B16, colscale, C, ldc, rows16, 0,
+                           h->stream);
 }
 
 int lu_invalidate(qd_context* h) {
