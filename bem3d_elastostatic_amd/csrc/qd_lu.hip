@@ -389,16 +389,26 @@ __global__ __launch_bounds__(256) void k_trsm_strip(float* __restrict__ W, long 
     }
     if (i + 1 < m / 32) load_diag(i + 1);  // in flight until the next block
     __syncthreads();
-    // x = inv b, then x += inv (b - L x)
+    // x = inv b, then x += inv (b - L x).  The 32-term sums are unrolled for
+    // the narrow strips (one element per thread) and left as loops for the
+    // wide ones (four per thread), whose unrolled form needs twice the
+    // registers and then no longer fits beside the trailing update's waves.
+    auto lower_dot = [&](const float (*mat)[33], int k, const float* vec, int stride) {
+      float acc = 0.0f;
+      if constexpr (kSW <= 8) {
+#pragma unroll
+        for (int j = 0; j < 32; ++j) acc += (j <= k) ? mat[k][j] * vec[j * stride] : 0.0f;
+      } else {
+        for (int j = 0; j <= k; ++j) acc += mat[k][j] * vec[j * stride];
+      }
+      return acc;
+    };
     float bv[kPer], xv[kPer];
 #pragma unroll
     for (int q = 0; q < kPer; ++q) {
       const int e = tid + 256 * q, k = e / kSW, cc = e % kSW;
       bv[q] = xs[(32 * i + k) * kSW + cc];
-      float acc = 0.0f;
-#pragma unroll
-      for (int j = 0; j < 32; ++j) acc += (j <= k) ? sinv[k][j] * xs[(32 * i + j) * kSW + cc] : 0.0f;
-      xv[q] = acc;
+      xv[q] = lower_dot(sinv, k, xs + (32 * i) * kSW + cc, kSW);
     }
 #pragma unroll
     for (int q = 0; q < kPer; ++q) {
@@ -409,10 +419,7 @@ __global__ __launch_bounds__(256) void k_trsm_strip(float* __restrict__ W, long 
 #pragma unroll
     for (int q = 0; q < kPer; ++q) {
       const int e = tid + 256 * q, k = e / kSW, cc = e % kSW;
-      float res = bv[q];
-#pragma unroll
-      for (int j = 0; j < 32; ++j) res -= (j <= k) ? sl[k][j] * sb[j][cc] : 0.0f;
-      bv[q] = res;  // the residual of row k
+      bv[q] -= lower_dot(sl, k, &sb[0][cc], kSW + 1);  // the residual of row k
     }
     __syncthreads();
 #pragma unroll
@@ -424,10 +431,7 @@ __global__ __launch_bounds__(256) void k_trsm_strip(float* __restrict__ W, long 
 #pragma unroll
     for (int q = 0; q < kPer; ++q) {
       const int e = tid + 256 * q, k = e / kSW, cc = e % kSW;
-      float acc = xv[q];
-#pragma unroll
-      for (int j = 0; j < 32; ++j) acc += (j <= k) ? sinv[k][j] * xs[(32 * i + j) * kSW + cc] : 0.0f;
-      bv[q] = acc;
+      bv[q] = xv[q] + lower_dot(sinv, k, xs + (32 * i) * kSW + cc, kSW);
     }
     __syncthreads();
 #pragma unroll
