@@ -1,6 +1,8 @@
 """qd_fp16_update (the FP16 trailing update of the mixed factorisation as an
 operator) against a plain PyTorch FP32 reference of the same operation:
-C -= colscale[j/64] * A16 B16^T, FP16 operands exact in FP32, FP32 sums."""
+C -= colscale[j/64] * A16 B16^T, FP16 operands exact in FP32, FP32 sums.
+The K = 1024 cases cover ragged M and N edges (N not a multiple of 4), row
+pitches wider than N and launches of many tiles."""
 import ctypes as C
 
 import pytest
@@ -9,7 +11,8 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("M,N,K,pad", [(300, 200, 64, 0), (513, 129, 96, 7), (256, 128, 32, 0),
-                                       (1000, 700, 1024, 64)])
+                                       (1000, 700, 1024, 64), (2000, 1500, 1024, 0), (1000, 701, 1024, 3),
+                                       (4100, 2050, 1024, 62)])
 def test_fp16_update_matches_torch(M, N, K, pad):
     import torch
     from bem3d_elastostatic_amd import _lib
