@@ -70,6 +70,9 @@ struct qd_lu_state {
   qd::DevBuf<int> ts_meta;       // interchanges of a leaf
   qd::DevBuf<float> ts_lu;       // LU of a leaf's winners block (32 x 32)
   qd::DevBuf<float> ts_top;      // a leaf's original top rows (32 x 32)
+  qd::DevBuf<unsigned char> trs_tmp;  // intermediate vector of the block triangular solves
+  qd::DevBuf<int> lb_flags;      // per-block publication flags of the look-back solves (+ error word)
+  int lb_epoch = 0;              // value the flags take in the latest look-back solve
   qd::DevBuf<float> ts_linv;     // inverses of the leaves' unit lower blocks (32 x 32 each)
   qd::DevBuf<float> inv32;       // inverted diagonal blocks of L and U
   qd::DevBuf<double> inv64;
@@ -1475,21 +1478,17 @@ __global__ __launch_bounds__(256) void k_apply_inv(const T* __restrict__ invb, l
 // at or beyond m (the partial last block) are masked before loading, since
 // they may lie past the end of the row.
 constexpr int kBuRows = 64, kBuMaxRhs = 8;
+
+// x[v][rbase .. rbase+kBuRows) (rows below rend skipped) -= W[rows, c0:c0+m]
+// x[v][c0:c0+m], with xr the lane's piece of x[v][c0:c0+m].
 template <typename T, int TB>
-__global__ __launch_bounds__(256) void k_block_update_rows(const T* __restrict__ W, long long ld, long long N,
-                                                           long long r0, long long r1, long long c0, int m, int nrhs,
-                                                           T* __restrict__ x) {
+__device__ __forceinline__ void bu_rows(const T* __restrict__ W, long long ld, long long N, long long rbase,
+                                        long long rend, long long c0, int m, int nrhs, T* __restrict__ x,
+                                        const T (&xr)[kBuMaxRhs][16 / sizeof(T)]) {
   constexpr int E = 16 / sizeof(T);  // elements per lane
-  static_assert(E * 32 == TB, "row piece layout");
   const int sub = threadIdx.x & 31, slot = threadIdx.x >> 5;
   const int col = sub * E;
-  T xr[kBuMaxRhs][E];
-#pragma unroll
-  for (int v = 0; v < kBuMaxRhs; ++v)
-#pragma unroll
-    for (int e = 0; e < E; ++e) xr[v][e] = (v < nrhs && col + e < m) ? x[(long long)v * N + c0 + col + e] : (T)0;
   const bool full = (m == TB);
-  const long long rbase = r0 + (long long)blockIdx.x * kBuRows;
   constexpr int kPass = kBuRows / 8;
   // all rows' loads first, then the reductions
   T w[kPass][E];
@@ -1499,7 +1498,7 @@ __global__ __launch_bounds__(256) void k_block_update_rows(const T* __restrict__
     const T* wr = W + r * ld + c0 + col;
 #pragma unroll
     for (int e = 0; e < E; ++e) w[pass][e] = (T)0;
-    if (r < r1) {
+    if (r < rend) {
       if (full) {
         if constexpr (sizeof(T) == 4) {
           const float4 q = *reinterpret_cast<const float4*>(wr);
@@ -1530,10 +1529,207 @@ __global__ __launch_bounds__(256) void k_block_update_rows(const T* __restrict__
 #pragma unroll
         for (int off = 16; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 32);
         const long long r = rbase + pass * 8 + slot;
-        if (sub == 0 && r < r1) x[(long long)v * N + r] -= sum;
+        if (sub == 0 && r < rend) x[(long long)v * N + r] -= sum;
       }
     }
   }
+}
+
+template <typename T, int TB>
+__device__ __forceinline__ void bu_load_xr(long long N, long long c0, int m, int nrhs, const T* __restrict__ x,
+                                           T (&xr)[kBuMaxRhs][16 / sizeof(T)]) {
+  constexpr int E = 16 / sizeof(T);
+  const int col = (threadIdx.x & 31) * E;
+#pragma unroll
+  for (int v = 0; v < kBuMaxRhs; ++v)
+#pragma unroll
+    for (int e = 0; e < E; ++e) xr[v][e] = (v < nrhs && col + e < m) ? x[(long long)v * N + c0 + col + e] : (T)0;
+}
+
+// x[v][rows] -= W[rows, c0:c0+m] x[v][c0:c0+m] for rows [r0, r1), m <= TB,
+// TB * sizeof(T) = 512 bytes.  32 lanes per row, each with one 16-byte
+// piece of it and of x; the row sum is reduced across the 32 lanes.  A
+// workgroup of 256 threads covers 8 rows per pass and kBuRows rows.  Columns
+// at or beyond m (the partial last block) are masked before loading, since
+// they may lie past the end of the row.
+template <typename T, int TB>
+__global__ __launch_bounds__(256) void k_block_update_rows(const T* __restrict__ W, long long ld, long long N,
+                                                           long long r0, long long r1, long long c0, int m, int nrhs,
+                                                           T* __restrict__ x) {
+  static_assert((16 / sizeof(T)) * 32 == TB, "row piece layout");
+  T xr[kBuMaxRhs][16 / sizeof(T)];
+  bu_load_xr<T, TB>(N, c0, m, nrhs, x, xr);
+  bu_rows<T, TB>(W, ld, N, r0 + (long long)blockIdx.x * kBuRows, r1, c0, m, nrhs, x, xr);
+}
+
+// One block step of a triangular solve, one launch: every workgroup forms the
+// block's solution x_b = inv_b y_b itself (y_b = the block's rows of the
+// working vector, final once all earlier blocks' updates are in), then updates
+// its share of rows [r0, r1) with it; workgroup 0 also writes x_b to the output
+// vector.  The working vector's block rows are never written here, so no
+// workgroup can read a block value another has already replaced.
+template <typename T, int TB>
+__global__ __launch_bounds__(256) void k_block_step(const T* __restrict__ W, long long ld, long long N, long long r0,
+                                                    long long r1, long long c0, int m, const T* __restrict__ invb,
+                                                    int nrhs, T* __restrict__ work, T* __restrict__ out) {
+  static_assert((16 / sizeof(T)) * 32 == TB, "row piece layout");
+  constexpr int E = 16 / sizeof(T);
+  __shared__ T sy[kBuMaxRhs][TB];
+  __shared__ T sxb[kBuMaxRhs][TB];
+  for (int i = threadIdx.x; i < TB * nrhs; i += 256) {
+    const int v = i / TB, r = i % TB;
+    sy[v][r] = (r < m) ? work[(long long)v * N + c0 + r] : (T)0;
+  }
+  __syncthreads();
+  // x_b: rows of the inverse read as 16-byte pieces, one (vector, row) per thread
+  for (int i = threadIdx.x; i < m * nrhs; i += 256) {
+    const int v = i / m, r = i % m;
+    const T* row = invb + (size_t)r * TB;
+    T acc = 0;
+#pragma unroll
+    for (int k0 = 0; k0 < TB; k0 += E) {
+      T q[E];
+      if constexpr (sizeof(T) == 4) {
+        const float4 t = *reinterpret_cast<const float4*>(row + k0);
+        q[0] = t.x;
+        q[1] = t.y;
+        q[2] = t.z;
+        q[3] = t.w;
+      } else {
+        const double2 t = *reinterpret_cast<const double2*>(row + k0);
+        q[0] = t.x;
+        q[1] = t.y;
+      }
+#pragma unroll
+      for (int e = 0; e < E; ++e) acc += q[e] * sy[v][k0 + e];
+    }
+    sxb[v][r] = acc;
+    if (blockIdx.x == 0) out[(long long)v * N + c0 + r] = acc;
+  }
+  __syncthreads();
+  T xr[kBuMaxRhs][E];
+  const int col = (threadIdx.x & 31) * E;
+#pragma unroll
+  for (int v = 0; v < kBuMaxRhs; ++v)
+#pragma unroll
+    for (int e = 0; e < E; ++e) xr[v][e] = (v < nrhs && col + e < m) ? sxb[v][col + e] : (T)0;
+  if (r1 > r0) bu_rows<T, TB>(W, ld, N, r0 + (long long)blockIdx.x * kBuRows, r1, c0, m, nrhs, work, xr);
+}
+
+// A whole triangular solve with one right-hand side in one launch ("decoupled
+// look-back"): workgroup i owns block b (b = i forward, nblk-1-i backward),
+// accumulates L[b, j] x_j over the blocks it depends on as each x_j is
+// published (a per-block flag set to this call's epoch), then forms
+// x_b = inv_b (y_b - sum) and publishes it.  A workgroup waits only on
+// blocks owned by workgroups with smaller indices, which are dispatched
+// before it.  The wait is bounded: after kLbMaxSpins polls (about 0.1 s) it
+// gives up and sets *err to the epoch, and once *err holds the epoch no
+// workgroup of the call waits any more (a wrong result, never a hang).  The W block of the next dependency is
+// loaded before its flag is polled.
+constexpr int kLbMaxSpins = 1 << 17;
+template <typename T, int TB>
+__global__ __launch_bounds__(256) void k_trsv_lookback(const T* __restrict__ W, long long ld, long long N, int lower,
+                                                       const T* __restrict__ inv, const T* __restrict__ y,
+                                                       T* out, int* flags, int epoch, int* err) {
+  constexpr int E = 16 / sizeof(T);  // elements of a lane's row piece
+  constexpr int kP = TB / 8;          // rows per thread (8 rows per pass)
+  static_assert(E * 32 == TB, "row piece layout");
+  __shared__ T sres[TB];
+  const long long nblk = (N + TB - 1) / TB;
+  const long long b = lower ? (long long)blockIdx.x : nblk - 1 - (long long)blockIdx.x;
+  const long long rb = b * TB;
+  const int m = (int)min((long long)TB, N - rb);
+  const int sub = threadIdx.x & 31, slot = threadIdx.x >> 5;
+  const int col = sub * E;
+  T acc[kP];
+#pragma unroll
+  for (int p = 0; p < kP; ++p) acc[p] = (T)0;
+  const long long ndep = lower ? b : nblk - 1 - b;
+  for (long long d = 0; d < ndep; ++d) {
+    const long long j = lower ? d : nblk - 1 - d;
+    const long long cj = j * TB;
+    const int mj = (int)min((long long)TB, N - cj);
+    T w[kP][E];
+#pragma unroll
+    for (int p = 0; p < kP; ++p) {
+      const int r = slot + 8 * p;
+      const T* src = W + (rb + r) * ld + cj + col;
+#pragma unroll
+      for (int e = 0; e < E; ++e) w[p][e] = (T)0;
+      if (r < m) {
+        if (col + E <= mj) {
+          if constexpr (sizeof(T) == 4) {
+            const float4 q = *reinterpret_cast<const float4*>(src);
+            w[p][0] = q.x;
+            w[p][1] = q.y;
+            w[p][2] = q.z;
+            w[p][3] = q.w;
+          } else {
+            const double2 q = *reinterpret_cast<const double2*>(src);
+            w[p][0] = q.x;
+            w[p][1] = q.y;
+          }
+        } else {
+#pragma unroll
+          for (int e = 0; e < E; ++e)
+            if (col + e < mj) w[p][e] = src[e];
+        }
+      }
+    }
+    if (threadIdx.x == 0) {
+      int spins = 0;
+      while (__hip_atomic_load(&flags[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+        if (++spins > kLbMaxSpins) {
+          atomicExch(err, epoch);
+          break;
+        }
+        if ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch) break;
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+    __syncthreads();
+    T xj[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+      xj[e] = (col + e < mj) ? __hip_atomic_load(&out[cj + col + e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                             : (T)0;
+#pragma unroll
+    for (int p = 0; p < kP; ++p)
+#pragma unroll
+      for (int e = 0; e < E; ++e) acc[p] += w[p][e] * xj[e];
+  }
+  // row sums across the 32 lanes, then the block's right-hand side
+#pragma unroll
+  for (int p = 0; p < kP; ++p) {
+    T sum = acc[p];
+#pragma unroll
+    for (int off = 16; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 32);
+    const int r = slot + 8 * p;
+    if (sub == 0) sres[r] = (r < m) ? y[rb + r] - sum : (T)0;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < m) {
+    const T* row = inv + (size_t)b * TB * TB + (size_t)threadIdx.x * TB;
+    T xb = 0;
+#pragma unroll
+    for (int k0 = 0; k0 < TB; k0 += E) {
+      if constexpr (sizeof(T) == 4) {
+        const float4 q = *reinterpret_cast<const float4*>(row + k0);
+        xb += q.x * sres[k0] + q.y * sres[k0 + 1] + q.z * sres[k0 + 2] + q.w * sres[k0 + 3];
+      } else {
+        const double2 q = *reinterpret_cast<const double2*>(row + k0);
+        xb += q.x * sres[k0] + q.y * sres[k0 + 1];
+      }
+    }
+    __hip_atomic_store(&out[rb + threadIdx.x], xb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  // The block's values and the flag are agent-scope atomics, which bypass the
+  // caches that are not coherent across XCDs, and the barrier waits until the
+  // value stores have completed: a reader that sees the flag sees the values.
+  // (Acquire/release fences here would write back or invalidate whole L2s at
+  // every block.)
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(&flags[b], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <typename T>
@@ -1548,6 +1744,15 @@ static int diag_inverses(qd_context* h, const T* W, long long ld, long long N, T
 }
 
 // nrhs <= kBuMaxRhs (lu_solve solves in groups of 8).
+// QUADESTH_LOOKBACK=0 keeps the block-step solves (one launch per block).
+static bool use_lookback() {
+  static const bool on = [] {
+    const char* e = std::getenv("QUADESTH_LOOKBACK");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
 template <typename T>
 static int tri_solve_inv(qd_context* h, const T* W, long long ld, long long N, int nrhs, T* x, const T* inv) {
   constexpr int TB = TriTB<T>::v;
@@ -1555,23 +1760,39 @@ static int tri_solve_inv(qd_context* h, const T* W, long long ld, long long N, i
   const long long nblk = (N + TB - 1) / TB;
   const T* invL = inv;
   const T* invU = inv + (size_t)nblk * TB * TB;
+  auto bm = [&](long long b) { return (int)std::min<long long>(TB, N - b * TB); };
+  // the solution of the forward sweep goes to tmp and is the backward
+  // sweep's working vector; the backward sweep writes its solution to x
+  QD_ALLOC(h, h->lu->trs_tmp, (size_t)N * nrhs * sizeof(T));
+  T* tmp = reinterpret_cast<T*>(h->lu->trs_tmp.p);
+  if (nrhs == 1 && use_lookback()) {
+    qd_lu_state* s = h->lu;
+    if ((long long)s->lb_flags.count < nblk + 1) {
+      QD_ALLOC(h, s->lb_flags, (size_t)nblk + 1);
+      QD_HIP(h, hipMemsetAsync(s->lb_flags.p, 0, s->lb_flags.count * sizeof(int), h->stream));
+    }
+    int* flags = s->lb_flags.p;
+    int* err = flags + nblk;  // the epoch of a call in which a wait gave up
+    int ep = ++s->lb_epoch;
+    hipLaunchKernelGGL((k_trsv_lookback<T, TB>), dim3((unsigned)nblk), dim3(256), 0, h->stream, W, ld, N, 1, invL,
+                       (const T*)x, tmp, flags, ep, err);
+    ep = ++s->lb_epoch;
+    hipLaunchKernelGGL((k_trsv_lookback<T, TB>), dim3((unsigned)nblk), dim3(256), 0, h->stream, W, ld, N, 0, invU,
+                       (const T*)tmp, x, flags, ep, err);
+    QD_HIP(h, hipGetLastError());
+    return QD_OK;
+  }
   for (long long b = 0; b < nblk; ++b) {
-    const long long i0 = b * TB;
-    const int m = (int)std::min<long long>(TB, N - i0);
-    hipLaunchKernelGGL((k_apply_inv<T, TB>), dim3(1), dim3(256), 0, h->stream, invL + (size_t)b * TB * TB, N, i0, m,
-                       nrhs, x);
-    if (i0 + m < N)
-      hipLaunchKernelGGL((k_block_update_rows<T, TB>), dim3((unsigned)((N - i0 - m + kBuRows - 1) / kBuRows)),
-                         dim3(256), 0, h->stream, W, ld, N, i0 + m, N, i0, m, nrhs, x);
+    const long long i0 = b * TB, r0 = i0 + bm(b);
+    const unsigned g = std::max(1u, (unsigned)((N - r0 + kBuRows - 1) / kBuRows));
+    hipLaunchKernelGGL((k_block_step<T, TB>), dim3(g), dim3(256), 0, h->stream, W, ld, N, r0, N, i0, bm(b),
+                       invL + (size_t)b * TB * TB, nrhs, x, tmp);
   }
   for (long long b = nblk - 1; b >= 0; --b) {
     const long long i0 = b * TB;
-    const int m = (int)std::min<long long>(TB, N - i0);
-    hipLaunchKernelGGL((k_apply_inv<T, TB>), dim3(1), dim3(256), 0, h->stream, invU + (size_t)b * TB * TB, N, i0, m,
-                       nrhs, x);
-    if (i0 > 0)
-      hipLaunchKernelGGL((k_block_update_rows<T, TB>), dim3((unsigned)((i0 + kBuRows - 1) / kBuRows)), dim3(256), 0,
-                         h->stream, W, ld, N, 0LL, i0, i0, m, nrhs, x);
+    const unsigned g = std::max(1u, (unsigned)((i0 + kBuRows - 1) / kBuRows));
+    hipLaunchKernelGGL((k_block_step<T, TB>), dim3(g), dim3(256), 0, h->stream, W, ld, N, 0LL, i0, i0, bm(b),
+                       invU + (size_t)b * TB * TB, nrhs, tmp, x);
   }
   QD_HIP(h, hipGetLastError());
   return QD_OK;
