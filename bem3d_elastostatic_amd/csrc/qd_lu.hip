@@ -71,7 +71,7 @@ struct qd_lu_state {
   qd::DevBuf<float> ts_lu;       // LU of a leaf's winners block (32 x 32)
   qd::DevBuf<float> ts_top;      // a leaf's original top rows (32 x 32)
   qd::DevBuf<unsigned char> trs_tmp;  // intermediate vector of the block triangular solves
-  qd::DevBuf<int> lb_flags;      // per-block publication flags of the look-back solves (+ error word)
+  qd::DevBuf<int> lb_flags;      // error word, then per-block publication flags of the look-back solves
   int lb_epoch = 0;              // value the flags take in the latest look-back solve
   qd::DevBuf<float> ts_linv;     // inverses of the leaves' unit lower blocks (32 x 32 each)
   qd::DevBuf<float> inv32;       // inverted diagonal blocks of L and U
@@ -1193,8 +1193,9 @@ static int tslu_leaf(LuCtx<float>& c, long long c0, int w, long long p0, long lo
   ts_tree_sizes(m, ncand, ncnt);
   if ((long long)s->ts_cand.count < ncand || (long long)s->ts_cnt.count < ncnt)
     return set_error(c.h, QD_ERR_STATE, "tournament buffers sized for fewer rows than %lld", m);
-  const unsigned nb = (unsigned)((m + kTsRows - 1) / kTsRows);
-  hipLaunchKernelGGL(k_tslu_tree, dim3(nb), dim3(64), 0, c.st, c.W, c.ld, c.N, c0, w, s->ts_cand.p, s->ts_cnt.p,
+  using Nd = TsNode<kTsTreeR, kTsTreeV>;
+  const unsigned nb = (unsigned)((m + Nd::rows - 1) / Nd::rows);
+  hipLaunchKernelGGL((k_tslu_tree<kTsTreeR, kTsTreeV>), dim3(nb), dim3(Nd::threads), 0, c.st, c.W, c.ld, c.N, c0, w, s->ts_cand.p, s->ts_cnt.p,
                      s->ts_lu.p, s->ts_top.p, s->ts_meta.p, s->ipiv_dev.p);
   const long long below = std::max<long long>(m - w, 1);
   const int row_blocks = (int)((below + 255) / 256);
@@ -1251,11 +1252,16 @@ static int update_cols(LuCtx<T>& c, long long j, int jb, long long c0, long long
 // Workgroup cap of the trailing update that runs beside a panel
 // (QUADESTH_GEMM_WGS; 0 = one workgroup per tile).  Measured on MI355X at
 // 76,806 DOF, caps of 320-512 did not shorten the factorisation: the panel
-// beside the update is slowed by the memory system, not by CU slots.
+// beside the update is slowed by the memory system, not by CU slots.  Nor did
+// CU-masked streams that kept the update off 16-64 CUs: the panel gained
+// 9-21%, the update lost about as much time with the CUs.
 static int lookahead_gemm_wgs() {
   if (const char* e = std::getenv("QUADESTH_GEMM_WGS")) return std::max(0, std::atoi(e));
   return 0;
 }
+
+template <typename T>
+static int factor_steps(LuCtx<T>& c, int NB, bool look);
 
 // Blocked right-looking LU with a look-ahead of one panel: each step solves
 // for and updates the next panel's columns first; that panel is then factored
@@ -1309,6 +1315,13 @@ static int factor(LuCtx<T>& c, int NB) {
     QD_HIP(c.h, hipEventCreateWithFlags(&s->ev_cols, hipEventDisableTiming));
     QD_HIP(c.h, hipEventCreateWithFlags(&s->ev_panel, hipEventDisableTiming));
   }
+  return factor_steps(c, NB, look);
+}
+
+template <typename T>
+static int factor_steps(LuCtx<T>& c, int NB, bool look) {
+  qd_lu_state* s = c.s;
+  const long long N = c.N;
   c.sc = &s->sc_main;
   LuCtx<T> cp = c;
   cp.st = s->aux;
@@ -1452,25 +1465,6 @@ __global__ __launch_bounds__(TB) void k_diag_inverse(const T* __restrict__ W, lo
   for (int i = 0; i < TB; ++i) out[i * TB + j] = sX[i][j];
 }
 
-// x_I <- inv_I x_I for block I (rows i0 .. i0+m), nrhs vectors of stride N.
-template <typename T, int TB>
-__global__ __launch_bounds__(256) void k_apply_inv(const T* __restrict__ invb, long long N, long long i0, int m,
-                                                   int nrhs, T* __restrict__ x) {
-  __shared__ T sx[8][TB];
-  for (int i = threadIdx.x; i < TB * nrhs; i += 256) {
-    const int v = i / TB, r = i % TB;
-    sx[v][r] = (r < m) ? x[(long long)v * N + i0 + r] : (T)0;
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < m * nrhs; i += 256) {
-    const int v = i / m, r = i % m;
-    const T* row = invb + (size_t)r * TB;
-    T s = 0;
-    for (int k = 0; k < TB; ++k) s += row[k] * sx[v][k];
-    x[(long long)v * N + i0 + r] = s;
-  }
-}
-
 // x[v][rows] -= W[rows, c0:c0+m] x[v][c0:c0+m] for rows [r0, r1), m <= TB,
 // TB * sizeof(T) = 512 bytes.  32 lanes per row, each with one 16-byte
 // piece of it and of x; the row sum is reduced across the 32 lanes.  A
@@ -1533,33 +1527,6 @@ __device__ __forceinline__ void bu_rows(const T* __restrict__ W, long long ld, l
       }
     }
   }
-}
-
-template <typename T, int TB>
-__device__ __forceinline__ void bu_load_xr(long long N, long long c0, int m, int nrhs, const T* __restrict__ x,
-                                           T (&xr)[kBuMaxRhs][16 / sizeof(T)]) {
-  constexpr int E = 16 / sizeof(T);
-  const int col = (threadIdx.x & 31) * E;
-#pragma unroll
-  for (int v = 0; v < kBuMaxRhs; ++v)
-#pragma unroll
-    for (int e = 0; e < E; ++e) xr[v][e] = (v < nrhs && col + e < m) ? x[(long long)v * N + c0 + col + e] : (T)0;
-}
-
-// x[v][rows] -= W[rows, c0:c0+m] x[v][c0:c0+m] for rows [r0, r1), m <= TB,
-// TB * sizeof(T) = 512 bytes.  32 lanes per row, each with one 16-byte
-// piece of it and of x; the row sum is reduced across the 32 lanes.  A
-// workgroup of 256 threads covers 8 rows per pass and kBuRows rows.  Columns
-// at or beyond m (the partial last block) are masked before loading, since
-// they may lie past the end of the row.
-template <typename T, int TB>
-__global__ __launch_bounds__(256) void k_block_update_rows(const T* __restrict__ W, long long ld, long long N,
-                                                           long long r0, long long r1, long long c0, int m, int nrhs,
-                                                           T* __restrict__ x) {
-  static_assert((16 / sizeof(T)) * 32 == TB, "row piece layout");
-  T xr[kBuMaxRhs][16 / sizeof(T)];
-  bu_load_xr<T, TB>(N, c0, m, nrhs, x, xr);
-  bu_rows<T, TB>(W, ld, N, r0 + (long long)blockIdx.x * kBuRows, r1, c0, m, nrhs, x, xr);
 }
 
 // One block step of a triangular solve, one launch: every workgroup forms the
@@ -1771,8 +1738,8 @@ static int tri_solve_inv(qd_context* h, const T* W, long long ld, long long N, i
       QD_ALLOC(h, s->lb_flags, (size_t)nblk + 1);
       QD_HIP(h, hipMemsetAsync(s->lb_flags.p, 0, s->lb_flags.count * sizeof(int), h->stream));
     }
-    int* flags = s->lb_flags.p;
-    int* err = flags + nblk;  // the epoch of a call in which a wait gave up
+    int* err = s->lb_flags.p;  // the epoch of a call in which a wait gave up
+    int* flags = s->lb_flags.p + 1;
     int ep = ++s->lb_epoch;
     hipLaunchKernelGGL((k_trsv_lookback<T, TB>), dim3((unsigned)nblk), dim3(256), 0, h->stream, W, ld, N, 1, invL,
                        (const T*)x, tmp, flags, ep, err);
@@ -2153,6 +2120,7 @@ int lu_solve(qd_context* h, int precision, int nrhs, double* b, long long ldb, d
   if (s->info > 0) return s->info;
   const long long N = s->N;
   const double t0 = now_ms();
+  const int lb_epoch0 = s->lb_epoch;
   QD_ALLOC(h, s->vec, (size_t)5 * 8 * N);
   QD_ALLOC(h, s->vecf, (size_t)8 * N);
   double* X = s->vec.p;                 // solutions
@@ -2246,6 +2214,14 @@ int lu_solve(qd_context* h, int precision, int nrhs, double* b, long long ldb, d
                              hipMemcpyDeviceToDevice, h->stream));
   QD_HIP(h, hipStreamSynchronize(h->stream));
   R->t_solve_ms = now_ms() - t0;
+  if (s->lb_epoch != lb_epoch0) {
+    // a look-back solve whose wait gave up leaves its epoch in the error word
+    int lb_err = 0;
+    QD_HIP(h, hipMemcpy(&lb_err, s->lb_flags.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (lb_err > lb_epoch0)
+      return set_error(h, QD_ERR_HIP, "look-back triangular solve: a wait for an earlier block timed out (call %d)",
+                       lb_err);
+  }
   if (precision != QD_FP64 && !R->converged)
     return set_error(h, QD_ERR_NOCONV, "refinement stopped at relative residual %.3e (tol %.1e) after %d sweeps",
                      R->residual, tol > 0 ? tol : 1e-14, R->iterations);

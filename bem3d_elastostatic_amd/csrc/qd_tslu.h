@@ -12,12 +12,12 @@ namespace qd {
 // only preconditions the FP64 refinement), in two launches per leaf.
 //
 // k_tslu_tree runs the whole selection tree.  A node is Gaussian elimination
-// with partial pivoting on up to 256 rows, four per lane of one wave, and
-// yields its 32 winning row numbers in pivot order.  A step needs no LDS and
-// no barrier: the pivot search is a DPP wave maximum, and the pivot row
-// reaches every lane through readlane (into SGPRs).  Level 0 has one node per
-// 256 rows of the leaf; each level above has one node per 8 nodes below, over
-// their 8 x 32 winners.  No workgroup waits for another: the last workgroup of
+// with partial pivoting on up to 512 rows, two per lane of four waves, and
+// yields its 32 winning row numbers in pivot order.  A step's pivot search is
+// a DPP maximum in each wave and one barrier to compare the waves' best rows,
+// whose values travel through LDS.  Level 0 has one node per 512 rows of the
+// leaf; each level above has one node per 16 nodes below, over their 16 x 32
+// winners.  No workgroup waits for another: the last workgroup of
 // a group to finish (an agent-scope counter) goes on to run the parent node,
 // and the workgroup that runs the root writes the winners' LU, the LAPACK
 // interchange sequence (ipiv) and the permutation it amounts to.  The rows are
@@ -32,23 +32,42 @@ namespace qd {
 // Both raise their waves' issue priority (s_setprio): they run beside the
 // trailing update on the same SIMDs and sit on the factorisation's critical
 // path.
-constexpr int kTsW = 32, kTsR = 4;
-constexpr int kTsRows = 64 * kTsR;     // rows of a node
-constexpr int kTsFan = kTsRows / kTsW;  // nodes per parent
+constexpr int kTsW = 32;
+// A node of the tree: V waves holding R rows per lane, 64 V R rows; a parent
+// takes the winners of fan = rows / 32 children.
+template <int R, int V>
+struct TsNode {
+  static constexpr int threads = 64 * V;
+  static constexpr int rows = threads * R;
+  static constexpr int fan = rows / kTsW;
+};
+// Measured on MI355X (one leaf of 76,800 rows: 85 us at 2 x 4 against 98 us
+// at 4 x 1): fewer levels outweigh the barrier each step costs with more
+// waves, and 120 VGPRs fit beside the trailing update more easily than 228.
+#ifndef QD_TS_R
+#define QD_TS_R 2
+#endif
+#ifndef QD_TS_V
+#define QD_TS_V 4
+#endif
+constexpr int kTsTreeR = QD_TS_R, kTsTreeV = QD_TS_V;
+static_assert(kTsTreeR >= 1 && kTsTreeR <= 4 && kTsTreeV >= 1 && kTsTreeV <= 16, "tree node shape");
 // meta layout: [0, 32) ipiv values, [32, 64) displaced positions, [64, 96)
 // the original top row each of them receives, [96] their count, [128, 160)
 // the original row that ends at top position c0 + k.
 constexpr int kTsMeta = 160;
 
 // Candidate-list entries and counters the tree needs for a leaf of m rows.
+template <int R = kTsTreeR, int V = kTsTreeV>
 static inline void ts_tree_sizes(long long m, long long& cand, long long& counters) {
-  long long nl = (m + kTsRows - 1) / kTsRows;
+  using Nd = TsNode<R, V>;
+  long long nl = (m + Nd::rows - 1) / Nd::rows;
   cand = 0;
   counters = 0;
   while (true) {
     cand += nl * kTsW;
     if (nl == 1) break;
-    const long long up = (nl + kTsFan - 1) / kTsFan;
+    const long long up = (nl + Nd::fan - 1) / Nd::fan;
     counters += up;
     nl = up;
   }
@@ -80,42 +99,89 @@ __device__ __forceinline__ float readlane_f(float v, int lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
-// One elimination step (column K) of a node.  A row's key is bits(|a|) + 1
-// (0 = no row), so the largest |a| wins; ties go to the lowest lane, then the
-// lane's first row.  The winning lane stores its row in LDS (under a uniform
-// branch on which of its rows won) and every lane reads it back by broadcast.
-// The step has no other branch, so the row registers need no copies between
-// steps: a node out of rows elects no row (its winner index reads -1) and
-// changes nothing, and columns K >= w are zero, so their steps change nothing
-// either (their winners are not used).  A scheduling barrier closes the step.
-template <int K>
-__device__ __forceinline__ void tn_step(float (&a)[kTsR][kTsW], int (&gi)[kTsR], int* __restrict__ swin,
-                                        float* __restrict__ srow, int* __restrict__ sidx,
-                                        float* __restrict__ lu_out) {
-  const int lane = threadIdx.x;
+// LDS of a node's elimination: per step parity, each wave's candidate (its
+// key, global row and the row itself).
+template <int V>
+struct TsNodeSmem {
+  float row[2][V][kTsW];
+  unsigned key[2][V];
+  int idx[2][V];
+};
+
+// A wave's pivot candidate for column K: a row's key is bits(|a|) + 1 (0 = no
+// row), so the largest |a| wins; ties go to the lowest lane, then the lane's
+// first row.  wmax is the wave's best key, held by row qw of lane wl.
+struct TsPick {
+  unsigned wmax;
+  int wl, qw;
+};
+
+template <int K, int R>
+__device__ __forceinline__ TsPick tn_pick(const float (&a)[R][kTsW], const int (&gi)[R]) {
   unsigned key = 0;
   int qb = 0;
 #pragma unroll
-  for (int q = 0; q < kTsR; ++q) {
+  for (int q = 0; q < R; ++q) {
     const unsigned kq = (gi[q] >= 0) ? __float_as_uint(fabsf(a[q][K])) + 1u : 0u;
     qb = (kq > key) ? q : qb;
     key = (kq > key) ? kq : key;
   }
-  const unsigned wmax = wave_max_u32(key);
-  const int wl = __ffsll((long long)__ballot(key == wmax)) - 1;
-  const int qw = __builtin_amdgcn_readlane(qb, wl);
+  TsPick p;
+  p.wmax = wave_max_u32(key);
+  p.wl = __ffsll((long long)__ballot(key == p.wmax)) - 1;
+  p.qw = __builtin_amdgcn_readlane(qb, p.wl);
+  return p;
+}
+
+// One elimination step (column K) of a node of 64 V R rows, R per lane of V
+// waves.  Each wave's candidate for the step (found at the end of the
+// previous step's update of column K) is stored by its lane in the LDS slot
+// of its wave for this step's parity: key, row number and row (under a
+// uniform branch on which of the lane's rows won).  After one barrier every
+// lane picks the best of the V slots (ties to the lowest wave) and reads that
+// row back by broadcast.  Slots alternate between two parities, so a wave
+// writing step K+1's candidate never meets a wave still reading step K's: to
+// pass step K+1's barrier every wave must have finished step K.  The update
+// does column K+1 first and finds the next candidate from it, so the wave
+// maximum's latency overlaps the remaining columns' FMAs.  The step has no
+// other branch, so the row registers need no copies between steps: a node
+// out of rows elects no row (its winner index reads -1) and changes nothing,
+// and columns K >= w are zero, so their steps change nothing either (their
+// winners are not used).
+template <int K, int R, int V>
+__device__ __forceinline__ void tn_step(float (&a)[R][kTsW], int (&gi)[R], int* __restrict__ swin,
+                                        TsNodeSmem<V>& sm, float* __restrict__ lu_out, TsPick& pk) {
+  constexpr int P = K & 1;
+  const int lane = threadIdx.x & 63;
+  const int wv = (V > 1) ? __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) : 0;
 #pragma unroll
-  for (int q = 0; q < kTsR; ++q)
-    if (q == qw && lane == wl) {
+  for (int q = 0; q < R; ++q)
+    if (q == pk.qw && lane == pk.wl) {
 #pragma unroll
-      for (int j = 0; j < kTsW; ++j) srow[j] = a[q][j];
-      *sidx = gi[q];
+      for (int j = 0; j < kTsW; ++j) sm.row[P][wv][j] = a[q][j];
+      sm.idx[P][wv] = gi[q];
+      sm.key[P][wv] = pk.wmax;
     }
+  int ws = 0;  // the winning wave
+  if constexpr (V == 1) {
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the row is in LDS
+    __builtin_amdgcn_wave_barrier();
+  } else {
+    __syncthreads();
+    unsigned best = sm.key[P][0];
 #pragma unroll
-  for (int q = 0; q < kTsR; ++q) gi[q] = (q == qw && lane == wl) ? -1 : gi[q];
-  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the row is in LDS
-  __builtin_amdgcn_wave_barrier();
-  const float4* s4 = reinterpret_cast<const float4*>(srow);
+    for (int v = 1; v < V; ++v) {
+      const unsigned kv = sm.key[P][v];
+      ws = (kv > best) ? v : ws;
+      best = (kv > best) ? kv : best;
+    }
+    ws = __builtin_amdgcn_readfirstlane(ws);
+  }
+  if (wv == ws) {
+#pragma unroll
+    for (int q = 0; q < R; ++q) gi[q] = (q == pk.qw && lane == pk.wl) ? -1 : gi[q];
+  }
+  const float4* s4 = reinterpret_cast<const float4*>(sm.row[P][ws]);
   float r[kTsW];
 #pragma unroll
   for (int j4 = K / 4; j4 < kTsW / 4; ++j4) {
@@ -125,31 +191,41 @@ __device__ __forceinline__ void tn_step(float (&a)[kTsR][kTsW], int (&gi)[kTsR],
     r[4 * j4 + 2] = v.z;
     r[4 * j4 + 3] = v.w;
   }
-  const int ib = *sidx;
-  if (lane == 0) swin[K] = ib;
+  const int ib = sm.idx[P][ws];
+  if (threadIdx.x == 0) swin[K] = ib;
   // the winner of step K, eliminated by the K earlier winners with its
   // multipliers kept in place, is row K of the winners' LU
-  if (lu_out && lane < kTsW) lu_out[K * kTsW + lane] = srow[lane];
+  if (lu_out && threadIdx.x < kTsW) lu_out[K * kTsW + threadIdx.x] = sm.row[P][ws][threadIdx.x];
   const float piv = r[K];
   const float rp = (piv != 0.0f) ? __builtin_amdgcn_rcpf(piv) : 0.0f;
+  float l[R];
 #pragma unroll
-  for (int q = 0; q < kTsR; ++q) {
-    const float l = (gi[q] >= 0) ? a[q][K] * rp : 0.0f;
-    a[q][K] = l;
-#pragma unroll
-    for (int j = K + 1; j < kTsW; ++j) a[q][j] -= l * r[j];
+  for (int q = 0; q < R; ++q) {
+    l[q] = (gi[q] >= 0) ? a[q][K] * rp : 0.0f;
+    a[q][K] = l[q];
   }
-  // every lane has read the row before the next step overwrites it
-  __builtin_amdgcn_s_waitcnt(0xc07f);
-  __builtin_amdgcn_wave_barrier();
+  if constexpr (K + 1 < kTsW) {
+#pragma unroll
+    for (int q = 0; q < R; ++q) a[q][K + 1] -= l[q] * r[K + 1];
+    pk = tn_pick<K + 1, R>(a, gi);
+  }
+#pragma unroll
+  for (int q = 0; q < R; ++q)
+#pragma unroll
+    for (int j = K + 2; j < kTsW; ++j) a[q][j] -= l[q] * r[j];
+  if constexpr (V == 1) {
+    // one wave: the row has been read before the next step's writes
+    __builtin_amdgcn_s_waitcnt(0xc07f);
+    __builtin_amdgcn_wave_barrier();
+  }
   __builtin_amdgcn_sched_barrier(0);
 }
 
-template <int... Ks>
-__device__ __forceinline__ void tn_all(std::integer_sequence<int, Ks...>, float (&a)[kTsR][kTsW], int (&gi)[kTsR],
-                                       int* __restrict__ swin, float* __restrict__ srow, int* __restrict__ sidx,
-                                       float* __restrict__ lu_out) {
-  (tn_step<Ks>(a, gi, swin, srow, sidx, lu_out), ...);
+template <int R, int V, int... Ks>
+__device__ __forceinline__ void tn_all(std::integer_sequence<int, Ks...>, float (&a)[R][kTsW], int (&gi)[R],
+                                       int* __restrict__ swin, TsNodeSmem<V>& sm, float* __restrict__ lu_out) {
+  TsPick pk = tn_pick<0, R>(a, gi);
+  (tn_step<Ks, R, V>(a, gi, swin, sm, lu_out, pk), ...);
 }
 
 // Loads the 32-column segment [c0, c0+32) of row g (zero beyond column w).
@@ -172,44 +248,47 @@ __device__ __forceinline__ void ts_load_row(const float* __restrict__ W, long lo
 // group counters, zero between leaves (the last arriver resets its own).
 // Outputs of the root: lu_out (w x w winners' LU, stride 32), top_out (the
 // original top rows' leaf columns, 32 x 32), meta, ipiv[c0 .. c0+w).
-// One wave per workgroup.
-__global__ __launch_bounds__(64) void k_tslu_tree(const float* __restrict__ W, long long ld, long long N,
-                                                  long long c0, int w, int* cand, unsigned* cnt,
-                                                  float* __restrict__ lu_out, float* __restrict__ top_out,
-                                                  int* __restrict__ meta, int* __restrict__ ipiv) {
+// 64 V threads per workgroup; a node's row slot t (0 .. 64 V R) is held by
+// thread t % (64 V) as its row t / (64 V).
+template <int R, int V>
+__global__ __launch_bounds__(64 * V) void k_tslu_tree(const float* __restrict__ W, long long ld, long long N,
+                                                      long long c0, int w, int* cand, unsigned* cnt,
+                                                      float* __restrict__ lu_out, float* __restrict__ top_out,
+                                                      int* __restrict__ meta, int* __restrict__ ipiv) {
+  using Nd = TsNode<R, V>;
+  constexpr int T = Nd::threads;
   __shared__ int swin[kTsW];
-  __shared__ __attribute__((aligned(16))) float srow[kTsW];
-  __shared__ int sidx;
+  __shared__ __attribute__((aligned(16))) TsNodeSmem<V> sm;
   __shared__ int slast;
   __builtin_amdgcn_s_setprio(3);
   const int tid = threadIdx.x;
   const long long m = N - c0;
-  long long nl = (m + kTsRows - 1) / kTsRows;  // nodes of the current level
+  long long nl = (m + Nd::rows - 1) / Nd::rows;  // nodes of the current level
   long long node = blockIdx.x;
   long long off = 0;   // first candidate entry of the current level
   long long coff = 0;  // first counter of the current level's groups
-  int gi[kTsR];
+  int gi[R];
 #pragma unroll
-  for (int q = 0; q < kTsR; ++q) {
-    const long long g = c0 + node * kTsRows + tid + 64 * q;
+  for (int q = 0; q < R; ++q) {
+    const long long g = c0 + node * Nd::rows + tid + T * q;
     gi[q] = (g < N) ? (int)g : -1;
   }
   while (true) {
-    float a[kTsR][kTsW];
+    float a[R][kTsW];
 #pragma unroll
-    for (int q = 0; q < kTsR; ++q) ts_load_row(W, ld, gi[q] >= 0 ? gi[q] : c0, c0, w, a[q]);
+    for (int q = 0; q < R; ++q) ts_load_row(W, ld, gi[q] >= 0 ? gi[q] : c0, c0, w, a[q]);
 #pragma unroll
-    for (int q = 0; q < kTsR; ++q)
+    for (int q = 0; q < R; ++q)
 #pragma unroll
       for (int j = 0; j < kTsW; ++j) a[q][j] = (gi[q] >= 0) ? a[q][j] : 0.0f;
     const bool root = (nl == 1);
-    tn_all(std::make_integer_sequence<int, kTsW>{}, a, gi, swin, srow, &sidx, root ? lu_out : nullptr);
+    tn_all<R, V>(std::make_integer_sequence<int, kTsW>{}, a, gi, swin, sm, root ? lu_out : nullptr);
     __syncthreads();
     if (root) break;
     if (tid < kTsW) cand[off + node * kTsW + tid] = (tid < w) ? swin[tid] : -1;
     // hand the winners on: the group's last node to finish runs the parent
-    const long long grp = node / kTsFan;
-    const unsigned gsz = (unsigned)min((long long)kTsFan, nl - kTsFan * grp);
+    const long long grp = node / Nd::fan;
+    const unsigned gsz = (unsigned)min((long long)Nd::fan, nl - Nd::fan * grp);
     __syncthreads();
     if (tid == 0) {
       __threadfence();
@@ -222,18 +301,18 @@ __global__ __launch_bounds__(64) void k_tslu_tree(const float* __restrict__ W, l
     if (!slast) return;
     __threadfence();
 #pragma unroll
-    for (int q = 0; q < kTsR; ++q) {
-      const int t = tid + 64 * q;
+    for (int q = 0; q < R; ++q) {
+      const int t = tid + T * q;
       int ng = -1;
       if (t < (int)(kTsW * gsz)) {
-        ng = __hip_atomic_load(&cand[off + kTsFan * grp * kTsW + t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ng = __hip_atomic_load(&cand[off + Nd::fan * grp * kTsW + t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (ng < c0 || ng >= N) ng = -1;
       }
       gi[q] = ng;
     }
     off += nl * kTsW;
-    coff += (nl + kTsFan - 1) / kTsFan;
-    nl = (nl + kTsFan - 1) / kTsFan;
+    coff += (nl + Nd::fan - 1) / Nd::fan;
+    nl = (nl + Nd::fan - 1) / Nd::fan;
     node = grp;
   }
   // root: the original top rows' leaf columns, then the interchange sequence
@@ -241,6 +320,7 @@ __global__ __launch_bounds__(64) void k_tslu_tree(const float* __restrict__ W, l
   // (ipiv[c0+k] = current position of p_k), formed over the 64 positions it
   // can touch: lanes 0..31 the top rows, lane 32+k the original position of
   // p_k when that lies below the top rows.
+  if (tid >= 64) return;
 #pragma unroll
   for (int i = tid; i < kTsW * kTsW / 4; i += 64) {
     const int r = i >> 3, j4 = (i & 7) * 4;
