@@ -1591,8 +1591,10 @@ __global__ __launch_bounds__(256) void k_block_step(const T* __restrict__ W, lon
 // blocks owned by workgroups with smaller indices, which are dispatched
 // before it.  The wait is bounded: after kLbMaxSpins polls (about 0.1 s) it
 // gives up and sets *err to the epoch, and once *err holds the epoch no
-// workgroup of the call waits any more (a wrong result, never a hang).  The W block of the next dependency is
-// loaded before its flag is polled.
+// workgroup of the call waits any more (a wrong result, never a hang; the
+// solve's caller reads *err and reports the failure).  The block's inverse is
+// loaded before the first wait and the W block of the next dependency before
+// its flag is polled, so a published x_j is followed only by its own loads.
 constexpr int kLbMaxSpins = 1 << 17;
 template <typename T, int TB>
 __global__ __launch_bounds__(256) void k_trsv_lookback(const T* __restrict__ W, long long ld, long long N, int lower,
@@ -1611,6 +1613,29 @@ __global__ __launch_bounds__(256) void k_trsv_lookback(const T* __restrict__ W, 
   T acc[kP];
 #pragma unroll
   for (int p = 0; p < kP; ++p) acc[p] = (T)0;
+  // the block's inverse, loaded before any wait: thread t holds part t % kParts
+  // of row t / kParts (kParts = 256 / TB parts of kIp elements)
+  constexpr int kParts = 256 / TB, kIp = TB / kParts;
+  static_assert(kIp % E == 0, "inverse parts are whole 16-byte pieces");
+  const int irow = threadIdx.x / kParts, ipart = threadIdx.x % kParts;
+  T pinv[kIp];
+  {
+    const T* src = inv + (size_t)b * TB * TB + (size_t)irow * TB + ipart * kIp;
+#pragma unroll
+    for (int k0 = 0; k0 < kIp; k0 += E) {
+      if constexpr (sizeof(T) == 4) {
+        const float4 q = *reinterpret_cast<const float4*>(src + k0);
+        pinv[k0] = q.x;
+        pinv[k0 + 1] = q.y;
+        pinv[k0 + 2] = q.z;
+        pinv[k0 + 3] = q.w;
+      } else {
+        const double2 q = *reinterpret_cast<const double2*>(src + k0);
+        pinv[k0] = q.x;
+        pinv[k0 + 1] = q.y;
+      }
+    }
+  }
   const long long ndep = lower ? b : nblk - 1 - b;
   for (long long d = 0; d < ndep; ++d) {
     const long long j = lower ? d : nblk - 1 - d;
@@ -1675,20 +1700,13 @@ __global__ __launch_bounds__(256) void k_trsv_lookback(const T* __restrict__ W, 
     if (sub == 0) sres[r] = (r < m) ? y[rb + r] - sum : (T)0;
   }
   __syncthreads();
-  if ((int)threadIdx.x < m) {
-    const T* row = inv + (size_t)b * TB * TB + (size_t)threadIdx.x * TB;
+  {
     T xb = 0;
 #pragma unroll
-    for (int k0 = 0; k0 < TB; k0 += E) {
-      if constexpr (sizeof(T) == 4) {
-        const float4 q = *reinterpret_cast<const float4*>(row + k0);
-        xb += q.x * sres[k0] + q.y * sres[k0 + 1] + q.z * sres[k0 + 2] + q.w * sres[k0 + 3];
-      } else {
-        const double2 q = *reinterpret_cast<const double2*>(row + k0);
-        xb += q.x * sres[k0] + q.y * sres[k0 + 1];
-      }
-    }
-    __hip_atomic_store(&out[rb + threadIdx.x], xb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int k = 0; k < kIp; ++k) xb += pinv[k] * sres[ipart * kIp + k];
+#pragma unroll
+    for (int off = 1; off < kParts; off <<= 1) xb += __shfl_xor(xb, off);
+    if (ipart == 0 && irow < m) __hip_atomic_store(&out[rb + irow], xb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   // The block's values and the flag are agent-scope atomics, which bypass the
   // caches that are not coherent across XCDs, and the barrier waits until the
