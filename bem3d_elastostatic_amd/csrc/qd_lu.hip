@@ -338,9 +338,11 @@ template <int kSW>
 __global__ __launch_bounds__(256) void k_trsm_strip(float* __restrict__ W, long long ld, long long r0, int m,
                                                     long long c0, long long c1, const float* __restrict__ linv) {
   extern __shared__ float xs[];  // m x kSW
-  __shared__ float sinv[32][33], sl[32][33];
-  __shared__ float sb[32][kSW + 1];
-  __shared__ float lbuf[2][kStripChunk][33];
+  // rows of 32 floats, 16-byte aligned: the triangular and chunk sums read
+  // them as float4 (all lanes of a half-wave read the same row)
+  __shared__ __attribute__((aligned(16))) float sinv[32][32], sl[32][32];
+  __shared__ float sb[32][kSW];
+  __shared__ __attribute__((aligned(16))) float lbuf[2][kStripChunk][32];
   constexpr int kSlots = 256 / kSW, kPer = 32 * kSW / 256;
   constexpr int kRowsPer = kStripChunk / kSlots;  // rows of a chunk per thread
   const int tid = threadIdx.x, c = tid % kSW, slot = tid / kSW;
@@ -361,10 +363,8 @@ __global__ __launch_bounds__(256) void k_trsm_strip(float* __restrict__ W, long 
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int e = tid + 256 * u, rr = e >> 3, k4 = e & 7;
-      lbuf[buf][rr][4 * k4] = v[u].x;
-      lbuf[buf][rr][4 * k4 + 1] = v[u].y;
-      lbuf[buf][rr][4 * k4 + 2] = v[u].z;
-      lbuf[buf][rr][4 * k4 + 3] = v[u].w;
+      const float4 t = v[u];
+      *reinterpret_cast<float4*>(&lbuf[buf][rr][4 * k4]) = t;
     }
   };
   // a diagonal block's inverse and L, four elements of each per thread
@@ -392,54 +392,62 @@ __global__ __launch_bounds__(256) void k_trsm_strip(float* __restrict__ W, long 
     }
     if (i + 1 < m / 32) load_diag(i + 1);  // in flight until the next block
     __syncthreads();
-    // x = inv b, then x += inv (b - L x).  The 32-term sums are unrolled for
-    // the narrow strips (one element per thread) and left as loops for the
-    // wide ones (four per thread), whose unrolled form needs twice the
-    // registers and then no longer fits beside the trailing update's waves.
-    auto lower_dot = [&](const float (*mat)[33], int k, const float* vec, int stride) {
-      float acc = 0.0f;
-      if constexpr (kSW <= 8) {
+    // x = inv b, then x += inv (b - L x).  A thread's kPer elements share
+    // their column (cc = tid % kSW) and sit in rows k_q = tid / kSW + q 256 /
+    // kSW, so one pass over j serves all of them: the vector elements are
+    // read once per j, the matrix rows four entries at a time, and the kPer
+    // sums are independent chains.
+    auto lower_dot = [&](const float (*mat)[32], const float* vec, int stride, float (&acc)[kPer]) {
+      const int k0 = tid / kSW;
+      const int kmax = k0 + (256 / kSW) * (kPer - 1);
 #pragma unroll
-        for (int j = 0; j < 32; ++j) acc += (j <= k) ? mat[k][j] * vec[j * stride] : 0.0f;
-      } else {
-        for (int j = 0; j <= k; ++j) acc += mat[k][j] * vec[j * stride];
+      for (int q = 0; q < kPer; ++q) acc[q] = 0.0f;
+#pragma unroll 2
+      for (int j4 = 0; j4 <= kmax / 4; ++j4) {
+        const float v0 = vec[(4 * j4) * stride], v1 = vec[(4 * j4 + 1) * stride];
+        const float v2 = vec[(4 * j4 + 2) * stride], v3 = vec[(4 * j4 + 3) * stride];
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+          const int k = k0 + (256 / kSW) * q;
+          const float4 m4 = *reinterpret_cast<const float4*>(&mat[k][4 * j4]);
+          acc[q] += (4 * j4 <= k) ? m4.x * v0 : 0.0f;
+          acc[q] += (4 * j4 + 1 <= k) ? m4.y * v1 : 0.0f;
+          acc[q] += (4 * j4 + 2 <= k) ? m4.z * v2 : 0.0f;
+          acc[q] += (4 * j4 + 3 <= k) ? m4.w * v3 : 0.0f;
+        }
       }
-      return acc;
     };
-    float bv[kPer], xv[kPer];
+    const int cc = tid % kSW;
+    float bv[kPer], xv[kPer], dv[kPer];
 #pragma unroll
     for (int q = 0; q < kPer; ++q) {
-      const int e = tid + 256 * q, k = e / kSW, cc = e % kSW;
+      const int k = (tid + 256 * q) / kSW;
       bv[q] = xs[(32 * i + k) * kSW + cc];
-      xv[q] = lower_dot(sinv, k, xs + (32 * i) * kSW + cc, kSW);
     }
+    lower_dot(sinv, xs + (32 * i) * kSW + cc, kSW, xv);
 #pragma unroll
     for (int q = 0; q < kPer; ++q) {
-      const int e = tid + 256 * q, k = e / kSW, cc = e % kSW;
+      const int k = (tid + 256 * q) / kSW;
       sb[k][cc] = xv[q];
     }
     __syncthreads();
+    lower_dot(sl, &sb[0][cc], kSW, dv);
 #pragma unroll
-    for (int q = 0; q < kPer; ++q) {
-      const int e = tid + 256 * q, k = e / kSW, cc = e % kSW;
-      bv[q] -= lower_dot(sl, k, &sb[0][cc], kSW + 1);  // the residual of row k
-    }
+    for (int q = 0; q < kPer; ++q) bv[q] -= dv[q];  // the residuals
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < kPer; ++q) {
-      const int e = tid + 256 * q, k = e / kSW, cc = e % kSW;
+      const int k = (tid + 256 * q) / kSW;
       xs[(32 * i + k) * kSW + cc] = bv[q];
     }
     __syncthreads();
+    lower_dot(sinv, xs + (32 * i) * kSW + cc, kSW, dv);
 #pragma unroll
-    for (int q = 0; q < kPer; ++q) {
-      const int e = tid + 256 * q, k = e / kSW, cc = e % kSW;
-      bv[q] = xv[q] + lower_dot(sinv, k, xs + (32 * i) * kSW + cc, kSW);
-    }
+    for (int q = 0; q < kPer; ++q) bv[q] = xv[q] + dv[q];
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < kPer; ++q) {
-      const int e = tid + 256 * q, k = e / kSW, cc = e % kSW;
+      const int k = (tid + 256 * q) / kSW;
       xs[(32 * i + k) * kSW + cc] = bv[q];
     }
     __syncthreads();
