@@ -741,58 +741,110 @@ __global__ __launch_bounds__(256, 2) void k_gemm64(long long M, long long N, int
   }
 }
 
-// Per-64-column maxima of |U12| (fp16 mode): colscale[b] = max |B[:, 64b..64b+64)|.
-__global__ __launch_bounds__(256) void k_colblock_scale(int K, long long N, const float* __restrict__ B,
-                                                        long long ldb, float* __restrict__ scale) {
-  __shared__ float red[256];
-  const long long c0 = (long long)blockIdx.x * 64;
-  float m = 0.0f;
-  for (int i = threadIdx.x; i < K * 64; i += 256) {
-    const int r = i / 64, c = i % 64;
-    if (c0 + c < N) m = fmaxf(m, fabsf(B[(long long)r * ldb + c0 + c]));
-  }
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) scale[blockIdx.x] = red[0] > 0.0f ? red[0] : 1.0f;
-}
-
 // FP16 trailing update.  Operands are converted once per update: A16 = L21
 // rounded to fp16 (|l| <= 1 under partial pivoting, no scale needed), B16 =
 // U12 transposed to (N x K) and divided by the per-64-column maxima colscale.
 // Both are K-contiguous, K padded to a multiple of 32 with zeros.
+//
+// A16: one thread per 8 halves; rows read as two float4 when vec (A and lda
+// 16-byte aligned, K a multiple of 8).
 __global__ __launch_bounds__(256) void k_cvt_a16(long long M, int K, int Kp, const float* __restrict__ A, long long lda,
-                                                 _Float16* __restrict__ A16) {
-  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;  // one thread per 8 halves
+                                                 _Float16* __restrict__ A16, int vec) {
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
   const int per = Kp / 8;
   if (idx >= M * per) return;
   const long long r = idx / per;
   const int k0 = (int)(idx % per) * 8;
   h8 v;
+  if (vec && k0 + 8 <= K) {
+    const float4 p0 = *reinterpret_cast<const float4*>(A + r * lda + k0);
+    const float4 p1 = *reinterpret_cast<const float4*>(A + r * lda + k0 + 4);
+    v[0] = (_Float16)p0.x;
+    v[1] = (_Float16)p0.y;
+    v[2] = (_Float16)p0.z;
+    v[3] = (_Float16)p0.w;
+    v[4] = (_Float16)p1.x;
+    v[5] = (_Float16)p1.y;
+    v[6] = (_Float16)p1.z;
+    v[7] = (_Float16)p1.w;
+  } else {
 #pragma unroll
-  for (int q = 0; q < 8; ++q) v[q] = (_Float16)((k0 + q < K) ? A[r * lda + k0 + q] : 0.0f);
+    for (int q = 0; q < 8; ++q) v[q] = (_Float16)((k0 + q < K) ? A[r * lda + k0 + q] : 0.0f);
+  }
   *(h8*)(A16 + r * Kp + k0) = v;
 }
 
-__global__ __launch_bounds__(256) void k_cvt_b16t(int K, long long N, int Kp, const float* __restrict__ B, long long ldb,
-                                                  const float* __restrict__ colscale, _Float16* __restrict__ B16) {
-  __shared__ float tile[32][33];
-  const long long n0 = (long long)blockIdx.x * 32;
-  const int k0 = blockIdx.y * 32;
-  const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;
-  for (int q = ty; q < 32; q += 8) {
-    const int k = k0 + q;
-    const long long n = n0 + tx;
-    tile[q][tx] = (k < K && n < N) ? B[(long long)k * ldb + n] / colscale[n >> 6] : 0.0f;  // tile[k][n]
+// B16 and colscale in one pass per 64-column block of B (K x N, row-major):
+// the block's max |b| (1 where it is 0) goes to colscale, then the block is
+// transposed through LDS in 32-row slices and written as B16[n][k] = b / scale
+// (rows k >= K zero).  The second read of the block mostly hits L2.  vec: B
+// and ldb 16-byte aligned, so interior rows are read as float4.
+__global__ __launch_bounds__(256) void k_scale_cvt_b16t(int K, long long N, int Kp, const float* __restrict__ B,
+                                                        long long ldb, float* __restrict__ colscale,
+                                                        _Float16* __restrict__ B16, int vec) {
+  __shared__ float red[4];
+  __shared__ float tile[32][65];
+  const int tid = threadIdx.x;
+  const long long c0 = (long long)blockIdx.x * 64;
+  const int cw = (int)min(64LL, N - c0);  // columns of this block
+  const bool fullw = vec && cw == 64;
+  const int cq = (tid % 16) * 4, rq = tid / 16;  // 16 rows per pass, 4 columns per thread
+  float m = 0.0f;
+  if (fullw) {
+#pragma unroll 4
+    for (int r = rq; r < K; r += 16) {
+      const float4 v = *reinterpret_cast<const float4*>(B + (long long)r * ldb + c0 + cq);
+      m = fmaxf(m, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
+    }
+  } else {
+    for (int r = rq; r < K; r += 16)
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        if (cq + t < cw) m = fmaxf(m, fabsf(B[(long long)r * ldb + c0 + cq + t]));
   }
+  // block maximum: DPP-free wave reduction through shuffles, then 4 waves
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+  if ((tid & 63) == 0) red[tid >> 6] = m;
   __syncthreads();
-  for (int q = ty; q < 32; q += 8) {
-    const long long n = n0 + q;
-    const int k = k0 + tx;
-    if (n < N && k < Kp) B16[n * Kp + k] = (_Float16)tile[tx][q];
+  const float mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+  const float scale = mx > 0.0f ? mx : 1.0f;
+  if (tid == 0) colscale[blockIdx.x] = scale;
+  // transpose: slice rows [k0, k0+32) into tile[k][n], then 64 rows of B16
+  const int wn = tid / 4, wk = (tid % 4) * 8;  // writer: column n, 8 k's
+  for (int k0 = 0; k0 < Kp; k0 += 32) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int kr = rq + 16 * h, k = k0 + kr;
+      float x0 = 0.0f, x1 = 0.0f, x2 = 0.0f, x3 = 0.0f;
+      if (k < K) {
+        if (fullw) {
+          const float4 v = *reinterpret_cast<const float4*>(B + (long long)k * ldb + c0 + cq);
+          x0 = v.x;
+          x1 = v.y;
+          x2 = v.z;
+          x3 = v.w;
+        } else {
+          const float* src = B + (long long)k * ldb + c0 + cq;
+          if (cq < cw) x0 = src[0];
+          if (cq + 1 < cw) x1 = src[1];
+          if (cq + 2 < cw) x2 = src[2];
+          if (cq + 3 < cw) x3 = src[3];
+        }
+      }
+      tile[kr][cq] = x0 / scale;
+      tile[kr][cq + 1] = x1 / scale;
+      tile[kr][cq + 2] = x2 / scale;
+      tile[kr][cq + 3] = x3 / scale;
+    }
+    __syncthreads();
+    if (wn < cw) {
+      h8 o;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) o[q] = (_Float16)tile[wk + q][wn];
+      *(h8*)(B16 + (c0 + wn) * Kp + k0 + wk) = o;
+    }
+    __syncthreads();
   }
 }
 
@@ -1084,7 +1136,15 @@ static int launch_f16_update(qd_context* h, bool trailing, long long M, long lon
   return QD_OK;
 }
 
-constexpr double kF16MinWork = 2.0e9;
+// Non-trailing FP32 updates of at least this many flops go through FP16
+// operand copies (QUADESTH_F16_MIN_WORK overrides it).
+static double f16_min_work() {
+  static const double v = [] {
+    const char* e = std::getenv("QUADESTH_F16_MIN_WORK");
+    return e ? std::atof(e) : 2.0e9;
+  }();
+  return v;
+}
 constexpr int kStripMaxRows = 1024;      // LDS of k_trsm_strip<8>: 32 KiB
 constexpr long long kStripNarrowCols = 8192;
 
@@ -1099,21 +1159,21 @@ static int gemm_sub(LuCtx<T>& c, long long M, long long Nn, int K, const T* A, c
   int rc_ = QD_OK;
   dim3 grid((unsigned)((Nn + GBN - 1) / GBN), (unsigned)((M + GBM - 1) / GBM));
   if constexpr (std::is_same<T, float>::value) {
-    // FP16 conversion costs three extra launches; below kF16MinWork flops the
-    // FP32 MFMA kernel on W directly is faster.
-    if (c.f16 && allow_f16 && (trailing || (double)M * (double)Nn * (double)K >= kF16MinWork)) {
+    // FP16 conversion costs three extra launches; below f16_min_work() flops
+    // the FP32 MFMA kernel on W directly is faster.
+    if (c.f16 && allow_f16 && (trailing || (double)M * (double)Nn * (double)K >= f16_min_work())) {
       const long long nb = (Nn + 63) / 64;
       const int Kp = (K + 31) / 32 * 32;
       QD_ALLOC(c.h, c.sc->colscale, (size_t)nb);
       QD_ALLOC(c.h, c.sc->h16a, (size_t)M * Kp);
       QD_ALLOC(c.h, c.sc->h16b, (size_t)Nn * Kp);
-      hipLaunchKernelGGL(k_colblock_scale, dim3((unsigned)nb), dim3(256), 0, c.st, K, Nn, B, c.ld,
-                         c.sc->colscale.p);
-      hipLaunchKernelGGL(k_cvt_b16t, dim3((unsigned)((Nn + 31) / 32), (unsigned)(Kp / 32)), dim3(256), 0,
-                         c.st, K, Nn, Kp, B, c.ld, c.sc->colscale.p, c.sc->h16b.p);
+      const int vb = ((reinterpret_cast<uintptr_t>(B) & 15) == 0 && c.ld % 4 == 0) ? 1 : 0;
+      const int va = ((reinterpret_cast<uintptr_t>(A) & 15) == 0 && c.ld % 4 == 0 && K % 8 == 0) ? 1 : 0;
+      hipLaunchKernelGGL(k_scale_cvt_b16t, dim3((unsigned)nb), dim3(256), 0, c.st, K, Nn, Kp, B, c.ld,
+                         c.sc->colscale.p, c.sc->h16b.p, vb);
       if (!reuse_a16)
         hipLaunchKernelGGL(k_cvt_a16, dim3((unsigned)((M * (Kp / 8) + 255) / 256)), dim3(256), 0, c.st, M, K,
-                           Kp, A, c.ld, c.sc->h16a.p);
+                           Kp, A, c.ld, c.sc->h16a.p, va);
       ProfScope ps(c.h, trailing ? QD_PROF_LU_TRAILING_GEMM : -1, 2.0 * (double)M * (double)Nn * K, c.st);
       const int rows16 = (c.ld % 4 == 0 && (reinterpret_cast<uintptr_t>(C) & 15) == 0) ? 1 : 0;
       if ((rc_ = launch_f16_update(c.h, trailing, M, Nn, Kp, c.sc->h16a.p, c.sc->h16b.p, c.sc->colscale.p, C, c.ld,
