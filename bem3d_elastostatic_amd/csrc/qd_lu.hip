@@ -634,6 +634,100 @@ __global__ __launch_bounds__(256) void k_gemm_sub(long long M, long long N, int 
   }
 }
 
+// FP32 C[M x N] -= A[M x K] B[K x N] for the narrow updates inside a panel
+// (N <= NW, NW = 32 or 64), MFMA f32 16x16x4.  128 rows per workgroup, 32 per
+// wave.  The sum over k may take the k's in any order as long as A and B
+// agree, so in each 32-wide k slice lane group g of an MFMA supplies k = 8 g +
+// step: a lane's A values for the slice are then 8 contiguous floats of its
+// row, read straight from memory as two float4 (the four lane groups cover the
+// row's 128 bytes).  The B slice (32 x NW) is staged in LDS.  The result goes
+// through LDS so that C is read and written as float4 along its rows.
+template <int NW>
+__global__ __launch_bounds__(256) void k_gemm_narrow(long long M, int N, int K, const float* __restrict__ A,
+                                                     long long lda, const float* __restrict__ B, long long ldb,
+                                                     float* __restrict__ C, long long ldc) {
+  constexpr int NT = NW / 16;
+  __shared__ float sB[32][NW + 1];
+  __shared__ __attribute__((aligned(16))) float sC[4][32][NW + 4];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int lr = lane & 15, lg = lane >> 4;
+  const long long m0 = (long long)blockIdx.x * 128 + wave * 32;
+  const bool avec = ((reinterpret_cast<uintptr_t>(A) & 15) == 0) && (lda % 4 == 0);
+  f4 acc[2][NT];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[i][j] = (f4){0.0f, 0.0f, 0.0f, 0.0f};
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    float a[2][8];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const long long row = m0 + 16 * i + lr;
+      const int kb = k0 + 8 * lg;
+      if (row < M && avec && kb + 8 <= K) {
+        const float4 p0 = *reinterpret_cast<const float4*>(A + row * lda + kb);
+        const float4 p1 = *reinterpret_cast<const float4*>(A + row * lda + kb + 4);
+        a[i][0] = p0.x;
+        a[i][1] = p0.y;
+        a[i][2] = p0.z;
+        a[i][3] = p0.w;
+        a[i][4] = p1.x;
+        a[i][5] = p1.y;
+        a[i][6] = p1.z;
+        a[i][7] = p1.w;
+      } else {
+#pragma unroll
+        for (int t = 0; t < 8; ++t) a[i][t] = (row < M && kb + t < K) ? A[row * lda + kb + t] : 0.0f;
+      }
+    }
+    if (k0 > 0) __syncthreads();  // the previous slice of B is consumed
+    for (int e = tid; e < 32 * NW; e += 256) {
+      const int kr = e / NW, c = e % NW;
+      sB[kr][c] = (k0 + kr < K && c < N) ? B[(long long)(k0 + kr) * ldb + c] : 0.0f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        const float b = sB[8 * lg + t][16 * j + lr];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][t], b, acc[i][j], 0, 0, 0);
+      }
+    }
+  }
+  // the wave's 32 x NW result into LDS (row 16 i + 4 lg + r, column 16 j + lr)
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) sC[wave][16 * i + 4 * lg + r][16 * j + lr] = acc[i][j][r];
+  __syncthreads();
+  const bool cvec = ((reinterpret_cast<uintptr_t>(C) & 15) == 0) && (ldc % 4 == 0);
+  constexpr int L4 = NW / 4;  // float4 pieces per row
+  for (int e = lane; e < 32 * L4; e += 64) {
+    const int rr = e / L4, c4 = (e % L4) * 4;
+    const long long row = m0 + rr;
+    if (row >= M) continue;
+    const float4 d = *reinterpret_cast<const float4*>(&sC[wave][rr][c4]);
+    float* cp = C + row * ldc + c4;
+    if (cvec && c4 + 4 <= N) {
+      float4 cv = *reinterpret_cast<const float4*>(cp);
+      cv.x -= d.x;
+      cv.y -= d.y;
+      cv.z -= d.z;
+      cv.w -= d.w;
+      *reinterpret_cast<float4*>(cp) = cv;
+    } else {
+      if (c4 < N) cp[0] -= d.x;
+      if (c4 + 1 < N) cp[1] -= d.y;
+      if (c4 + 2 < N) cp[2] -= d.z;
+      if (c4 + 3 < N) cp[3] -= d.w;
+    }
+  }
+}
+
 // FP64 C[M x N] -= A[M x K] B[K x N] (row-major, inside W) with MFMA
 // f64 16x16x4.  128 x 128 tile per 256-thread workgroup (4 waves of 64 x 64,
 // 4 x 4 MFMA tiles), K steps of 16 through double-buffered LDS with register
@@ -1136,6 +1230,15 @@ static int launch_f16_update(qd_context* h, bool trailing, long long M, long lon
   return QD_OK;
 }
 
+// QUADESTH_NARROW_GEMM=0 sends the narrow FP32 updates to the square-tile kernel.
+static bool narrow_gemm() {
+  static const bool on = [] {
+    const char* e = std::getenv("QUADESTH_NARROW_GEMM");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
 // Non-trailing FP32 updates of at least this many flops go through FP16
 // operand copies (QUADESTH_F16_MIN_WORK overrides it).
 static double f16_min_work() {
@@ -1194,6 +1297,18 @@ static int gemm_sub(LuCtx<T>& c, long long M, long long Nn, int K, const T* A, c
     return QD_OK;
   }
   ProfScope ps(c.h, trailing ? QD_PROF_LU_TRAILING_GEMM : -1, 2.0 * (double)M * (double)Nn * K, c.st);
+  if constexpr (std::is_same<T, float>::value) {
+    // narrow updates (inside the panels): one launch of the column-narrow kernel
+    if (Nn <= 64 && narrow_gemm()) {
+      const dim3 gn((unsigned)((M + 127) / 128));
+      if (Nn <= 32)
+        hipLaunchKernelGGL(k_gemm_narrow<32>, gn, dim3(256), 0, c.st, M, (int)Nn, K, A, c.ld, B, c.ld, C, c.ld);
+      else
+        hipLaunchKernelGGL(k_gemm_narrow<64>, gn, dim3(256), 0, c.st, M, (int)Nn, K, A, c.ld, B, c.ld, C, c.ld);
+      QD_HIP(c.h, hipGetLastError());
+      return QD_OK;
+    }
+  }
   hipLaunchKernelGGL(k_gemm_sub<T>, grid, dim3(256), 0, c.st, M, Nn, K, A, c.ld, B, c.ld, C, c.ld);
   QD_HIP(c.h, hipGetLastError());
   return QD_OK;

@@ -507,7 +507,14 @@ __global__ __launch_bounds__(256) void k_tslu_finish(float* __restrict__ W, long
   }
   tf_solve(std::make_integer_sequence<int, kTsW>{}, x, w, sU, sR);
   float* dst = W + r * ld + c0;
-  for (int j = 0; j < w; ++j) dst[j] = x[j];
+  if (w == kTsW) {
+    // the segment is 128-byte aligned (c0 a multiple of 32, ld of 64)
+#pragma unroll
+    for (int j4 = 0; j4 < kTsW / 4; ++j4)
+      *reinterpret_cast<float4*>(dst + 4 * j4) = make_float4(x[4 * j4], x[4 * j4 + 1], x[4 * j4 + 2], x[4 * j4 + 3]);
+  } else {
+    for (int j = 0; j < w; ++j) dst[j] = x[j];
+  }
 }
 
 }  // namespace qd
