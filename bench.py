@@ -450,7 +450,7 @@ def main():
                            tflops=(work / (ms * 1e-3) / 1e12) if ms > 0 else None)
     extra["kernels"] = prof_out
     # Dominant single kernel by device time.  lu_panel is excluded: it is the
-    # panel factorisation's whole sequence (about 150 small dependent
+    # panel factorisation's whole sequence (about 130 small dependent
     # launches per panel), whose time is reported in "kernels" and "phases_s".
     dom = max((k for k in prof_out if k != "lu_panel"), key=lambda k: prof_out[k]["ms"])
     d = prof_out[dom]
