@@ -59,6 +59,7 @@ struct qd_lu_state {
     qd::DevBuf<_Float16> h16a, h16b;
   };
   F16Scratch sc_main, sc_panel;
+  F16Scratch sc_solve;  // the inner updates of the main stream's U12 solve, so sc_main keeps L21
   hipStream_t aux = nullptr;     // panel stream of the look-ahead (non-blocking, default priority)
   hipEvent_t ev_cols = nullptr, ev_panel = nullptr;
   qd::DevBuf<double> vec;        // solve / refinement work vectors
@@ -1213,6 +1214,7 @@ struct LuCtx {
   bool calu;  // tournament pivoting in the leaves (preconditioner factorisations)
   hipStream_t st = nullptr;                   // stream of every launch of this context
   qd_lu_state::F16Scratch* sc = nullptr;      // FP16 operand copies of this context
+  qd_lu_state::F16Scratch* sc_solve = nullptr;  // those of trsm_lower's inner updates (nullptr: sc)
 };
 
 // The FP16 update C -= (A16 B16^T) * colscale on stream st; max_wgs > 0
@@ -1347,7 +1349,9 @@ static int trsm_lower(LuCtx<T>& c, long long r0, int m, long long c0, long long 
   int rc = trsm_lower(c, r0, m1, c0, c1);
   if (rc) return rc;
   // B2 -= L21 X1
-  rc = gemm_sub(c, m - m1, c1 - c0, m1, c.W + (r0 + m1) * c.ld + r0, c.W + r0 * c.ld + c0,
+  LuCtx<T> cs = c;
+  if (c.sc_solve) cs.sc = c.sc_solve;
+  rc = gemm_sub(cs, m - m1, c1 - c0, m1, c.W + (r0 + m1) * c.ld + r0, c.W + r0 * c.ld + c0,
                 c.W + (r0 + m1) * c.ld + c0, true);
   if (rc) return rc;
   return trsm_lower(c, r0 + m1, m - m1, c0, c1);
@@ -1490,6 +1494,9 @@ static int factor(LuCtx<T>& c, int NB) {
     QD_ALLOC(c.h, s->sc_panel.h16a, (size_t)N * Kh);
     QD_ALLOC(c.h, s->sc_panel.h16b, Kh * Kh);
     QD_ALLOC(c.h, s->sc_panel.colscale, (size_t)(NB / 64 + 2));
+    QD_ALLOC(c.h, s->sc_solve.h16a, Kh * Kh);
+    QD_ALLOC(c.h, s->sc_solve.h16b, (size_t)N * Kh);
+    QD_ALLOC(c.h, s->sc_solve.colscale, (size_t)(N / 64 + 2));
   }
   const char* la = std::getenv("QUADESTH_LOOKAHEAD");
   const bool look = !(la && la[0] == '0');
@@ -1506,7 +1513,9 @@ static int factor_steps(LuCtx<T>& c, int NB, bool look) {
   qd_lu_state* s = c.s;
   const long long N = c.N;
   c.sc = &s->sc_main;
+  c.sc_solve = &s->sc_solve;
   LuCtx<T> cp = c;
+  cp.sc_solve = nullptr;
   cp.st = s->aux;
   cp.sc = &s->sc_panel;
   QD_HIP(c.h, hipMemsetAsync(s->piv_state.p, 0, 2 * sizeof(int), c.st));
@@ -1534,7 +1543,9 @@ static int factor_steps(LuCtx<T>& c, int NB, bool look) {
         // be held up enqueuing the panel's many small launches, and the
         // main stream would idle meanwhile.
         if ((rc = trsm_lower(c, j, jb, nj + njb, N))) return rc;
-        if ((rc = update_cols(c, j, jb, nj + njb, N, false, lookahead_gemm_wgs()))) return rc;
+        // L21's FP16 copy from the update above is still in sc_main (the U12
+        // solve's inner updates use sc_solve)
+        if ((rc = update_cols(c, j, jb, nj + njb, N, true, lookahead_gemm_wgs()))) return rc;
         QD_HIP(c.h, hipStreamWaitEvent(cp.st, s->ev_cols, 0));
         if ((rc = panel_step(cp, nj, njb))) return rc;
         QD_HIP(c.h, hipEventRecord(s->ev_panel, cp.st));
