@@ -360,12 +360,16 @@ __global__ __launch_bounds__(256) void k_trsm_strip(float* __restrict__ W, long 
       v[u] = *reinterpret_cast<const float4*>(W + (r0 + row) * ld + d0 + 4 * k4);
     }
   };
+  // the wide strips read chunk rows as MFMA operands, 16 rows per read at the
+  // same column: their 16-byte pieces are XOR-swizzled by row so that those
+  // reads spread over the banks
+  auto lswz = [](int rr) { return (kSW == 32) ? (rr & 7) : 0; };
   auto stage_chunk = [&](int buf, const float4 (&v)[4]) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int e = tid + 256 * u, rr = e >> 3, k4 = e & 7;
       const float4 t = v[u];
-      *reinterpret_cast<float4*>(&lbuf[buf][rr][4 * k4]) = t;
+      *reinterpret_cast<float4*>(&lbuf[buf][rr][4 * (k4 ^ lswz(rr))]) = t;
     }
   };
   // a diagonal block's inverse and L, four elements of each per thread
@@ -388,90 +392,175 @@ __global__ __launch_bounds__(256) void k_trsm_strip(float* __restrict__ W, long 
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int e = tid + 256 * u, rr = e >> 5, kk = e & 31;
-      sinv[rr][kk] = pinv[u];
-      sl[rr][kk] = pl[u];
+      const int ks = (((kk >> 2) ^ lswz(rr)) << 2) | (kk & 3);
+      sinv[rr][ks] = pinv[u];
+      sl[rr][ks] = pl[u];
     }
     if (i + 1 < m / 32) load_diag(i + 1);  // in flight until the next block
     __syncthreads();
-    // x = inv b, then x += inv (b - L x).  A thread's kPer elements share
-    // their column (cc = tid % kSW) and sit in rows k_q = tid / kSW + q 256 /
-    // kSW, so one pass over j serves all of them: the vector elements are
-    // read once per j, the matrix rows four entries at a time, and the kPer
-    // sums are independent chains.
-    auto lower_dot = [&](const float (*mat)[32], const float* vec, int stride, float (&acc)[kPer]) {
-      const int k0 = tid / kSW;
-      const int kmax = k0 + (256 / kSW) * (kPer - 1);
+    if constexpr (kSW == 32) {
+      // x = inv b, then x += inv (b - L x), as three MFMA f32 16x16x4
+      // products on the 32 x 32 block: wave w forms the output tile at rows
+      // 16 (w >> 1), columns 16 (w & 1).  The triangular factors hold zeros
+      // above the diagonal, so the products are dense.  The k's are permuted
+      // as in the chunk update: A rows (inverse or L) are read as two
+      // swizzled 16-byte pieces, B (32 x 32, row-major) element by element.
+      const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lg = lane >> 4;
+      const int ti = wave >> 1, tj = wave & 1, arow = 16 * ti + lr;
+      auto mfma_blk = [&](const float (*mat)[32], const float* bsrc) {
+        const float4 a0 = *reinterpret_cast<const float4*>(&mat[arow][4 * ((2 * lg) ^ lswz(arow))]);
+        const float4 a1 = *reinterpret_cast<const float4*>(&mat[arow][4 * ((2 * lg + 1) ^ lswz(arow))]);
+        const float av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+        f4 acc = (f4){0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-      for (int q = 0; q < kPer; ++q) acc[q] = 0.0f;
-#pragma unroll 2
-      for (int j4 = 0; j4 <= kmax / 4; ++j4) {
-        const float v0 = vec[(4 * j4) * stride], v1 = vec[(4 * j4 + 1) * stride];
-        const float v2 = vec[(4 * j4 + 2) * stride], v3 = vec[(4 * j4 + 3) * stride];
+        for (int t = 0; t < 8; ++t)
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[t], bsrc[(8 * lg + t) * 32 + 16 * tj + lr], acc, 0, 0, 0);
+        return acc;
+      };
+      float* xb = xs + (32 * i) * kSW;  // the block's rows of the strip
+      float bown[4];
 #pragma unroll
-        for (int q = 0; q < kPer; ++q) {
-          const int k = k0 + (256 / kSW) * q;
-          const float4 m4 = *reinterpret_cast<const float4*>(&mat[k][4 * j4]);
-          acc[q] += (4 * j4 <= k) ? m4.x * v0 : 0.0f;
-          acc[q] += (4 * j4 + 1 <= k) ? m4.y * v1 : 0.0f;
-          acc[q] += (4 * j4 + 2 <= k) ? m4.z * v2 : 0.0f;
-          acc[q] += (4 * j4 + 3 <= k) ? m4.w * v3 : 0.0f;
-        }
-      }
-    };
-    const int cc = tid % kSW;
-    float bv[kPer], xv[kPer], dv[kPer];
+      for (int q = 0; q < 4; ++q) bown[q] = xb[(16 * ti + 4 * lg + q) * 32 + 16 * tj + lr];
+      const f4 xv = mfma_blk(sinv, xb);
 #pragma unroll
-    for (int q = 0; q < kPer; ++q) {
-      const int k = (tid + 256 * q) / kSW;
-      bv[q] = xs[(32 * i + k) * kSW + cc];
-    }
-    lower_dot(sinv, xs + (32 * i) * kSW + cc, kSW, xv);
-#pragma unroll
-    for (int q = 0; q < kPer; ++q) {
-      const int k = (tid + 256 * q) / kSW;
-      sb[k][cc] = xv[q];
-    }
-    __syncthreads();
-    lower_dot(sl, &sb[0][cc], kSW, dv);
-#pragma unroll
-    for (int q = 0; q < kPer; ++q) bv[q] -= dv[q];  // the residuals
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < kPer; ++q) {
-      const int k = (tid + 256 * q) / kSW;
-      xs[(32 * i + k) * kSW + cc] = bv[q];
-    }
-    __syncthreads();
-    lower_dot(sinv, xs + (32 * i) * kSW + cc, kSW, dv);
-#pragma unroll
-    for (int q = 0; q < kPer; ++q) bv[q] = xv[q] + dv[q];
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < kPer; ++q) {
-      const int k = (tid + 256 * q) / kSW;
-      xs[(32 * i + k) * kSW + cc] = bv[q];
-    }
-    __syncthreads();
-    // the rows below, chunk by chunk: X_r -= L[r, block i] X_i
-    float xi[32];
-#pragma unroll
-    for (int k = 0; k < 32; ++k) xi[k] = xs[(32 * i + k) * kSW + c];
-    if (nch > 0) stage_chunk(0, pf);
-    __syncthreads();
-    for (int ch = 0; ch < nch; ++ch) {
-      const int first = below + ch * kStripChunk;
-      if (ch + 1 < nch) load_chunk(d0, first + kStripChunk, pf);
-      const int buf = ch & 1;
-#pragma unroll
-      for (int t = 0; t < kRowsPer; ++t) {
-        const int rr = slot + kSlots * t, r = first + rr;
-        float sum = 0.0f;
-#pragma unroll
-        for (int k = 0; k < 32; ++k) sum += lbuf[buf][rr][k] * xi[k];
-        if (r < m) xs[r * kSW + c] -= sum;
-      }
-      if (ch + 1 < nch) stage_chunk(buf ^ 1, pf);
+      for (int q = 0; q < 4; ++q) sb[16 * ti + 4 * lg + q][16 * tj + lr] = xv[q];
       __syncthreads();
+      const f4 lx = mfma_blk(sl, &sb[0][0]);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) xb[(16 * ti + 4 * lg + q) * 32 + 16 * tj + lr] = bown[q] - lx[q];  // residual
+      __syncthreads();
+      const f4 dx = mfma_blk(sinv, xb);
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 4; ++q) xb[(16 * ti + 4 * lg + q) * 32 + 16 * tj + lr] = xv[q] + dx[q];
+      __syncthreads();
+    } else {
+      // x = inv b, then x += inv (b - L x).  A thread's kPer elements share
+      // their column (cc = tid % kSW) and sit in rows k_q = tid / kSW + q 256 /
+      // kSW, so one pass over j serves all of them: the vector elements are
+      // read once per j, the matrix rows four entries at a time, and the kPer
+      // sums are independent chains.
+      auto lower_dot = [&](const float (*mat)[32], const float* vec, int stride, float (&acc)[kPer]) {
+        const int k0 = tid / kSW;
+        const int kmax = k0 + (256 / kSW) * (kPer - 1);
+  #pragma unroll
+        for (int q = 0; q < kPer; ++q) acc[q] = 0.0f;
+  #pragma unroll 2
+        for (int j4 = 0; j4 <= kmax / 4; ++j4) {
+          const float v0 = vec[(4 * j4) * stride], v1 = vec[(4 * j4 + 1) * stride];
+          const float v2 = vec[(4 * j4 + 2) * stride], v3 = vec[(4 * j4 + 3) * stride];
+  #pragma unroll
+          for (int q = 0; q < kPer; ++q) {
+            const int k = k0 + (256 / kSW) * q;
+            const float4 m4 = *reinterpret_cast<const float4*>(&mat[k][4 * j4]);
+            acc[q] += (4 * j4 <= k) ? m4.x * v0 : 0.0f;
+            acc[q] += (4 * j4 + 1 <= k) ? m4.y * v1 : 0.0f;
+            acc[q] += (4 * j4 + 2 <= k) ? m4.z * v2 : 0.0f;
+            acc[q] += (4 * j4 + 3 <= k) ? m4.w * v3 : 0.0f;
+          }
+        }
+      };
+      const int cc = tid % kSW;
+      float bv[kPer], xv[kPer], dv[kPer];
+  #pragma unroll
+      for (int q = 0; q < kPer; ++q) {
+        const int k = (tid + 256 * q) / kSW;
+        bv[q] = xs[(32 * i + k) * kSW + cc];
+      }
+      lower_dot(sinv, xs + (32 * i) * kSW + cc, kSW, xv);
+  #pragma unroll
+      for (int q = 0; q < kPer; ++q) {
+        const int k = (tid + 256 * q) / kSW;
+        sb[k][cc] = xv[q];
+      }
+      __syncthreads();
+      lower_dot(sl, &sb[0][cc], kSW, dv);
+  #pragma unroll
+      for (int q = 0; q < kPer; ++q) bv[q] -= dv[q];  // the residuals
+      __syncthreads();
+  #pragma unroll
+      for (int q = 0; q < kPer; ++q) {
+        const int k = (tid + 256 * q) / kSW;
+        xs[(32 * i + k) * kSW + cc] = bv[q];
+      }
+      __syncthreads();
+      lower_dot(sinv, xs + (32 * i) * kSW + cc, kSW, dv);
+  #pragma unroll
+      for (int q = 0; q < kPer; ++q) bv[q] = xv[q] + dv[q];
+      __syncthreads();
+  #pragma unroll
+      for (int q = 0; q < kPer; ++q) {
+        const int k = (tid + 256 * q) / kSW;
+        xs[(32 * i + k) * kSW + cc] = bv[q];
+      }
+      __syncthreads();
+    }
+    // the rows below, chunk by chunk: X_r -= L[r, block i] X_i
+    if (nch > 0) stage_chunk(0, pf);
+    if constexpr (kSW == 32) {
+      // MFMA f32 16x16x4: wave w updates chunk rows [32 w, 32 w + 32) x the
+      // 32 columns.  The k's of a step are permuted (lane group g supplies
+      // k = 8 g + t at step t), so a lane's L values are two 16-byte pieces
+      // of its row and its X_i values are loaded once per block.
+      const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lg = lane >> 4;
+      float xbf[2][8];
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int t = 0; t < 8; ++t) xbf[j][t] = xs[(32 * i + 8 * lg + t) * kSW + 16 * j + lr];
+      __syncthreads();
+      for (int ch = 0; ch < nch; ++ch) {
+        const int first = below + ch * kStripChunk;
+        if (ch + 1 < nch) load_chunk(d0, first + kStripChunk, pf);
+        const int buf = ch & 1;
+        f4 acc[2][2];
+#pragma unroll
+        for (int i2 = 0; i2 < 2; ++i2) {
+          const int rr = 32 * wave + 16 * i2 + lr;
+          const float* rowp = &lbuf[buf][rr][0];
+          const float4 a0 = *reinterpret_cast<const float4*>(rowp + 4 * ((2 * lg) ^ lswz(rr)));
+          const float4 a1 = *reinterpret_cast<const float4*>(rowp + 4 * ((2 * lg + 1) ^ lswz(rr)));
+          const float av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            acc[i2][j] = (f4){0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+              acc[i2][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[t], xbf[j][t], acc[i2][j], 0, 0, 0);
+          }
+        }
+#pragma unroll
+        for (int i2 = 0; i2 < 2; ++i2)
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int q4 = 0; q4 < 4; ++q4) {
+              const int r = first + 32 * wave + 16 * i2 + 4 * lg + q4;
+              if (r < m) xs[r * kSW + 16 * j + lr] -= acc[i2][j][q4];
+            }
+        if (ch + 1 < nch) stage_chunk(buf ^ 1, pf);
+        __syncthreads();
+      }
+    } else {
+      float xi[32];
+#pragma unroll
+      for (int k = 0; k < 32; ++k) xi[k] = xs[(32 * i + k) * kSW + c];
+      __syncthreads();
+      for (int ch = 0; ch < nch; ++ch) {
+        const int first = below + ch * kStripChunk;
+        if (ch + 1 < nch) load_chunk(d0, first + kStripChunk, pf);
+        const int buf = ch & 1;
+#pragma unroll
+        for (int t = 0; t < kRowsPer; ++t) {
+          const int rr = slot + kSlots * t, r = first + rr;
+          float sum = 0.0f;
+#pragma unroll
+          for (int k = 0; k < 32; ++k) sum += lbuf[buf][rr][k] * xi[k];
+          if (r < m) xs[r * kSW + c] -= sum;
+        }
+        if (ch + 1 < nch) stage_chunk(buf ^ 1, pf);
+        __syncthreads();
+      }
     }
   }
   if (cv)
