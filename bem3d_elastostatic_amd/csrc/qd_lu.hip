@@ -327,30 +327,36 @@ __global__ __launch_bounds__(256) void k_trsm_small(T* __restrict__ W, long long
 // L unit lower at W[r0:r0+m, r0:r0+m] (m a multiple of 32) whose 32 x 32
 // diagonal blocks are the tournament leaves, with their inverses in linv
 // (block at rows r0+32b: linv + (r0/32 + b) * 1024, row-major).  One workgroup
-// per strip of kSW columns, held in LDS (m x kSW floats).  Block by block, the
-// strip's 32 rows are solved with the block's inverse plus one correction
-// (x += inv (b - L x), which brings the explicit inverse to the accuracy of
-// substitution), and the rows below subtract L[rows, block] times them.  Those
-// rows go in chunks of kStripChunk whose L segments are staged through LDS by
-// coalesced loads, the next chunk's loads in flight while one is used.
+// per strip of kSW columns (8 or 32), held in LDS (m x kSW floats).  Block by
+// block, the strip's 32 rows are solved with the block's inverse plus one
+// correction (x += inv (b - L x), which brings the explicit inverse to the
+// accuracy of substitution), and the rows below subtract L[rows, block] times
+// them.  Those rows go in chunks of kStripChunk whose L segments are staged
+// through LDS by coalesced loads, the next chunk's loads in flight while one
+// is used.  All products are MFMA f32 16x16x4 (a 16-column tile; the narrow
+// strips use half of it).
 constexpr int kStripChunk = 128;
 
 template <int kSW>
 __global__ __launch_bounds__(256) void k_trsm_strip(float* __restrict__ W, long long ld, long long r0, int m,
                                                     long long c0, long long c1, const float* __restrict__ linv) {
+  static_assert(kSW == 8 || kSW == 32, "strip widths");
   extern __shared__ float xs[];  // m x kSW
-  // rows of 32 floats, 16-byte aligned: the triangular and chunk sums read
-  // them as float4 (all lanes of a half-wave read the same row)
+  // rows of 32 floats whose 16-byte pieces are XOR-swizzled by row (piece p
+  // of row r at p ^ (r & 7)): an MFMA operand read takes two pieces of 16
+  // rows at once, and the swizzle spreads those over the banks
   __shared__ __attribute__((aligned(16))) float sinv[32][32], sl[32][32];
   __shared__ float sb[32][kSW];
   __shared__ __attribute__((aligned(16))) float lbuf[2][kStripChunk][32];
-  constexpr int kSlots = 256 / kSW, kPer = 32 * kSW / 256;
-  constexpr int kRowsPer = kStripChunk / kSlots;  // rows of a chunk per thread
+  constexpr int kSlots = 256 / kSW;
+  constexpr int NT = (kSW + 15) / 16;  // MFMA column tiles
   const int tid = threadIdx.x, c = tid % kSW, slot = tid / kSW;
+  const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lg = lane >> 4;
   const long long col = c0 + (long long)blockIdx.x * kSW + c;
   const bool cv = col < c1;
   for (int r = slot; r < m; r += kSlots) xs[r * kSW + c] = cv ? W[(r0 + r) * ld + col] : 0.0f;
   const float* inv0 = linv + (r0 / 32) * 1024;
+  auto swz = [](int r) { return r & 7; };
   // a chunk's L segments: 128 rows x 32 floats, four float4 per thread
   auto load_chunk = [&](long long d0, int first, float4 (&v)[4]) {
 #pragma unroll
@@ -360,17 +366,28 @@ __global__ __launch_bounds__(256) void k_trsm_strip(float* __restrict__ W, long 
       v[u] = *reinterpret_cast<const float4*>(W + (r0 + row) * ld + d0 + 4 * k4);
     }
   };
-  // the wide strips read chunk rows as MFMA operands, 16 rows per read at the
-  // same column: their 16-byte pieces are XOR-swizzled by row so that those
-  // reads spread over the banks
-  auto lswz = [](int rr) { return (kSW == 32) ? (rr & 7) : 0; };
   auto stage_chunk = [&](int buf, const float4 (&v)[4]) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int e = tid + 256 * u, rr = e >> 3, k4 = e & 7;
       const float4 t = v[u];
-      *reinterpret_cast<float4*>(&lbuf[buf][rr][4 * (k4 ^ lswz(rr))]) = t;
+      *reinterpret_cast<float4*>(&lbuf[buf][rr][4 * (k4 ^ swz(rr))]) = t;
     }
+  };
+  // the k's of an MFMA step are permuted (lane group g supplies k = 8 g + t
+  // at step t), so a lane's A values for the 32 k's are pieces 2g and 2g+1
+  // of its row
+  auto a_row = [&](const float* rowp, int r, float (&av)[8]) {
+    const float4 a0 = *reinterpret_cast<const float4*>(rowp + 4 * ((2 * lg) ^ swz(r)));
+    const float4 a1 = *reinterpret_cast<const float4*>(rowp + 4 * ((2 * lg + 1) ^ swz(r)));
+    av[0] = a0.x;
+    av[1] = a0.y;
+    av[2] = a0.z;
+    av[3] = a0.w;
+    av[4] = a1.x;
+    av[5] = a1.y;
+    av[6] = a1.z;
+    av[7] = a1.w;
   };
   // a diagonal block's inverse and L, four elements of each per thread
   float pinv[4], pl[4];
@@ -383,6 +400,23 @@ __global__ __launch_bounds__(256) void k_trsm_strip(float* __restrict__ W, long 
       pl[u] = (kk < rr) ? W[(d0 + rr) * ld + d0 + kk] : (kk == rr ? 1.0f : 0.0f);
     }
   };
+  // diagonal-block products: wave w forms the output tile at rows 16 (w % 2),
+  // columns 16 (w / 2) (the narrow strips have one column tile, so waves 2
+  // and 3 idle)
+  const int ti = wave & 1, tj = wave >> 1;
+  const bool dwave = tj < NT;
+  const int arow = 16 * ti + lr, bcol = 16 * tj + lr;
+  const bool bvalid = bcol < kSW;
+  auto mfma_blk = [&](const float (*mat)[32], const float* bsrc) {
+    float av[8];
+    a_row(mat[arow], arow, av);
+    f4 acc = (f4){0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[t], bvalid ? bsrc[(8 * lg + t) * kSW + bcol] : 0.0f, acc, 0, 0,
+                                                 0);
+    return acc;
+  };
   load_diag(0);
   for (int i = 0; i < m / 32; ++i) {
     const long long d0 = r0 + 32 * i;  // the diagonal block's first row and column
@@ -392,175 +426,83 @@ __global__ __launch_bounds__(256) void k_trsm_strip(float* __restrict__ W, long 
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int e = tid + 256 * u, rr = e >> 5, kk = e & 31;
-      const int ks = (((kk >> 2) ^ lswz(rr)) << 2) | (kk & 3);
+      const int ks = (((kk >> 2) ^ swz(rr)) << 2) | (kk & 3);
       sinv[rr][ks] = pinv[u];
       sl[rr][ks] = pl[u];
     }
     if (i + 1 < m / 32) load_diag(i + 1);  // in flight until the next block
     __syncthreads();
-    if constexpr (kSW == 32) {
-      // x = inv b, then x += inv (b - L x), as three MFMA f32 16x16x4
-      // products on the 32 x 32 block: wave w forms the output tile at rows
-      // 16 (w >> 1), columns 16 (w & 1).  The triangular factors hold zeros
-      // above the diagonal, so the products are dense.  The k's are permuted
-      // as in the chunk update: A rows (inverse or L) are read as two
-      // swizzled 16-byte pieces, B (32 x 32, row-major) element by element.
-      const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lg = lane >> 4;
-      const int ti = wave >> 1, tj = wave & 1, arow = 16 * ti + lr;
-      auto mfma_blk = [&](const float (*mat)[32], const float* bsrc) {
-        const float4 a0 = *reinterpret_cast<const float4*>(&mat[arow][4 * ((2 * lg) ^ lswz(arow))]);
-        const float4 a1 = *reinterpret_cast<const float4*>(&mat[arow][4 * ((2 * lg + 1) ^ lswz(arow))]);
-        const float av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
-        f4 acc = (f4){0.0f, 0.0f, 0.0f, 0.0f};
+    // x = inv b, then x += inv (b - L x): three products on the 32 x kSW
+    // block.  The triangular factors hold zeros above the diagonal, so the
+    // products are dense.
+    float* xb = xs + (32 * i) * kSW;  // the block's rows of the strip
+    f4 xv = (f4){0.0f, 0.0f, 0.0f, 0.0f};
+    float bown[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (dwave) {
+      if (bvalid)
 #pragma unroll
-        for (int t = 0; t < 8; ++t)
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[t], bsrc[(8 * lg + t) * 32 + 16 * tj + lr], acc, 0, 0, 0);
-        return acc;
-      };
-      float* xb = xs + (32 * i) * kSW;  // the block's rows of the strip
-      float bown[4];
+        for (int q = 0; q < 4; ++q) bown[q] = xb[(16 * ti + 4 * lg + q) * kSW + bcol];
+      xv = mfma_blk(sinv, xb);
+      if (bvalid)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) bown[q] = xb[(16 * ti + 4 * lg + q) * 32 + 16 * tj + lr];
-      const f4 xv = mfma_blk(sinv, xb);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) sb[16 * ti + 4 * lg + q][16 * tj + lr] = xv[q];
-      __syncthreads();
-      const f4 lx = mfma_blk(sl, &sb[0][0]);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) xb[(16 * ti + 4 * lg + q) * 32 + 16 * tj + lr] = bown[q] - lx[q];  // residual
-      __syncthreads();
-      const f4 dx = mfma_blk(sinv, xb);
-      __syncthreads();
-#pragma unroll
-      for (int q = 0; q < 4; ++q) xb[(16 * ti + 4 * lg + q) * 32 + 16 * tj + lr] = xv[q] + dx[q];
-      __syncthreads();
-    } else {
-      // x = inv b, then x += inv (b - L x).  A thread's kPer elements share
-      // their column (cc = tid % kSW) and sit in rows k_q = tid / kSW + q 256 /
-      // kSW, so one pass over j serves all of them: the vector elements are
-      // read once per j, the matrix rows four entries at a time, and the kPer
-      // sums are independent chains.
-      auto lower_dot = [&](const float (*mat)[32], const float* vec, int stride, float (&acc)[kPer]) {
-        const int k0 = tid / kSW;
-        const int kmax = k0 + (256 / kSW) * (kPer - 1);
-  #pragma unroll
-        for (int q = 0; q < kPer; ++q) acc[q] = 0.0f;
-  #pragma unroll 2
-        for (int j4 = 0; j4 <= kmax / 4; ++j4) {
-          const float v0 = vec[(4 * j4) * stride], v1 = vec[(4 * j4 + 1) * stride];
-          const float v2 = vec[(4 * j4 + 2) * stride], v3 = vec[(4 * j4 + 3) * stride];
-  #pragma unroll
-          for (int q = 0; q < kPer; ++q) {
-            const int k = k0 + (256 / kSW) * q;
-            const float4 m4 = *reinterpret_cast<const float4*>(&mat[k][4 * j4]);
-            acc[q] += (4 * j4 <= k) ? m4.x * v0 : 0.0f;
-            acc[q] += (4 * j4 + 1 <= k) ? m4.y * v1 : 0.0f;
-            acc[q] += (4 * j4 + 2 <= k) ? m4.z * v2 : 0.0f;
-            acc[q] += (4 * j4 + 3 <= k) ? m4.w * v3 : 0.0f;
-          }
-        }
-      };
-      const int cc = tid % kSW;
-      float bv[kPer], xv[kPer], dv[kPer];
-  #pragma unroll
-      for (int q = 0; q < kPer; ++q) {
-        const int k = (tid + 256 * q) / kSW;
-        bv[q] = xs[(32 * i + k) * kSW + cc];
-      }
-      lower_dot(sinv, xs + (32 * i) * kSW + cc, kSW, xv);
-  #pragma unroll
-      for (int q = 0; q < kPer; ++q) {
-        const int k = (tid + 256 * q) / kSW;
-        sb[k][cc] = xv[q];
-      }
-      __syncthreads();
-      lower_dot(sl, &sb[0][cc], kSW, dv);
-  #pragma unroll
-      for (int q = 0; q < kPer; ++q) bv[q] -= dv[q];  // the residuals
-      __syncthreads();
-  #pragma unroll
-      for (int q = 0; q < kPer; ++q) {
-        const int k = (tid + 256 * q) / kSW;
-        xs[(32 * i + k) * kSW + cc] = bv[q];
-      }
-      __syncthreads();
-      lower_dot(sinv, xs + (32 * i) * kSW + cc, kSW, dv);
-  #pragma unroll
-      for (int q = 0; q < kPer; ++q) bv[q] = xv[q] + dv[q];
-      __syncthreads();
-  #pragma unroll
-      for (int q = 0; q < kPer; ++q) {
-        const int k = (tid + 256 * q) / kSW;
-        xs[(32 * i + k) * kSW + cc] = bv[q];
-      }
-      __syncthreads();
+        for (int q = 0; q < 4; ++q) sb[16 * ti + 4 * lg + q][bcol] = xv[q];
     }
-    // the rows below, chunk by chunk: X_r -= L[r, block i] X_i
+    __syncthreads();
+    if (dwave) {
+      const f4 lx = mfma_blk(sl, &sb[0][0]);
+      if (bvalid)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) xb[(16 * ti + 4 * lg + q) * kSW + bcol] = bown[q] - lx[q];  // residual
+    }
+    __syncthreads();
+    f4 dx = (f4){0.0f, 0.0f, 0.0f, 0.0f};
+    if (dwave) dx = mfma_blk(sinv, xb);
+    __syncthreads();
+    if (dwave && bvalid)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) xb[(16 * ti + 4 * lg + q) * kSW + bcol] = xv[q] + dx[q];
+    __syncthreads();
+    // the rows below, chunk by chunk: X_r -= L[r, block i] X_i.  Wave w
+    // updates chunk rows [32 w, 32 w + 32) x the strip's columns; its X_i
+    // values are loaded once per block.
     if (nch > 0) stage_chunk(0, pf);
-    if constexpr (kSW == 32) {
-      // MFMA f32 16x16x4: wave w updates chunk rows [32 w, 32 w + 32) x the
-      // 32 columns.  The k's of a step are permuted (lane group g supplies
-      // k = 8 g + t at step t), so a lane's L values are two 16-byte pieces
-      // of its row and its X_i values are loaded once per block.
-      const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lg = lane >> 4;
-      float xbf[2][8];
+    float xbf[NT][8];
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < NT; ++j)
 #pragma unroll
-        for (int t = 0; t < 8; ++t) xbf[j][t] = xs[(32 * i + 8 * lg + t) * kSW + 16 * j + lr];
-      __syncthreads();
-      for (int ch = 0; ch < nch; ++ch) {
-        const int first = below + ch * kStripChunk;
-        if (ch + 1 < nch) load_chunk(d0, first + kStripChunk, pf);
-        const int buf = ch & 1;
-        f4 acc[2][2];
+      for (int t = 0; t < 8; ++t)
+        xbf[j][t] = (16 * j + lr < kSW) ? xb[(8 * lg + t) * kSW + 16 * j + lr] : 0.0f;
+    __syncthreads();
+    for (int ch = 0; ch < nch; ++ch) {
+      const int first = below + ch * kStripChunk;
+      if (ch + 1 < nch) load_chunk(d0, first + kStripChunk, pf);
+      const int buf = ch & 1;
+      f4 acc[2][NT];
 #pragma unroll
-        for (int i2 = 0; i2 < 2; ++i2) {
-          const int rr = 32 * wave + 16 * i2 + lr;
-          const float* rowp = &lbuf[buf][rr][0];
-          const float4 a0 = *reinterpret_cast<const float4*>(rowp + 4 * ((2 * lg) ^ lswz(rr)));
-          const float4 a1 = *reinterpret_cast<const float4*>(rowp + 4 * ((2 * lg + 1) ^ lswz(rr)));
-          const float av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+      for (int i2 = 0; i2 < 2; ++i2) {
+        const int rr = 32 * wave + 16 * i2 + lr;
+        float av[8];
+        a_row(&lbuf[buf][rr][0], rr, av);
 #pragma unroll
-          for (int j = 0; j < 2; ++j) {
-            acc[i2][j] = (f4){0.0f, 0.0f, 0.0f, 0.0f};
+        for (int j = 0; j < NT; ++j) {
+          acc[i2][j] = (f4){0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-            for (int t = 0; t < 8; ++t)
-              acc[i2][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[t], xbf[j][t], acc[i2][j], 0, 0, 0);
-          }
+          for (int t = 0; t < 8; ++t)
+            acc[i2][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[t], xbf[j][t], acc[i2][j], 0, 0, 0);
         }
+      }
 #pragma unroll
-        for (int i2 = 0; i2 < 2; ++i2)
+      for (int i2 = 0; i2 < 2; ++i2)
 #pragma unroll
-          for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < NT; ++j)
+          if (16 * j + lr < kSW)
 #pragma unroll
             for (int q4 = 0; q4 < 4; ++q4) {
               const int r = first + 32 * wave + 16 * i2 + 4 * lg + q4;
               if (r < m) xs[r * kSW + 16 * j + lr] -= acc[i2][j][q4];
             }
-        if (ch + 1 < nch) stage_chunk(buf ^ 1, pf);
-        __syncthreads();
-      }
-    } else {
-      float xi[32];
-#pragma unroll
-      for (int k = 0; k < 32; ++k) xi[k] = xs[(32 * i + k) * kSW + c];
+      if (ch + 1 < nch) stage_chunk(buf ^ 1, pf);
       __syncthreads();
-      for (int ch = 0; ch < nch; ++ch) {
-        const int first = below + ch * kStripChunk;
-        if (ch + 1 < nch) load_chunk(d0, first + kStripChunk, pf);
-        const int buf = ch & 1;
-#pragma unroll
-        for (int t = 0; t < kRowsPer; ++t) {
-          const int rr = slot + kSlots * t, r = first + rr;
-          float sum = 0.0f;
-#pragma unroll
-          for (int k = 0; k < 32; ++k) sum += lbuf[buf][rr][k] * xi[k];
-          if (r < m) xs[r * kSW + c] -= sum;
-        }
-        if (ch + 1 < nch) stage_chunk(buf ^ 1, pf);
-        __syncthreads();
-      }
     }
   }
   if (cv)
