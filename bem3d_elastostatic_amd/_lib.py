@@ -21,11 +21,11 @@ PROF_KINDS = {"lu_trailing_gemm": 0, "lu_panel": 1, "assembly_h": 2, "assembly_g
 
 class SolveReport(C.Structure):
     _fields_ = [("precision", C.c_int), ("info", C.c_int), ("iterations", C.c_int),
-                ("converged", C.c_int), ("gmres", C.c_int), ("pad_", C.c_int),
+                ("converged", C.c_int), ("gmres", C.c_int), ("pivoting", C.c_int),
                 ("residual", C.c_double), ("t_factor_ms", C.c_double), ("t_solve_ms", C.c_double)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for (k, _) in self._fields_ if k != "pad_"}
+        return {k: getattr(self, k) for (k, _) in self._fields_}
 
 
 class QuadesthError(RuntimeError):

@@ -30,7 +30,9 @@
 #include <algorithm>
 #include <cstdint>
 #include <chrono>
+#include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <cmath>
 #include <type_traits>
 #include <utility>
@@ -75,6 +77,8 @@ struct qd_lu_state {
   qd::DevBuf<int> lb_flags;      // error word, then per-block publication flags of the look-back solves
   int lb_epoch = 0;              // value the flags take in the latest look-back solve
   qd::DevBuf<float> ts_linv;     // inverses of the leaves' unit lower blocks (32 x 32 each)
+  qd::DevBuf<unsigned> growth;   // diagonal-pivot factorisations: largest |l| (float bits)
+  bool diag_pivots = false;      // the held factors took their diagonal pivots
   qd::DevBuf<float> inv32;       // inverted diagonal blocks of L and U
   qd::DevBuf<double> inv64;
   double anorm = -1.0;           // ||A||_inf of the FP64 matrix
@@ -1246,6 +1250,7 @@ struct LuCtx {
   hipStream_t st = nullptr;                   // stream of every launch of this context
   qd_lu_state::F16Scratch* sc = nullptr;      // FP16 operand copies of this context
   qd_lu_state::F16Scratch* sc_solve = nullptr;  // those of trsm_lower's inner updates (nullptr: sc)
+  bool diag = false;  // with calu: the leaves take their diagonal pivots instead (checked by growth)
 };
 
 // The FP16 update C -= (A16 B16^T) * colscale on stream st; max_wgs > 0
@@ -1413,6 +1418,18 @@ static int tslu_leaf(LuCtx<float>& c, long long c0, int w, long long p0, long lo
     return set_error(c.h, QD_ERR_STATE, "tournament buffers sized for fewer rows than %lld", m);
   using Nd = TsNode<kTsTreeR, kTsTreeV>;
   const unsigned nb = (unsigned)((m + Nd::rows - 1) / Nd::rows);
+  if (c.diag) {
+    // diagonal pivots: no interchanges, so the finish permutes nothing
+    hipLaunchKernelGGL(k_leaf_diag, dim3(1), dim3(64), 0, c.st, c.W, c.ld, c.N, c0, w, s->ts_lu.p, s->ts_top.p,
+                       s->ts_meta.p, s->ipiv_dev.p, s->growth.p);
+    const long long below = std::max<long long>(m - w, 1);
+    const int row_blocks = (int)((below + 255) / 256);
+    hipLaunchKernelGGL(k_tslu_finish, dim3((unsigned)(row_blocks + 1)), dim3(256), 0, c.st, c.W, c.ld, c.N, c0, w,
+                       p0, p0, (const float*)s->ts_lu.p, (const float*)s->ts_top.p, (const int*)s->ts_meta.p,
+                       row_blocks, s->ts_linv.p + (c0 / kTsW) * kTsW * kTsW, s->growth.p);
+    QD_HIP(c.h, hipGetLastError());
+    return QD_OK;
+  }
   hipLaunchKernelGGL((k_tslu_tree<kTsTreeR, kTsTreeV>), dim3(nb), dim3(Nd::threads), 0, c.st, c.W, c.ld, c.N, c0, w, s->ts_cand.p, s->ts_cnt.p,
                      s->ts_lu.p, s->ts_top.p, s->ts_meta.p, s->ipiv_dev.p);
   const long long below = std::max<long long>(m - w, 1);
@@ -1420,7 +1437,7 @@ static int tslu_leaf(LuCtx<float>& c, long long c0, int w, long long p0, long lo
   const int col_blocks = (int)((p1 - p0 + 255) / 256);
   hipLaunchKernelGGL(k_tslu_finish, dim3((unsigned)(row_blocks + col_blocks + 1)), dim3(256), 0, c.st, c.W, c.ld, c.N,
                      c0, w, p0, p1, (const float*)s->ts_lu.p, (const float*)s->ts_top.p, (const int*)s->ts_meta.p,
-                     row_blocks, s->ts_linv.p + (c0 / kTsW) * kTsW * kTsW);
+                     row_blocks, s->ts_linv.p + (c0 / kTsW) * kTsW * kTsW, (unsigned*)nullptr);
   QD_HIP(c.h, hipGetLastError());
   return QD_OK;
 }
@@ -1465,6 +1482,19 @@ static int update_cols(LuCtx<T>& c, long long j, int jb, long long c0, long long
   const long long r0 = j + jb;
   return gemm_sub(c, c.N - r0, c1 - c0, jb, c.W + r0 * c.ld + j, c.W + j * c.ld + c0, c.W + r0 * c.ld + c0, true,
                   true, reuse_a16, max_wgs);
+}
+
+// Largest multiplier |l| with which a diagonal-pivot factorisation is kept.
+// At 1 every diagonal pivot is the largest |a| of its column below it, i.e.
+// the pivot partial pivoting (LAPACK's first largest) would have taken, so a
+// kept factorisation has partial pivoting's pivot sequence under this
+// factorisation's own rounding.
+constexpr float kDiagMaxMult = 1.0f;
+
+// QUADESTH_PIVOT=tournament skips the diagonal-pivot attempt.
+static bool diag_pivots_allowed() {
+  const char* e = std::getenv("QUADESTH_PIVOT");
+  return !(e && std::strcmp(e, "tournament") == 0);
 }
 
 // Workgroup cap of the trailing update that runs beside a panel
@@ -1515,6 +1545,8 @@ static int factor(LuCtx<T>& c, int NB) {
                                     hipFuncAttributeMaxDynamicSharedMemorySize,
                                     kStripMaxRows * 8 * (int)sizeof(float)));
     if (fresh) QD_HIP(c.h, hipMemsetAsync(s->ts_cnt.p, 0, s->ts_cnt.count * sizeof(unsigned), c.st));
+    QD_ALLOC(c.h, s->growth, 1);
+    QD_HIP(c.h, hipMemsetAsync(s->growth.p, 0, sizeof(unsigned), c.st));
   }
   if (c.f16) {
     // sized up front, so that nothing is reallocated while two streams run
@@ -1556,13 +1588,15 @@ static int factor_steps(LuCtx<T>& c, int NB, bool look) {
     const int jb = (int)std::min<long long>(NB, N - j);
     const long long nj = j + jb;
     const int njb = nj < N ? (int)std::min<long long>(NB, N - nj) : 0;
-    // interchanges on the columns left and right of the panel
-    if (j > 0)
+    // interchanges on the columns left and right of the panel (none with
+    // diagonal pivots)
+    if (j > 0 && !c.diag)
       hipLaunchKernelGGL(k_laswp<T>, dim3((unsigned)((j + 255) / 256)), dim3(256), 0, c.st, c.W, c.ld, 0LL,
                          j, (int)j, (int)nj, s->ipiv_dev.p);
     if (nj < N) {
-      hipLaunchKernelGGL(k_laswp<T>, dim3((unsigned)((N - nj + 255) / 256)), dim3(256), 0, c.st, c.W, c.ld, nj, N,
-                         (int)j, (int)nj, s->ipiv_dev.p);
+      if (!c.diag)
+        hipLaunchKernelGGL(k_laswp<T>, dim3((unsigned)((N - nj + 255) / 256)), dim3(256), 0, c.st, c.W, c.ld, nj, N,
+                           (int)j, (int)nj, s->ipiv_dev.p);
       QD_HIP(c.h, hipGetLastError());
       if (look && nj + njb < N) {
         // the next panel's columns first: U12 and its update, then the panel
@@ -2219,10 +2253,28 @@ static int ensure_factors(qd_context* h, int precision, qd_solve_report* rep) {
   } else {
     s->W64.release();
     QD_ALLOC(h, s->W32, (size_t)ld * (size_t)N);
-    hipLaunchKernelGGL(k_transpose_in<float>, tg, dim3(256), 0, h->stream, N, h->A.p, h->lda, s->W32.p, ld);
-    QD_HIP(h, hipGetLastError());
-    LuCtx<float> c{h, s, s->W32.p, ld, N, precision == QD_FP16_IR, true, h->stream, &s->sc_main};
-    rc = factor(c, NB);
+    // Diagonal pivots first: the factors only precondition the FP64
+    // refinement, and the leaves then skip the pivot search.  The largest
+    // multiplier decides afterwards; above kDiagMaxMult (or on a zero pivot)
+    // the factorisation is redone with tournament pivoting.
+    bool diag = diag_pivots_allowed();
+    for (int attempt = 0; attempt < 2; ++attempt) {
+      hipLaunchKernelGGL(k_transpose_in<float>, tg, dim3(256), 0, h->stream, N, h->A.p, h->lda, s->W32.p, ld);
+      QD_HIP(h, hipGetLastError());
+      LuCtx<float> c{h, s, s->W32.p, ld, N, precision == QD_FP16_IR, true, h->stream, &s->sc_main};
+      c.diag = diag;
+      rc = factor(c, NB);
+      if (rc || !diag) break;
+      unsigned gb = 0;
+      QD_HIP(h, hipMemcpyAsync(&gb, s->growth.p, sizeof(gb), hipMemcpyDeviceToHost, h->stream));
+      QD_HIP(h, hipStreamSynchronize(h->stream));
+      float g;
+      std::memcpy(&g, &gb, sizeof(g));
+      if (std::getenv("QUADESTH_DEBUG_PIVOT")) std::fprintf(stderr, "[quadesth] diagonal pivots: max |l| = %g\n", g);
+      if (g <= kDiagMaxMult) break;  // NaN and +inf fail the test too
+      diag = false;
+    }
+    s->diag_pivots = diag;
   }
   if (rc) return rc;
   {
@@ -2345,6 +2397,7 @@ int lu_solve(qd_context* h, int precision, int nrhs, double* b, long long ldb, d
       worst.iterations = std::max(worst.iterations, r.iterations);
       worst.converged = (v0 == 0) ? r.converged : (worst.converged && r.converged);
       worst.gmres |= r.gmres;
+      worst.pivoting |= r.pivoting;
       worst.residual = std::max(worst.residual, r.residual);
       worst.t_factor_ms += r.t_factor_ms;
       worst.t_solve_ms += r.t_solve_ms;
@@ -2360,6 +2413,7 @@ int lu_solve(qd_context* h, int precision, int nrhs, double* b, long long ldb, d
   if (rc) return rc;
   qd_lu_state* s = h->lu;
   R->info = s->info;
+  R->pivoting = (precision != QD_FP64 && s->diag_pivots) ? 1 : 0;
   if (s->info > 0) return s->info;
   const long long N = s->N;
   const double t0 = now_ms();

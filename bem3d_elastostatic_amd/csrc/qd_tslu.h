@@ -367,6 +367,73 @@ __global__ __launch_bounds__(64 * V) void k_tslu_tree(const float* __restrict__ 
   }
 }
 
+// Largest magnitude of a float as its bit pattern (NaN above +inf), folded
+// into *growth; called by whole waves.
+__device__ __forceinline__ void ts_growth_max(float v, unsigned* growth) {
+  unsigned b = __float_as_uint(fabsf(v));
+  b = wave_max_u32(b);
+  if ((threadIdx.x & 63) == 0) atomicMax(growth, b);
+}
+
+// A leaf with diagonal pivots (the top w rows in their order), in the tree's
+// output format for k_tslu_finish, for factorisations that try diagonal
+// pivoting first.  One wave; lane k holds row c0+k.  The largest multiplier
+// |l| goes to *growth (a zero pivot counts as +inf), so that the host can
+// reject the factorisation.
+__global__ __launch_bounds__(64) void k_leaf_diag(const float* __restrict__ W, long long ld, long long N,
+                                                  long long c0, int w, float* __restrict__ lu_out,
+                                                  float* __restrict__ top_out, int* __restrict__ meta,
+                                                  int* __restrict__ ipiv, unsigned* __restrict__ growth) {
+  __shared__ __attribute__((aligned(16))) float srow[kTsW];
+  const int lane = threadIdx.x;
+  float a[kTsW];
+  ts_load_row(W, ld, (lane < w) ? c0 + lane : c0, c0, w, a);
+  if (lane >= w)
+#pragma unroll
+    for (int j = 0; j < kTsW; ++j) a[j] = 0.0f;
+  if (lane < kTsW)
+#pragma unroll
+    for (int j4 = 0; j4 < kTsW / 4; ++j4)
+      *reinterpret_cast<float4*>(top_out + lane * kTsW + 4 * j4) =
+          make_float4(a[4 * j4], a[4 * j4 + 1], a[4 * j4 + 2], a[4 * j4 + 3]);
+  float gmax = 0.0f;
+#pragma unroll
+  for (int k = 0; k < kTsW; ++k) {
+    if (k < w) {
+      if (lane == k) {
+#pragma unroll
+        for (int j4 = 0; j4 < kTsW / 4; ++j4) {
+          const float4 v = make_float4(a[4 * j4], a[4 * j4 + 1], a[4 * j4 + 2], a[4 * j4 + 3]);
+          *reinterpret_cast<float4*>(srow + 4 * j4) = v;
+          *reinterpret_cast<float4*>(lu_out + k * kTsW + 4 * j4) = v;
+        }
+      }
+      __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the row is in LDS
+      __builtin_amdgcn_wave_barrier();
+      const float piv = srow[k];
+      if (piv == 0.0f) gmax = __int_as_float(0x7f800000);
+      const float rp = (piv != 0.0f) ? 1.0f / piv : 0.0f;
+      if (lane > k && lane < w) {
+        const float l = a[k] * rp;
+        a[k] = l;
+        gmax = fmaxf(gmax, fabsf(l));
+#pragma unroll
+        for (int j = k + 1; j < kTsW; ++j) a[j] -= l * srow[j];
+      }
+      // every lane has read the row before the next step writes it
+      __builtin_amdgcn_s_waitcnt(0xc07f);
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  ts_growth_max(gmax, growth);
+  if (lane < kTsW) {
+    meta[lane] = (int)c0 + lane;
+    meta[128 + lane] = (lane < w) ? (int)c0 + lane : -1;
+    if (lane < w) ipiv[c0 + lane] = (int)c0 + lane;
+  }
+  if (lane == 0) meta[96] = 0;
+}
+
 // L21 = A21 U11^-1 (one row per thread, U11 read from LDS by broadcast),
 // against the root's winners' LU, which workgroup 0 also writes into the top
 // rows; then, in workgroups row_blocks.., the permutation on the panel's
@@ -403,7 +470,8 @@ __global__ __launch_bounds__(256) void k_tslu_finish(float* __restrict__ W, long
                                                      int w, long long p0, long long p1,
                                                      const float* __restrict__ lu_in,
                                                      const float* __restrict__ top_in, const int* __restrict__ meta,
-                                                     int row_blocks, float* __restrict__ linv_out) {
+                                                     int row_blocks, float* __restrict__ linv_out,
+                                                     unsigned* __restrict__ growth) {
   __shared__ __attribute__((aligned(16))) float sU[kTsW][kTsW];
   __builtin_amdgcn_s_setprio(3);
   __shared__ float sR[kTsW];
@@ -486,6 +554,28 @@ __global__ __launch_bounds__(256) void k_tslu_finish(float* __restrict__ W, long
   }
   __syncthreads();
   const long long r = c0 + w + (long long)blockIdx.x * 256 + tid;
+  if (growth) {
+    // diagonal pivots: no row is displaced; fold this row's largest |l| into
+    // *growth (whole waves; rows past N count as 0)
+    float x[kTsW];
+    float g = 0.0f;
+    if (r < N) {
+      ts_load_row(W, ld, r, c0, w, x);
+      tf_solve(std::make_integer_sequence<int, kTsW>{}, x, w, sU, sR);
+#pragma unroll
+      for (int j = 0; j < kTsW; ++j) g = fmaxf(g, fabsf(x[j]));
+      float* dst = W + r * ld + c0;
+      if (w == kTsW) {
+#pragma unroll
+        for (int j4 = 0; j4 < kTsW / 4; ++j4)
+          *reinterpret_cast<float4*>(dst + 4 * j4) = make_float4(x[4 * j4], x[4 * j4 + 1], x[4 * j4 + 2], x[4 * j4 + 3]);
+      } else {
+        for (int j = 0; j < w; ++j) dst[j] = x[j];
+      }
+    }
+    ts_growth_max(g, growth);
+    return;
+  }
   if (r >= N) return;
   int src = -1;
   const int nd = snd;

@@ -34,7 +34,7 @@ module quadesth_capi
   integer(c_int), parameter :: QD_SOMIGLIANA_REFERENCE = 0, QD_SOMIGLIANA_CORRECTED = 1
 
   type, bind(C) :: qd_solve_report
-    integer(c_int) :: precision = 0, info = 0, iterations = 0, converged = 0, gmres = 0, pad_ = 0
+    integer(c_int) :: precision = 0, info = 0, iterations = 0, converged = 0, gmres = 0, pivoting = 0
     real(c_double) :: residual = 0.0_c_double, t_factor_ms = 0.0_c_double, t_solve_ms = 0.0_c_double
   end type qd_solve_report
 

@@ -61,7 +61,10 @@ typedef struct {
   int iterations;       /* refinement sweeps taken (0 for QD_FP64) */
   int converged;        /* 1 when the residual met tol (or QD_FP64) */
   int gmres;            /* 1 when the GMRES fallback was used */
-  int pad_;
+  int pivoting;         /* mixed precisions: 1 when the factors kept their
+                           diagonal pivots (every multiplier |l| <= 1, so
+                           partial pivoting's choice), 0 when they use
+                           tournament pivoting */
   double residual;      /* max over rhs of ||b-Ax||_inf/(||A||_inf||x||_inf+||b||_inf) */
   double t_factor_ms;   /* device time of the factorisation */
   double t_solve_ms;    /* device time of substitutions + refinement */

@@ -1,0 +1,61 @@
+"""Pivoting of the mixed-precision factorisations.
+
+They first keep their diagonal pivots and accept the factors only when every
+multiplier satisfies |l| <= 1 (then each pivot is the one partial pivoting
+would choose); otherwise they refactor with tournament pivoting.  The BEM
+systems of the golden cases pass the check.  The same system with its rows
+reversed (and its right-hand side with them) has the same solution but fails
+it, so it exercises the fallback."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import golden, problem_from_golden, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _hip():
+    """The HIP runtime this process already uses (the library is linked to it)."""
+    with open("/proc/self/maps") as f:
+        for line in f:
+            if "libamdhip64" in line:
+                return C.CDLL(line.split()[-1])
+    raise RuntimeError("HIP runtime not loaded")
+
+
+@pytest.mark.parametrize("precision", ["fp16", "fp32"])
+def test_diagonal_pivots_kept(precision):
+    from bem3d_elastostatic_amd.quadesth import Quadesth
+    g = golden("cube8")
+    q = Quadesth(problem_from_golden(g))
+    q.ghmat8()
+    print(f"[pivoting] cube8 {precision}: diagonal", flush=True)
+    X, rep = q.solve_linear_system(precision=precision, tol=1e-14)
+    assert rep["pivoting"] == 1, rep
+    assert rep["converged"] == 1 and rep["residual"] <= 1e-14
+    assert rel_err(X, g["X"]) < 1e-10
+
+
+@pytest.mark.parametrize("precision", ["fp16", "fp32"])
+def test_reversed_rows_fall_back_to_tournament(precision):
+    from bem3d_elastostatic_amd.quadesth import Quadesth
+    g = golden("cube8")
+    q = Quadesth(problem_from_golden(g))
+    q.ghmat8()
+    A, FI = q.system()
+    N = A.shape[0]
+    pa, lda, _ = q.system_device()
+    # column-major with leading dimension lda, rows reversed
+    Ar = np.zeros((N, lda))
+    Ar[:, :N] = A[::-1, :].T
+    hip = _hip()
+    hip.hipMemcpy.restype = C.c_int
+    rc = hip.hipMemcpy(C.c_void_p(pa), Ar.ctypes.data_as(C.c_void_p), C.c_size_t(Ar.nbytes), C.c_int(1))
+    assert rc == 0
+    print(f"[pivoting] cube8 reversed rows {precision}: fallback", flush=True)
+    X, rep = q.solve_linear_system(FI[::-1].copy(), precision=precision, tol=1e-14)
+    assert rep["pivoting"] == 0, rep
+    assert rep["converged"] == 1 and rep["residual"] <= 1e-14
+    assert rel_err(X, g["X"]) < 1e-10
