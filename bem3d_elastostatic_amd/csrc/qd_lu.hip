@@ -64,6 +64,7 @@ struct qd_lu_state {
   F16Scratch sc_solve;  // the inner updates of the main stream's U12 solve, so sc_main keeps L21
   hipStream_t aux = nullptr;     // panel stream of the look-ahead (non-blocking, default priority)
   hipEvent_t ev_cols = nullptr, ev_panel = nullptr;
+  hipEvent_t ev_top = nullptr, ev_u12 = nullptr;  // split steps: next U12 rows updated / solved
   qd::DevBuf<double> vec;        // solve / refinement work vectors
   qd::DevBuf<float> vecf;
   qd::DevBuf<double> gm;         // GMRES basis
@@ -85,6 +86,8 @@ struct qd_lu_state {
   ~qd_lu_state() {
     if (ev_cols) (void)hipEventDestroy(ev_cols);
     if (ev_panel) (void)hipEventDestroy(ev_panel);
+    if (ev_top) (void)hipEventDestroy(ev_top);
+    if (ev_u12) (void)hipEventDestroy(ev_u12);
     if (aux) (void)hipStreamDestroy(aux);
   }
 };
@@ -1497,6 +1500,42 @@ static bool diag_pivots_allowed() {
   return !(e && std::strcmp(e, "tournament") == 0);
 }
 
+// The FP16 trailing update of columns [c0, c1) in two launches: the next
+// panel's rows [j+jb, j+jb+split) first (ev recorded after it), then the rest.
+// L21's FP16 copy is already in c.sc (converted by the update of the next
+// panel's columns on the same stream).
+template <typename T>
+static int update_split(LuCtx<T>& c, long long j, int jb, long long c0, long long c1, long long split,
+                        hipEvent_t ev) {
+  const long long r0 = j + jb, M = c.N - r0, Nn = c1 - c0;
+  const int K = jb, Kp = (K + 31) / 32 * 32;
+  const float* B = reinterpret_cast<const float*>(c.W + j * c.ld + c0);
+  const long long nb = (Nn + 63) / 64;
+  QD_ALLOC(c.h, c.sc->colscale, (size_t)nb);
+  QD_ALLOC(c.h, c.sc->h16b, (size_t)Nn * Kp);
+  const int vb = ((reinterpret_cast<uintptr_t>(B) & 15) == 0 && c.ld % 4 == 0) ? 1 : 0;
+  hipLaunchKernelGGL(k_scale_cvt_b16t, dim3((unsigned)nb), dim3(256), 0, c.st, K, Nn, Kp, B, c.ld, c.sc->colscale.p,
+                     c.sc->h16b.p, vb);
+  float* C = reinterpret_cast<float*>(c.W + r0 * c.ld + c0);
+  const int rows16 = (c.ld % 4 == 0 && (reinterpret_cast<uintptr_t>(C) & 15) == 0) ? 1 : 0;
+  int rc;
+  {
+    ProfScope ps(c.h, QD_PROF_LU_TRAILING_GEMM, 2.0 * (double)split * (double)Nn * K, c.st);
+    if ((rc = launch_f16_update(c.h, true, split, Nn, Kp, c.sc->h16a.p, c.sc->h16b.p, c.sc->colscale.p, C, c.ld,
+                                rows16, 0, c.st)))
+      return rc;
+  }
+  QD_HIP(c.h, hipEventRecord(ev, c.st));
+  {
+    ProfScope ps(c.h, QD_PROF_LU_TRAILING_GEMM, 2.0 * (double)(M - split) * (double)Nn * K, c.st);
+    if ((rc = launch_f16_update(c.h, true, M - split, Nn, Kp, c.sc->h16a.p + (size_t)split * Kp, c.sc->h16b.p,
+                                c.sc->colscale.p, C + split * c.ld, c.ld, rows16, 0, c.st)))
+      return rc;
+  }
+  QD_HIP(c.h, hipGetLastError());
+  return QD_OK;
+}
+
 // Workgroup cap of the trailing update that runs beside a panel
 // (QUADESTH_GEMM_WGS; 0 = one workgroup per tile).  Measured on MI355X at
 // 76,806 DOF, caps of 320-512 did not shorten the factorisation: the panel
@@ -1567,6 +1606,8 @@ static int factor(LuCtx<T>& c, int NB) {
     QD_HIP(c.h, hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking));
     QD_HIP(c.h, hipEventCreateWithFlags(&s->ev_cols, hipEventDisableTiming));
     QD_HIP(c.h, hipEventCreateWithFlags(&s->ev_panel, hipEventDisableTiming));
+    QD_HIP(c.h, hipEventCreateWithFlags(&s->ev_top, hipEventDisableTiming));
+    QD_HIP(c.h, hipEventCreateWithFlags(&s->ev_u12, hipEventDisableTiming));
   }
   return factor_steps(c, NB, look);
 }
@@ -1581,6 +1622,14 @@ static int factor_steps(LuCtx<T>& c, int NB, bool look) {
   cp.sc_solve = nullptr;
   cp.st = s->aux;
   cp.sc = &s->sc_panel;
+  // With diagonal pivots (no interchanges) and FP16 updates, a step's big
+  // update goes in two launches, the next panel's rows first; the aux stream
+  // then solves the next step's U12 (beyond the panel after next) beside the
+  // rest of the update, once the next panel is factored.
+  const bool split = look && c.diag && c.f16;
+  LuCtx<T> ca = cp;  // the next U12 solve on the aux stream
+  ca.sc_solve = &s->sc_solve;
+  bool u12_ready = false;  // this step's U12 beyond the next panel is solved (ev_u12)
   QD_HIP(c.h, hipMemsetAsync(s->piv_state.p, 0, 2 * sizeof(int), c.st));
   int rc;
   if ((rc = panel_step(c, 0, (int)std::min<long long>(NB, N)))) return rc;
@@ -1607,15 +1656,38 @@ static int factor_steps(LuCtx<T>& c, int NB, bool look) {
         // The rest of the step is enqueued before the panel: the host can
         // be held up enqueuing the panel's many small launches, and the
         // main stream would idle meanwhile.
-        if ((rc = trsm_lower(c, j, jb, nj + njb, N))) return rc;
+        if (u12_ready) {
+          QD_HIP(c.h, hipStreamWaitEvent(c.st, s->ev_u12, 0));
+        } else if ((rc = trsm_lower(c, j, jb, nj + njb, N))) {
+          return rc;
+        }
         // L21's FP16 copy from the update above is still in sc_main (the U12
         // solve's inner updates use sc_solve)
-        if ((rc = update_cols(c, j, jb, nj + njb, N, true, lookahead_gemm_wgs()))) return rc;
+        if (split) {
+          if ((rc = update_split(c, j, jb, nj + njb, N, njb, s->ev_top))) return rc;
+        } else if ((rc = update_cols(c, j, jb, nj + njb, N, true, lookahead_gemm_wgs()))) {
+          return rc;
+        }
         QD_HIP(c.h, hipStreamWaitEvent(cp.st, s->ev_cols, 0));
         if ((rc = panel_step(cp, nj, njb))) return rc;
         QD_HIP(c.h, hipEventRecord(s->ev_panel, cp.st));
+        u12_ready = false;
+        if (split) {
+          // the next step's U12 beyond its next panel, when that step takes
+          // this branch too: its rows are final once the first launch above
+          // is done, and its L11 once the panel is
+          const long long nj2 = nj + njb;
+          const int njb2 = nj2 < N ? (int)std::min<long long>(NB, N - nj2) : 0;
+          if (nj2 < N && nj2 + njb2 < N) {
+            QD_HIP(c.h, hipStreamWaitEvent(ca.st, s->ev_top, 0));
+            if ((rc = trsm_lower(ca, nj, njb, nj2 + njb2, N))) return rc;
+            QD_HIP(c.h, hipEventRecord(s->ev_u12, ca.st));
+            u12_ready = true;
+          }
+        }
         QD_HIP(c.h, hipStreamWaitEvent(c.st, s->ev_panel, 0));
       } else {
+        if (u12_ready) return set_error(c.h, QD_ERR_STATE, "factor: U12 solved ahead for a step without look-ahead");
         if ((rc = trsm_lower(c, j, jb, nj, N))) return rc;
         if ((rc = update_cols(c, j, jb, nj, N, false))) return rc;
         if (njb > 0 && (rc = panel_step(c, nj, njb))) return rc;
