@@ -98,7 +98,8 @@ class Pipeline:
         if phases is not None:
             phases.append(dict(assembly=t1 - t0, solve=t2 - t1, reorganize=t3 - t2, internal=t4 - t3,
                                factor_ms=rep.t_factor_ms, refine_ms=rep.t_solve_ms, iterations=rep.iterations,
-                               gmres=rep.gmres, residual=rep.residual, counts=counts.tolist()))
+                               gmres=rep.gmres, residual=rep.residual, pivoting=rep.pivoting,
+                               counts=counts.tolist()))
         return FI, DFI
 
 _HIP = None
@@ -163,13 +164,16 @@ def cpu_baseline(p, FI, DFI, budget_s=25.0):
 HBM_PEAK_GBS = 8000.0
 
 
-def trailing_shapes(N, precision):
+def trailing_shapes(N, precision, diagonal=True):
     """(M, Nn, K) of every outer trailing-update launch of one factorisation:
     the library's panel width (FP16 1024, else 512; QUADESTH_NB overrides) and
-    its look-ahead split (the next panel's columns, then the rest)."""
+    its look-ahead split (the next panel's columns, then the rest).  With
+    diagonal pivots the FP16 update of the rest is two launches, the next
+    panel's rows first."""
     nb = int(os.environ.get("QUADESTH_NB", "1024" if precision == "fp16" else "512"))
     nb = max(64, nb // 32 * 32)
     look = os.environ.get("QUADESTH_LOOKAHEAD", "1")[:1] != "0"
+    split = look and diagonal and precision == "fp16"
     out = []
     for j in range(0, N, nb):
         jb = min(nb, N - j)
@@ -179,7 +183,11 @@ def trailing_shapes(N, precision):
         njb = min(nb, N - nj)
         if look and nj + njb < N:
             out.append((N - nj, njb, jb))
-            out.append((N - nj, N - nj - njb, jb))
+            if split:
+                out.append((njb, N - nj - njb, jb))
+                out.append((N - nj - njb, N - nj - njb, jb))
+            else:
+                out.append((N - nj, N - nj - njb, jb))
         else:
             out.append((N - nj, N - nj, jb))
     return out
@@ -203,12 +211,12 @@ def measured_traffic(N, precision):
     return None
 
 
-def trailing_roofline(N, precision, d):
+def trailing_roofline(N, precision, d, diagonal=True):
     """Roofline of the outer trailing update from its algorithmic work: 2 M Nn K
     flops and, in bytes, the C read and write (8 per FP32 element) plus one
     pass over the operands in the precision the kernel reads them.  The bound
     is whichever floor (peak MFMA rate or 8 TB/s) is higher for these shapes."""
-    shapes = trailing_shapes(N, precision)
+    shapes = trailing_shapes(N, precision, diagonal)
     ob = {"fp16": 2, "fp32": 4, "fp64": 8}[precision]   # operand bytes
     cb = {"fp16": 8, "fp32": 8, "fp64": 16}[precision]  # C read + write bytes per element
     flops = sum(2.0 * m * n * k for m, n, k in shapes)
@@ -361,7 +369,7 @@ def main():
                          "c4: adjoint gradient on the perturbed cube")
     ap.add_argument("--fd-dirs", type=int, default=64, help="c4: directions of the finite-difference check")
     ap.add_argument("--trailing-update-only", action="store_true",
-                    help="only the 149 FP16 trailing-update launches of one 76,806-DOF factorisation "
+                    help="only the FP16 trailing-update launches (223) of one 76,806-DOF factorisation "
                          "(qd_fp16_update on synthetic operands), for hardware counter passes")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -455,7 +463,7 @@ def main():
     dom = max((k for k in prof_out if k != "lu_panel"), key=lambda k: prof_out[k]["ms"])
     d = prof_out[dom]
     if dom == "lu_trailing_gemm":
-        roofline = trailing_roofline(N, args.precision, d)
+        roofline = trailing_roofline(N, args.precision, d, diagonal=bool(phases[-1]["pivoting"]))
     else:
         peak = PEAK_TFLOPS["fp64_vector"]
         achieved = d["tflops"] or 0.0
