@@ -75,7 +75,8 @@ struct qd_lu_state {
   qd::DevBuf<float> ts_lu;       // LU of a leaf's winners block (32 x 32)
   qd::DevBuf<float> ts_top;      // a leaf's original top rows (32 x 32)
   qd::DevBuf<unsigned char> trs_tmp;  // intermediate vector of the block triangular solves
-  qd::DevBuf<int> lb_flags;      // error word, then per-block publication flags of the look-back solves
+  qd::DevBuf<int> lb_flags;      // error word, then per-block and per-part flags of the look-back solves
+  qd::DevBuf<unsigned char> lb_part;  // look-back solves: partial sums of the parts of each block row
   int lb_epoch = 0;              // value the flags take in the latest look-back solve
   qd::DevBuf<float> ts_linv;     // inverses of the leaves' unit lower blocks (32 x 32 each)
   qd::DevBuf<unsigned> growth;   // diagonal-pivot factorisations: largest |l| (float bits)
@@ -1915,42 +1916,69 @@ __global__ __launch_bounds__(256) void k_block_step(const T* __restrict__ W, lon
 }
 
 // A whole triangular solve with one right-hand side in one launch ("decoupled
-// look-back"): workgroup i owns block b (b = i forward, nblk-1-i backward),
-// accumulates L[b, j] x_j over the blocks it depends on as each x_j is
-// published (a per-block flag set to this call's epoch), then forms
-// x_b = inv_b (y_b - sum) and publishes it.  A workgroup waits only on
-// blocks owned by workgroups with smaller indices, which are dispatched
-// before it.  The wait is bounded: after kLbMaxSpins polls (about 0.1 s) it
-// gives up and sets *err to the epoch, and once *err holds the epoch no
-// workgroup of the call waits any more (a wrong result, never a hang; the
-// solve's caller reads *err and reports the failure).  The block's inverse is
-// loaded before the first wait and the W block of the next dependency before
-// its flag is polled, so a published x_j is followed only by its own loads.
+// look-back").  Block row b (b = i forward, nblk-1-i backward for the i-th
+// group of kLbParts workgroups) is shared by kLbParts workgroups: part p
+// accumulates L[b, j] x_j over the dependencies j with j % kLbParts == p, as
+// each x_j is published (a per-block flag set to this call's epoch).  Parts
+// 0 .. kLbParts-2 publish their partial sums (flag per part); the last part
+// waits for them, forms x_b = inv_b (y_b - sum) and publishes it.  The long
+// block rows near the end are thus streamed by several workgroups at once.
+// A workgroup waits only on workgroups with smaller indices (earlier blocks,
+// or earlier parts of its own block), which are dispatched before it.  The
+// waits are bounded: after kLbMaxSpins polls (about 0.1 s) one gives up and
+// sets *err to the epoch, and once *err holds the epoch no workgroup of the
+// call waits any more (a wrong result, never a hang; the solve's caller reads
+// *err and reports the failure).  Values and flags are relaxed agent-scope
+// atomics (they bypass the caches that are not coherent across XCDs), and a
+// barrier after the value stores orders them before the flag.  The last
+// part loads the block's inverse before its first wait, and every part loads
+// the W block of its next dependency before polling that dependency's flag.
 constexpr int kLbMaxSpins = 1 << 17;
+constexpr int kLbParts = 4;
+
+// One bounded wait of thread 0 for flags[i] == epoch.
+__device__ __forceinline__ void lb_wait(const int* flags, long long i, int epoch, int* err) {
+  int spins = 0;
+  while (__hip_atomic_load(&flags[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+    if (++spins > kLbMaxSpins) {
+      atomicExch(err, epoch);
+      break;
+    }
+    if ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch) break;
+    __builtin_amdgcn_s_sleep(2);
+  }
+}
+
+// flags: [nblk] block flags, then [nblk * kLbParts] part flags; part:
+// partial sums, nblk * kLbParts * TB.
 template <typename T, int TB>
 __global__ __launch_bounds__(256) void k_trsv_lookback(const T* __restrict__ W, long long ld, long long N, int lower,
                                                        const T* __restrict__ inv, const T* __restrict__ y,
-                                                       T* out, int* flags, int epoch, int* err) {
+                                                       T* out, T* part, int* flags, int epoch, int* err) {
   constexpr int E = 16 / sizeof(T);  // elements of a lane's row piece
   constexpr int kP = TB / 8;          // rows per thread (8 rows per pass)
   static_assert(E * 32 == TB, "row piece layout");
   __shared__ T sres[TB];
   const long long nblk = (N + TB - 1) / TB;
-  const long long b = lower ? (long long)blockIdx.x : nblk - 1 - (long long)blockIdx.x;
+  const long long grp = (long long)blockIdx.x / kLbParts;
+  const int pidx = (int)(blockIdx.x % kLbParts);
+  const bool last = (pidx == kLbParts - 1);
+  const long long b = lower ? grp : nblk - 1 - grp;
   const long long rb = b * TB;
   const int m = (int)min((long long)TB, N - rb);
   const int sub = threadIdx.x & 31, slot = threadIdx.x >> 5;
   const int col = sub * E;
+  int* pflags = flags + nblk;
   T acc[kP];
 #pragma unroll
   for (int p = 0; p < kP; ++p) acc[p] = (T)0;
-  // the block's inverse, loaded before any wait: thread t holds part t % kParts
-  // of row t / kParts (kParts = 256 / TB parts of kIp elements)
+  // the block's inverse (last part only), loaded before any wait: thread t
+  // holds part t % kParts of row t / kParts (kParts = 256 / TB parts of kIp)
   constexpr int kParts = 256 / TB, kIp = TB / kParts;
   static_assert(kIp % E == 0, "inverse parts are whole 16-byte pieces");
   const int irow = threadIdx.x / kParts, ipart = threadIdx.x % kParts;
   T pinv[kIp];
-  {
+  if (last) {
     const T* src = inv + (size_t)b * TB * TB + (size_t)irow * TB + ipart * kIp;
 #pragma unroll
     for (int k0 = 0; k0 < kIp; k0 += E) {
@@ -1968,7 +1996,7 @@ __global__ __launch_bounds__(256) void k_trsv_lookback(const T* __restrict__ W, 
     }
   }
   const long long ndep = lower ? b : nblk - 1 - b;
-  for (long long d = 0; d < ndep; ++d) {
+  for (long long d = pidx; d < ndep; d += kLbParts) {
     const long long j = lower ? d : nblk - 1 - d;
     const long long cj = j * TB;
     const int mj = (int)min((long long)TB, N - cj);
@@ -1999,17 +2027,7 @@ __global__ __launch_bounds__(256) void k_trsv_lookback(const T* __restrict__ W, 
         }
       }
     }
-    if (threadIdx.x == 0) {
-      int spins = 0;
-      while (__hip_atomic_load(&flags[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-        if (++spins > kLbMaxSpins) {
-          atomicExch(err, epoch);
-          break;
-        }
-        if ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch) break;
-        __builtin_amdgcn_s_sleep(2);
-      }
-    }
+    if (threadIdx.x == 0) lb_wait(flags, j, epoch, err);
     __syncthreads();
     T xj[E];
 #pragma unroll
@@ -2021,14 +2039,42 @@ __global__ __launch_bounds__(256) void k_trsv_lookback(const T* __restrict__ W, 
 #pragma unroll
       for (int e = 0; e < E; ++e) acc[p] += w[p][e] * xj[e];
   }
-  // row sums across the 32 lanes, then the block's right-hand side
+  // row sums across the 32 lanes
+  T rs[kP];
 #pragma unroll
   for (int p = 0; p < kP; ++p) {
     T sum = acc[p];
 #pragma unroll
     for (int off = 16; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 32);
+    rs[p] = sum;
+  }
+  T* mypart = part + ((size_t)b * kLbParts + pidx) * TB;
+  if (!last) {
+    // publish this part's sums
+#pragma unroll
+    for (int p = 0; p < kP; ++p) {
+      const int r = slot + 8 * p;
+      if (sub == 0) __hip_atomic_store(&mypart[r], rs[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      __hip_atomic_store(&pflags[b * kLbParts + pidx], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  // the last part: the other parts' sums, then the block's right-hand side
+  if (threadIdx.x == 0)
+    for (int q = 0; q < kLbParts - 1; ++q) lb_wait(pflags, b * kLbParts + q, epoch, err);
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < kP; ++p) {
     const int r = slot + 8 * p;
-    if (sub == 0) sres[r] = (r < m) ? y[rb + r] - sum : (T)0;
+    if (sub == 0) {
+      T sum = rs[p];
+      for (int q = 0; q < kLbParts - 1; ++q)
+        sum += __hip_atomic_load(&part[((size_t)b * kLbParts + q) * TB + r], __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+      sres[r] = (r < m) ? y[rb + r] - sum : (T)0;
+    }
   }
   __syncthreads();
   {
@@ -2039,11 +2085,6 @@ __global__ __launch_bounds__(256) void k_trsv_lookback(const T* __restrict__ W, 
     for (int off = 1; off < kParts; off <<= 1) xb += __shfl_xor(xb, off);
     if (ipart == 0 && irow < m) __hip_atomic_store(&out[rb + irow], xb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  // The block's values and the flag are agent-scope atomics, which bypass the
-  // caches that are not coherent across XCDs, and the barrier waits until the
-  // value stores have completed: a reader that sees the flag sees the values.
-  // (Acquire/release fences here would write back or invalidate whole L2s at
-  // every block.)
   __syncthreads();
   if (threadIdx.x == 0) __hip_atomic_store(&flags[b], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -2083,18 +2124,22 @@ static int tri_solve_inv(qd_context* h, const T* W, long long ld, long long N, i
   T* tmp = reinterpret_cast<T*>(h->lu->trs_tmp.p);
   if (nrhs == 1 && use_lookback()) {
     qd_lu_state* s = h->lu;
-    if ((long long)s->lb_flags.count < nblk + 1) {
-      QD_ALLOC(h, s->lb_flags, (size_t)nblk + 1);
+    const long long nflags = 1 + nblk * (1 + kLbParts);
+    if ((long long)s->lb_flags.count < nflags) {
+      QD_ALLOC(h, s->lb_flags, (size_t)nflags);
       QD_HIP(h, hipMemsetAsync(s->lb_flags.p, 0, s->lb_flags.count * sizeof(int), h->stream));
     }
+    QD_ALLOC(h, s->lb_part, (size_t)nblk * kLbParts * TB * sizeof(T));
+    T* part = reinterpret_cast<T*>(s->lb_part.p);
     int* err = s->lb_flags.p;  // the epoch of a call in which a wait gave up
     int* flags = s->lb_flags.p + 1;
     int ep = ++s->lb_epoch;
-    hipLaunchKernelGGL((k_trsv_lookback<T, TB>), dim3((unsigned)nblk), dim3(256), 0, h->stream, W, ld, N, 1, invL,
-                       (const T*)x, tmp, flags, ep, err);
+    const dim3 g((unsigned)(nblk * kLbParts));
+    hipLaunchKernelGGL((k_trsv_lookback<T, TB>), g, dim3(256), 0, h->stream, W, ld, N, 1, invL, (const T*)x, tmp, part,
+                       flags, ep, err);
     ep = ++s->lb_epoch;
-    hipLaunchKernelGGL((k_trsv_lookback<T, TB>), dim3((unsigned)nblk), dim3(256), 0, h->stream, W, ld, N, 0, invU,
-                       (const T*)tmp, x, flags, ep, err);
+    hipLaunchKernelGGL((k_trsv_lookback<T, TB>), g, dim3(256), 0, h->stream, W, ld, N, 0, invU, (const T*)tmp, x, part,
+                       flags, ep, err);
     QD_HIP(h, hipGetLastError());
     return QD_OK;
   }
