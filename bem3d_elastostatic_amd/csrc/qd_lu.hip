@@ -1423,14 +1423,13 @@ static int tslu_leaf(LuCtx<float>& c, long long c0, int w, long long p0, long lo
   using Nd = TsNode<kTsTreeR, kTsTreeV>;
   const unsigned nb = (unsigned)((m + Nd::rows - 1) / Nd::rows);
   if (c.diag) {
-    // diagonal pivots: no interchanges, so the finish permutes nothing
-    hipLaunchKernelGGL(k_leaf_diag, dim3(1), dim3(64), 0, c.st, c.W, c.ld, c.N, c0, w, s->ts_lu.p, s->ts_top.p,
-                       s->ts_meta.p, s->ipiv_dev.p, s->growth.p);
+    // diagonal pivots (leaves of up to kDW columns): no interchanges
+    if (w > kDW) return set_error(c.h, QD_ERR_STATE, "diagonal-pivot leaf of %d columns", w);
+    hipLaunchKernelGGL(k_leaf_diag, dim3(1), dim3(64), 0, c.st, (const float*)c.W, c.ld, c.N, c0, w, s->ts_lu.p,
+                       s->ts_linv.p + (c0 / kTsW) * kTsW * kTsW, s->ipiv_dev.p, s->growth.p);
     const long long below = std::max<long long>(m - w, 1);
-    const int row_blocks = (int)((below + 255) / 256);
-    hipLaunchKernelGGL(k_tslu_finish, dim3((unsigned)(row_blocks + 1)), dim3(256), 0, c.st, c.W, c.ld, c.N, c0, w,
-                       p0, p0, (const float*)s->ts_lu.p, (const float*)s->ts_top.p, (const int*)s->ts_meta.p,
-                       row_blocks, s->ts_linv.p + (c0 / kTsW) * kTsW * kTsW, s->growth.p);
+    hipLaunchKernelGGL(k_finish_diag, dim3((unsigned)((below + 255) / 256)), dim3(256), 0, c.st, c.W, c.ld, c.N, c0,
+                       w, (const float*)s->ts_lu.p, s->growth.p);
     QD_HIP(c.h, hipGetLastError());
     return QD_OK;
   }
@@ -1441,7 +1440,7 @@ static int tslu_leaf(LuCtx<float>& c, long long c0, int w, long long p0, long lo
   const int col_blocks = (int)((p1 - p0 + 255) / 256);
   hipLaunchKernelGGL(k_tslu_finish, dim3((unsigned)(row_blocks + col_blocks + 1)), dim3(256), 0, c.st, c.W, c.ld, c.N,
                      c0, w, p0, p1, (const float*)s->ts_lu.p, (const float*)s->ts_top.p, (const int*)s->ts_meta.p,
-                     row_blocks, s->ts_linv.p + (c0 / kTsW) * kTsW * kTsW, (unsigned*)nullptr);
+                     row_blocks, s->ts_linv.p + (c0 / kTsW) * kTsW * kTsW);
   QD_HIP(c.h, hipGetLastError());
   return QD_OK;
 }
@@ -1451,7 +1450,7 @@ template <typename T>
 static int panel(LuCtx<T>& c, long long c0, int w, long long p0, long long p1) {
   int rc;
   if constexpr (std::is_same<T, float>::value) {
-    if (w <= kLeaf && c.calu) return tslu_leaf(c, c0, w, p0, p1);
+    if (w <= (c.diag ? kDW : kLeaf) && c.calu) return tslu_leaf(c, c0, w, p0, p1);
   }
   if (w <= kLeaf) {
     const long long lend = c0 + w;
@@ -1462,7 +1461,8 @@ static int panel(LuCtx<T>& c, long long c0, int w, long long p0, long long p1) {
     }
     return QD_OK;
   }
-  const int w1 = ((w / 2) + kLeaf - 1) / kLeaf * kLeaf;
+  const int lw = (std::is_same<T, float>::value && c.calu && c.diag) ? kDW : kLeaf;
+  const int w1 = ((w / 2) + lw - 1) / lw * lw;
   if ((rc = panel(c, c0, w1, p0, p1))) return rc;
   if ((rc = trsm_lower(c, c0, w1, c0 + w1, c0 + w))) return rc;
   if ((rc = gemm_sub(c, c.N - c0 - w1, w - w1, w1, c.W + (c0 + w1) * c.ld + c0, c.W + c0 * c.ld + c0 + w1,
@@ -1575,9 +1575,9 @@ static int factor(LuCtx<T>& c, int NB) {
     QD_ALLOC(c.h, s->ts_cand, (size_t)ncand);
     QD_ALLOC(c.h, s->ts_cnt, (size_t)ncnt + 1);
     QD_ALLOC(c.h, s->ts_meta, (size_t)kTsMeta);
-    QD_ALLOC(c.h, s->ts_lu, (size_t)kTsW * kTsW);
+    QD_ALLOC(c.h, s->ts_lu, (size_t)kDW * kDW);
     QD_ALLOC(c.h, s->ts_top, (size_t)kTsW * kTsW);
-    QD_ALLOC(c.h, s->ts_linv, (size_t)((N + kTsW - 1) / kTsW + 1) * kTsW * kTsW);
+    QD_ALLOC(c.h, s->ts_linv, (size_t)((N + kTsW - 1) / kTsW + 2) * kTsW * kTsW);
     // the strip solves keep their strips in dynamic LDS
     QD_HIP(c.h, hipFuncSetAttribute(reinterpret_cast<const void*>(&k_trsm_strip<32>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, 256 * 32 * (int)sizeof(float)));

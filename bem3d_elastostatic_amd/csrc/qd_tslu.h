@@ -375,65 +375,6 @@ __device__ __forceinline__ void ts_growth_max(float v, unsigned* growth) {
   if ((threadIdx.x & 63) == 0) atomicMax(growth, b);
 }
 
-// A leaf with diagonal pivots (the top w rows in their order), in the tree's
-// output format for k_tslu_finish, for factorisations that try diagonal
-// pivoting first.  One wave; lane k holds row c0+k.  The largest multiplier
-// |l| goes to *growth (a zero pivot counts as +inf), so that the host can
-// reject the factorisation.
-__global__ __launch_bounds__(64) void k_leaf_diag(const float* __restrict__ W, long long ld, long long N,
-                                                  long long c0, int w, float* __restrict__ lu_out,
-                                                  float* __restrict__ top_out, int* __restrict__ meta,
-                                                  int* __restrict__ ipiv, unsigned* __restrict__ growth) {
-  __shared__ __attribute__((aligned(16))) float srow[kTsW];
-  const int lane = threadIdx.x;
-  float a[kTsW];
-  ts_load_row(W, ld, (lane < w) ? c0 + lane : c0, c0, w, a);
-  if (lane >= w)
-#pragma unroll
-    for (int j = 0; j < kTsW; ++j) a[j] = 0.0f;
-  if (lane < kTsW)
-#pragma unroll
-    for (int j4 = 0; j4 < kTsW / 4; ++j4)
-      *reinterpret_cast<float4*>(top_out + lane * kTsW + 4 * j4) =
-          make_float4(a[4 * j4], a[4 * j4 + 1], a[4 * j4 + 2], a[4 * j4 + 3]);
-  float gmax = 0.0f;
-#pragma unroll
-  for (int k = 0; k < kTsW; ++k) {
-    if (k < w) {
-      if (lane == k) {
-#pragma unroll
-        for (int j4 = 0; j4 < kTsW / 4; ++j4) {
-          const float4 v = make_float4(a[4 * j4], a[4 * j4 + 1], a[4 * j4 + 2], a[4 * j4 + 3]);
-          *reinterpret_cast<float4*>(srow + 4 * j4) = v;
-          *reinterpret_cast<float4*>(lu_out + k * kTsW + 4 * j4) = v;
-        }
-      }
-      __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the row is in LDS
-      __builtin_amdgcn_wave_barrier();
-      const float piv = srow[k];
-      if (piv == 0.0f) gmax = __int_as_float(0x7f800000);
-      const float rp = (piv != 0.0f) ? 1.0f / piv : 0.0f;
-      if (lane > k && lane < w) {
-        const float l = a[k] * rp;
-        a[k] = l;
-        gmax = fmaxf(gmax, fabsf(l));
-#pragma unroll
-        for (int j = k + 1; j < kTsW; ++j) a[j] -= l * srow[j];
-      }
-      // every lane has read the row before the next step writes it
-      __builtin_amdgcn_s_waitcnt(0xc07f);
-      __builtin_amdgcn_wave_barrier();
-    }
-  }
-  ts_growth_max(gmax, growth);
-  if (lane < kTsW) {
-    meta[lane] = (int)c0 + lane;
-    meta[128 + lane] = (lane < w) ? (int)c0 + lane : -1;
-    if (lane < w) ipiv[c0 + lane] = (int)c0 + lane;
-  }
-  if (lane == 0) meta[96] = 0;
-}
-
 // L21 = A21 U11^-1 (one row per thread, U11 read from LDS by broadcast),
 // against the root's winners' LU, which workgroup 0 also writes into the top
 // rows; then, in workgroups row_blocks.., the permutation on the panel's
@@ -470,8 +411,7 @@ __global__ __launch_bounds__(256) void k_tslu_finish(float* __restrict__ W, long
                                                      int w, long long p0, long long p1,
                                                      const float* __restrict__ lu_in,
                                                      const float* __restrict__ top_in, const int* __restrict__ meta,
-                                                     int row_blocks, float* __restrict__ linv_out,
-                                                     unsigned* __restrict__ growth) {
+                                                     int row_blocks, float* __restrict__ linv_out) {
   __shared__ __attribute__((aligned(16))) float sU[kTsW][kTsW];
   __builtin_amdgcn_s_setprio(3);
   __shared__ float sR[kTsW];
@@ -554,28 +494,6 @@ __global__ __launch_bounds__(256) void k_tslu_finish(float* __restrict__ W, long
   }
   __syncthreads();
   const long long r = c0 + w + (long long)blockIdx.x * 256 + tid;
-  if (growth) {
-    // diagonal pivots: no row is displaced; fold this row's largest |l| into
-    // *growth (whole waves; rows past N count as 0)
-    float x[kTsW];
-    float g = 0.0f;
-    if (r < N) {
-      ts_load_row(W, ld, r, c0, w, x);
-      tf_solve(std::make_integer_sequence<int, kTsW>{}, x, w, sU, sR);
-#pragma unroll
-      for (int j = 0; j < kTsW; ++j) g = fmaxf(g, fabsf(x[j]));
-      float* dst = W + r * ld + c0;
-      if (w == kTsW) {
-#pragma unroll
-        for (int j4 = 0; j4 < kTsW / 4; ++j4)
-          *reinterpret_cast<float4*>(dst + 4 * j4) = make_float4(x[4 * j4], x[4 * j4 + 1], x[4 * j4 + 2], x[4 * j4 + 3]);
-      } else {
-        for (int j = 0; j < w; ++j) dst[j] = x[j];
-      }
-    }
-    ts_growth_max(g, growth);
-    return;
-  }
   if (r >= N) return;
   int src = -1;
   const int nd = snd;
@@ -603,8 +521,184 @@ __global__ __launch_bounds__(256) void k_tslu_finish(float* __restrict__ W, long
     for (int j4 = 0; j4 < kTsW / 4; ++j4)
       *reinterpret_cast<float4*>(dst + 4 * j4) = make_float4(x[4 * j4], x[4 * j4 + 1], x[4 * j4 + 2], x[4 * j4 + 3]);
   } else {
-    for (int j = 0; j < w; ++j) dst[j] = x[j];
+#pragma unroll
+    for (int j = 0; j < kTsW; ++j)
+      if (j < w) dst[j] = x[j];
   }
+}
+
+// ---------------------------------------------------------------------------
+// Leaves with diagonal pivots (the factorisations that try them first), 64
+// columns wide: a wave holds the leaf's 64 top rows, one per lane, and the
+// recursion above has one level fewer than with 32-column leaves.
+constexpr int kDW = 64;
+
+// Loads the 64-column segment [c0, c0+64) of row g (zero beyond column w).
+__device__ __forceinline__ void dw_load_row(const float* __restrict__ W, long long ld, long long g, long long c0,
+                                            int w, float (&r)[kDW]) {
+  const float4* src = reinterpret_cast<const float4*>(W + g * ld + c0);
+#pragma unroll
+  for (int j4 = 0; j4 < kDW / 4; ++j4) {
+    const float4 v = src[j4];
+    r[4 * j4 + 0] = (4 * j4 + 0 < w) ? v.x : 0.0f;
+    r[4 * j4 + 1] = (4 * j4 + 1 < w) ? v.y : 0.0f;
+    r[4 * j4 + 2] = (4 * j4 + 2 < w) ? v.z : 0.0f;
+    r[4 * j4 + 3] = (4 * j4 + 3 < w) ? v.w : 0.0f;
+  }
+}
+
+// One step (column K) of the top block's elimination: lane K's row is final
+// and goes to LDS (slot K & 1, so one barrier per step suffices: a lane
+// writing slot K & 1 again at K + 2 has passed the barrier of step K + 1,
+// which every lane reaches only after reading slot K & 1) and to lu_out.
+template <int K>
+__device__ __forceinline__ void dw_step(float (&a)[kDW], int w, float (*srow)[kDW], float* __restrict__ lu_out,
+                                        float& gmax) {
+  if (K < w) {
+    const int lane = threadIdx.x;
+    if (lane == K) {
+#pragma unroll
+      for (int j4 = 0; j4 < kDW / 4; ++j4) {
+        const float4 v = make_float4(a[4 * j4], a[4 * j4 + 1], a[4 * j4 + 2], a[4 * j4 + 3]);
+        *reinterpret_cast<float4*>(&srow[K & 1][4 * j4]) = v;
+        *reinterpret_cast<float4*>(lu_out + K * kDW + 4 * j4) = v;
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the row is in LDS
+    __builtin_amdgcn_wave_barrier();
+    const float piv = srow[K & 1][K];
+    if (piv == 0.0f) gmax = __int_as_float(0x7f800000);
+    const float rp = (piv != 0.0f) ? __builtin_amdgcn_rcpf(piv) : 0.0f;
+    if (lane > K && lane < w) {
+      const float l = a[K] * rp;
+      a[K] = l;
+      gmax = fmaxf(gmax, fabsf(l));
+      const float4* r4 = reinterpret_cast<const float4*>(&srow[K & 1][0]);
+#pragma unroll
+      for (int j4 = (K + 1) / 4; j4 < kDW / 4; ++j4) {
+        const float4 v = r4[j4];
+        if (4 * j4 > K) a[4 * j4] -= l * v.x;
+        if (4 * j4 + 1 > K) a[4 * j4 + 1] -= l * v.y;
+        if (4 * j4 + 2 > K) a[4 * j4 + 2] -= l * v.z;
+        if (4 * j4 + 3 > K) a[4 * j4 + 3] -= l * v.w;
+      }
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+template <int... Ks>
+__device__ __forceinline__ void dw_all(std::integer_sequence<int, Ks...>, float (&a)[kDW], int w,
+                                       float (*srow)[kDW], float* __restrict__ lu_out, float& gmax) {
+  (dw_step<Ks>(a, w, srow, lu_out, gmax), ...);
+}
+
+// The top w x w block (w <= 64) of a diagonal-pivot leaf, one wave: its LU
+// (multipliers below the diagonal) to lu_out (64 x 64), the inverses of the
+// unit lower factor's two 32 x 32 diagonal blocks to linv_out (two 32 x 32
+// blocks, for the strip solves), ipiv = identity, and the largest multiplier
+// |l| (a zero pivot counts as +inf) folded into *growth.
+__global__ __launch_bounds__(64) void k_leaf_diag(const float* __restrict__ W, long long ld, long long N,
+                                                  long long c0, int w, float* __restrict__ lu_out,
+                                                  float* __restrict__ linv_out, int* __restrict__ ipiv,
+                                                  unsigned* __restrict__ growth) {
+  __shared__ __attribute__((aligned(16))) float srow[2][kDW];
+  __shared__ float sL[kDW][kDW + 1];
+  const int lane = threadIdx.x;
+  float a[kDW];
+  dw_load_row(W, ld, (lane < w) ? c0 + lane : c0, c0, w, a);
+  if (lane >= w)
+#pragma unroll
+    for (int j = 0; j < kDW; ++j) a[j] = 0.0f;
+  float gmax = 0.0f;
+  dw_all(std::make_integer_sequence<int, kDW>{}, a, w, srow, lu_out, gmax);
+  ts_growth_max(gmax, growth);
+  if (lane < w) ipiv[c0 + lane] = (int)c0 + lane;
+  // the unit lower factor (identity beyond w) for the inverses: lane r's
+  // multipliers are its row of L
+#pragma unroll
+  for (int j = 0; j < kDW; ++j) sL[lane][j] = (j < lane && lane < w) ? a[j] : 0.0f;
+  __builtin_amdgcn_s_waitcnt(0xc07f);
+  __builtin_amdgcn_wave_barrier();
+  // lane t: column t % 32 of the inverse of diagonal block t / 32
+  const int blk = lane >> 5, cc = lane & 31, o = 32 * blk;
+  float y[32];
+#pragma unroll
+  for (int r = 0; r < 32; ++r) {
+    float acc = (r == cc) ? 1.0f : 0.0f;
+#pragma unroll
+    for (int k = 0; k < r; ++k) acc -= sL[o + r][o + k] * y[k];
+    y[r] = acc;
+  }
+#pragma unroll
+  for (int r = 0; r < 32; ++r) linv_out[blk * 1024 + r * 32 + cc] = y[r];
+}
+
+template <int K>
+__device__ __forceinline__ void dwf_step(float (&x)[kDW], int w, const float (*__restrict__ sU)[kDW],
+                                         const float* __restrict__ sR) {
+  if (K < w) {
+    x[K] *= sR[K];
+    const float4* u4 = reinterpret_cast<const float4*>(sU[K]);
+#pragma unroll
+    for (int j4 = (K + 1) / 4; j4 < kDW / 4; ++j4) {
+      const float4 v = u4[j4];
+      if (4 * j4 > K) x[4 * j4] -= x[K] * v.x;
+      if (4 * j4 + 1 > K) x[4 * j4 + 1] -= x[K] * v.y;
+      if (4 * j4 + 2 > K) x[4 * j4 + 2] -= x[K] * v.z;
+      if (4 * j4 + 3 > K) x[4 * j4 + 3] -= x[K] * v.w;
+    }
+  }
+}
+
+template <int... Ks>
+__device__ __forceinline__ void dwf_all(std::integer_sequence<int, Ks...>, float (&x)[kDW], int w,
+                                        const float (*__restrict__ sU)[kDW], const float* __restrict__ sR) {
+  (dwf_step<Ks>(x, w, sU, sR), ...);
+}
+
+// L21 = A21 U11^-1 for the rows below a diagonal-pivot leaf's top block, one
+// thread per row, U11 from lu_in by LDS broadcast; workgroup 0 also writes the
+// block's LU into W's top rows.  The largest |l| goes to *growth.
+__global__ __launch_bounds__(256) void k_finish_diag(float* __restrict__ W, long long ld, long long N, long long c0,
+                                                     int w, const float* __restrict__ lu_in,
+                                                     unsigned* __restrict__ growth) {
+  __shared__ __attribute__((aligned(16))) float sU[kDW][kDW];
+  __shared__ float sR[kDW];
+  __builtin_amdgcn_s_setprio(3);
+  const int tid = threadIdx.x;
+  for (int i = tid; i < kDW * kDW; i += 256) {
+    const int r = i / kDW, j = i % kDW;
+    const float v = (r < w && j < w) ? lu_in[i] : 0.0f;
+    sU[r][j] = (j >= r) ? v : 0.0f;
+    if (blockIdx.x == 0 && r < w && j < w) W[(c0 + r) * ld + c0 + j] = v;
+  }
+  __syncthreads();
+  if (tid < kDW) {
+    const float d = sU[tid][tid];
+    sR[tid] = (tid < w && d != 0.0f) ? 1.0f / d : 0.0f;
+  }
+  __syncthreads();
+  const long long r = c0 + w + (long long)blockIdx.x * 256 + tid;
+  float g = 0.0f;
+  if (r < N) {
+    float x[kDW];
+    dw_load_row(W, ld, r, c0, w, x);
+    dwf_all(std::make_integer_sequence<int, kDW>{}, x, w, sU, sR);
+#pragma unroll
+    for (int j = 0; j < kDW; ++j) g = fmaxf(g, fabsf(x[j]));
+    float* dst = W + r * ld + c0;
+    if (w == kDW) {
+#pragma unroll
+      for (int j4 = 0; j4 < kDW / 4; ++j4)
+        *reinterpret_cast<float4*>(dst + 4 * j4) = make_float4(x[4 * j4], x[4 * j4 + 1], x[4 * j4 + 2], x[4 * j4 + 3]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < kDW; ++j)
+        if (j < w) dst[j] = x[j];
+    }
+  }
+  ts_growth_max(g, growth);
 }
 
 }  // namespace qd
