@@ -604,6 +604,7 @@ __global__ __launch_bounds__(64) void k_leaf_diag(const float* __restrict__ W, l
                                                   unsigned* __restrict__ growth) {
   __shared__ __attribute__((aligned(16))) float srow[2][kDW];
   __shared__ float sL[kDW][kDW + 1];
+  __builtin_amdgcn_s_setprio(3);
   const int lane = threadIdx.x;
   float a[kDW];
   dw_load_row(W, ld, (lane < w) ? c0 + lane : c0, c0, w, a);
