@@ -355,7 +355,6 @@ __global__ __launch_bounds__(256) void k_trsm_strip(float* __restrict__ W, long 
   // rows at once, and the swizzle spreads those over the banks
   __shared__ __attribute__((aligned(16))) float sinv[32][32], sl[32][32];
   __shared__ float sb[32][kSW];
-  __shared__ __attribute__((aligned(16))) float lbuf[2][kStripChunk][32];
   constexpr int kSlots = 256 / kSW;
   constexpr int NT = (kSW + 15) / 16;  // MFMA column tiles
   const int tid = threadIdx.x, c = tid % kSW, slot = tid / kSW;
@@ -365,21 +364,24 @@ __global__ __launch_bounds__(256) void k_trsm_strip(float* __restrict__ W, long 
   for (int r = slot; r < m; r += kSlots) xs[r * kSW + c] = cv ? W[(r0 + r) * ld + col] : 0.0f;
   const float* inv0 = linv + (r0 / 32) * 1024;
   auto swz = [](int r) { return r & 7; };
-  // a chunk's L segments: 128 rows x 32 floats, four float4 per thread
-  auto load_chunk = [&](long long d0, int first, float4 (&v)[4]) {
+  // a chunk's L rows as MFMA A operands, straight from memory (L is shared by
+  // every workgroup and stays in L2): wave w's rows first + 32 w + 16 i + lr,
+  // lane group g's k = 8 g .. 8 g + 7 of the diagonal block's columns
+  auto load_a = [&](long long d0, int first, float (&av)[2][8]) {
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int e = tid + 256 * u, rr = e >> 3, k4 = e & 7;
-      const int row = min(first + rr, m - 1);
-      v[u] = *reinterpret_cast<const float4*>(W + (r0 + row) * ld + d0 + 4 * k4);
-    }
-  };
-  auto stage_chunk = [&](int buf, const float4 (&v)[4]) {
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int e = tid + 256 * u, rr = e >> 3, k4 = e & 7;
-      const float4 t = v[u];
-      *reinterpret_cast<float4*>(&lbuf[buf][rr][4 * (k4 ^ swz(rr))]) = t;
+    for (int i2 = 0; i2 < 2; ++i2) {
+      const int row = min(first + 32 * wave + 16 * i2 + lr, m - 1);
+      const float* src = W + (r0 + row) * ld + d0 + 8 * lg;
+      const float4 p0 = *reinterpret_cast<const float4*>(src);
+      const float4 p1 = *reinterpret_cast<const float4*>(src + 4);
+      av[i2][0] = p0.x;
+      av[i2][1] = p0.y;
+      av[i2][2] = p0.z;
+      av[i2][3] = p0.w;
+      av[i2][4] = p1.x;
+      av[i2][5] = p1.y;
+      av[i2][6] = p1.z;
+      av[i2][7] = p1.w;
     }
   };
   // the k's of an MFMA step are permuted (lane group g supplies k = 8 g + t
@@ -429,8 +431,8 @@ __global__ __launch_bounds__(256) void k_trsm_strip(float* __restrict__ W, long 
   for (int i = 0; i < m / 32; ++i) {
     const long long d0 = r0 + 32 * i;  // the diagonal block's first row and column
     const int below = 32 * (i + 1), nch = (m - below + kStripChunk - 1) / kStripChunk;
-    float4 pf[4];
-    if (nch > 0) load_chunk(d0, below, pf);  // in flight during the block solve
+    float an[2][8];
+    if (nch > 0) load_a(d0, below, an);  // in flight during the block solve
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int e = tid + 256 * u, rr = e >> 5, kk = e & 31;
@@ -471,34 +473,33 @@ __global__ __launch_bounds__(256) void k_trsm_strip(float* __restrict__ W, long 
       for (int q = 0; q < 4; ++q) xb[(16 * ti + 4 * lg + q) * kSW + bcol] = xv[q] + dx[q];
     __syncthreads();
     // the rows below, chunk by chunk: X_r -= L[r, block i] X_i.  Wave w
-    // updates chunk rows [32 w, 32 w + 32) x the strip's columns; its X_i
-    // values are loaded once per block.
-    if (nch > 0) stage_chunk(0, pf);
+    // updates chunk rows [32 w, 32 w + 32) x the strip's columns, which no
+    // other wave touches, so the chunks need no barrier; its X_i values are
+    // loaded once per block and the next chunk's L rows while one is used.
     float xbf[NT][8];
 #pragma unroll
     for (int j = 0; j < NT; ++j)
 #pragma unroll
       for (int t = 0; t < 8; ++t)
         xbf[j][t] = (16 * j + lr < kSW) ? xb[(8 * lg + t) * kSW + 16 * j + lr] : 0.0f;
-    __syncthreads();
     for (int ch = 0; ch < nch; ++ch) {
       const int first = below + ch * kStripChunk;
-      if (ch + 1 < nch) load_chunk(d0, first + kStripChunk, pf);
-      const int buf = ch & 1;
+      float av[2][8];
+#pragma unroll
+      for (int i2 = 0; i2 < 2; ++i2)
+#pragma unroll
+        for (int t = 0; t < 8; ++t) av[i2][t] = an[i2][t];
+      if (ch + 1 < nch) load_a(d0, first + kStripChunk, an);
       f4 acc[2][NT];
 #pragma unroll
-      for (int i2 = 0; i2 < 2; ++i2) {
-        const int rr = 32 * wave + 16 * i2 + lr;
-        float av[8];
-        a_row(&lbuf[buf][rr][0], rr, av);
+      for (int i2 = 0; i2 < 2; ++i2)
 #pragma unroll
         for (int j = 0; j < NT; ++j) {
           acc[i2][j] = (f4){0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
           for (int t = 0; t < 8; ++t)
-            acc[i2][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[t], xbf[j][t], acc[i2][j], 0, 0, 0);
+            acc[i2][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i2][t], xbf[j][t], acc[i2][j], 0, 0, 0);
         }
-      }
 #pragma unroll
       for (int i2 = 0; i2 < 2; ++i2)
 #pragma unroll
@@ -509,8 +510,6 @@ __global__ __launch_bounds__(256) void k_trsm_strip(float* __restrict__ W, long 
               const int r = first + 32 * wave + 16 * i2 + 4 * lg + q4;
               if (r < m) xs[r * kSW + 16 * j + lr] -= acc[i2][j][q4];
             }
-      if (ch + 1 < nch) stage_chunk(buf ^ 1, pf);
-      __syncthreads();
     }
   }
   if (cv)
@@ -1923,14 +1922,15 @@ __global__ __launch_bounds__(256) void k_block_step(const T* __restrict__ W, lon
 // 0 .. kLbParts-2 publish their partial sums (flag per part); the last part
 // waits for them, forms x_b = inv_b (y_b - sum) and publishes it.  The long
 // block rows near the end are thus streamed by several workgroups at once.
+// A publication is: the values' stores, a wait for their completion
+// (vmcnt(0): __syncthreads alone waits only for LDS), a barrier, the flag.
 // A workgroup waits only on workgroups with smaller indices (earlier blocks,
 // or earlier parts of its own block), which are dispatched before it.  The
 // waits are bounded: after kLbMaxSpins polls (about 0.1 s) one gives up and
 // sets *err to the epoch, and once *err holds the epoch no workgroup of the
 // call waits any more (a wrong result, never a hang; the solve's caller reads
 // *err and reports the failure).  Values and flags are relaxed agent-scope
-// atomics (they bypass the caches that are not coherent across XCDs), and a
-// barrier after the value stores orders them before the flag.  The last
+// atomics (they bypass the caches that are not coherent across XCDs).  The last
 // part loads the block's inverse before its first wait, and every part loads
 // the W block of its next dependency before polling that dependency's flag.
 constexpr int kLbMaxSpins = 1 << 17;
@@ -2056,6 +2056,9 @@ __global__ __launch_bounds__(256) void k_trsv_lookback(const T* __restrict__ W, 
       const int r = slot + 8 * p;
       if (sub == 0) __hip_atomic_store(&mypart[r], rs[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    // the stores have completed before the flag is written: __syncthreads
+    // alone does not wait for global stores
+    __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0)
     __syncthreads();
     if (threadIdx.x == 0)
       __hip_atomic_store(&pflags[b * kLbParts + pidx], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -2085,6 +2088,7 @@ __global__ __launch_bounds__(256) void k_trsv_lookback(const T* __restrict__ W, 
     for (int off = 1; off < kParts; off <<= 1) xb += __shfl_xor(xb, off);
     if (ipart == 0 && irow < m) __hip_atomic_store(&out[rb + irow], xb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
+  __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): x_b's stores have completed
   __syncthreads();
   if (threadIdx.x == 0) __hip_atomic_store(&flags[b], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
