@@ -62,6 +62,7 @@ struct qd_lu_state {
   };
   F16Scratch sc_main, sc_panel;
   F16Scratch sc_solve;  // the inner updates of the main stream's U12 solve, so sc_main keeps L21
+  F16Scratch sc_u12;    // those of the aux stream's U12 solve, which can overlap the main one
   hipStream_t aux = nullptr;     // panel stream of the look-ahead (non-blocking, default priority)
   hipEvent_t ev_cols = nullptr, ev_panel = nullptr;
   hipEvent_t ev_top = nullptr, ev_u12 = nullptr;  // split steps: next U12 rows updated / solved
@@ -1289,6 +1290,15 @@ static double f16_min_work() {
   }();
   return v;
 }
+// QUADESTH_F16_INNER: FP16 for the inner updates of the triangular solves
+// (bit 0) and of the panels (bit 1); default both.
+static int f16_inner() {
+  static const int v = [] {
+    const char* e = std::getenv("QUADESTH_F16_INNER");
+    return e ? std::atoi(e) : 3;
+  }();
+  return v;
+}
 constexpr int kStripMaxRows = 1024;      // LDS of k_trsm_strip<8>: 32 KiB
 constexpr long long kStripNarrowCols = 8192;
 
@@ -1391,7 +1401,7 @@ static int trsm_lower(LuCtx<T>& c, long long r0, int m, long long c0, long long 
   LuCtx<T> cs = c;
   if (c.sc_solve) cs.sc = c.sc_solve;
   rc = gemm_sub(cs, m - m1, c1 - c0, m1, c.W + (r0 + m1) * c.ld + r0, c.W + r0 * c.ld + c0,
-                c.W + (r0 + m1) * c.ld + c0, true);
+                c.W + (r0 + m1) * c.ld + c0, (f16_inner() & 1) != 0);
   if (rc) return rc;
   return trsm_lower(c, r0 + m1, m - m1, c0, c1);
 }
@@ -1465,7 +1475,7 @@ static int panel(LuCtx<T>& c, long long c0, int w, long long p0, long long p1) {
   if ((rc = panel(c, c0, w1, p0, p1))) return rc;
   if ((rc = trsm_lower(c, c0, w1, c0 + w1, c0 + w))) return rc;
   if ((rc = gemm_sub(c, c.N - c0 - w1, w - w1, w1, c.W + (c0 + w1) * c.ld + c0, c.W + c0 * c.ld + c0 + w1,
-                     c.W + (c0 + w1) * c.ld + c0 + w1, true)))
+                     c.W + (c0 + w1) * c.ld + c0 + w1, (f16_inner() & 2) != 0)))
     return rc;
   return panel(c, c0 + w1, w - w1, p0, p1);
 }
@@ -1547,8 +1557,23 @@ static int lookahead_gemm_wgs() {
   return 0;
 }
 
+// Panel widths of a factorisation: `wide` columns while more than `wide_rem`
+// columns remain (wide = 0: never), `nb` after.  A wide panel halves the
+// passes of the trailing update over C where that update hides the panel; the
+// narrow one keeps the late, panel-bound steps short.
+struct PanelWidths {
+  int nb = 1024;
+  int wide = 0;
+  long long wide_rem = 0;
+  int at(long long j, long long N) const {
+    const int w = (wide > nb && N - j > wide_rem) ? wide : nb;
+    return (int)std::min<long long>(w, N - j);
+  }
+  int widest() const { return std::max(nb, wide); }
+};
+
 template <typename T>
-static int factor_steps(LuCtx<T>& c, int NB, bool look);
+static int factor_steps(LuCtx<T>& c, const PanelWidths& pw, bool look);
 
 // Blocked right-looking LU with a look-ahead of one panel: each step solves
 // for and updates the next panel's columns first; that panel is then factored
@@ -1558,8 +1583,9 @@ static int factor_steps(LuCtx<T>& c, int NB, bool look);
 // interchanges.
 // QUADESTH_LOOKAHEAD=0 runs the steps strictly in sequence.
 template <typename T>
-static int factor(LuCtx<T>& c, int NB) {
+static int factor(LuCtx<T>& c, const PanelWidths& pw) {
   qd_lu_state* s = c.s;
+  const int NB = pw.widest();
   const long long N = c.N;
   const size_t maxblocks = (size_t)((N + kColRowsPerBlock - 1) / kColRowsPerBlock) + 1;
   QD_ALLOC(c.h, s->piv_part, maxblocks);
@@ -1587,6 +1613,8 @@ static int factor(LuCtx<T>& c, int NB) {
     QD_ALLOC(c.h, s->growth, 1);
     QD_HIP(c.h, hipMemsetAsync(s->growth.p, 0, sizeof(unsigned), c.st));
   }
+  const char* la = std::getenv("QUADESTH_LOOKAHEAD");
+  const bool look = !(la && la[0] == '0');
   if (c.f16) {
     // sized up front, so that nothing is reallocated while two streams run
     const size_t Kp = (size_t)((NB + 31) / 32 * 32), Kh = (size_t)((NB / 2 + 31) / 32 * 32);
@@ -1599,9 +1627,12 @@ static int factor(LuCtx<T>& c, int NB) {
     QD_ALLOC(c.h, s->sc_solve.h16a, Kh * Kh);
     QD_ALLOC(c.h, s->sc_solve.h16b, (size_t)N * Kh);
     QD_ALLOC(c.h, s->sc_solve.colscale, (size_t)(N / 64 + 2));
+    if (look) {
+      QD_ALLOC(c.h, s->sc_u12.h16a, Kh * Kh);
+      QD_ALLOC(c.h, s->sc_u12.h16b, (size_t)N * Kh);
+      QD_ALLOC(c.h, s->sc_u12.colscale, (size_t)(N / 64 + 2));
+    }
   }
-  const char* la = std::getenv("QUADESTH_LOOKAHEAD");
-  const bool look = !(la && la[0] == '0');
   if (look && !s->aux) {
     QD_HIP(c.h, hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking));
     QD_HIP(c.h, hipEventCreateWithFlags(&s->ev_cols, hipEventDisableTiming));
@@ -1609,11 +1640,11 @@ static int factor(LuCtx<T>& c, int NB) {
     QD_HIP(c.h, hipEventCreateWithFlags(&s->ev_top, hipEventDisableTiming));
     QD_HIP(c.h, hipEventCreateWithFlags(&s->ev_u12, hipEventDisableTiming));
   }
-  return factor_steps(c, NB, look);
+  return factor_steps(c, pw, look);
 }
 
 template <typename T>
-static int factor_steps(LuCtx<T>& c, int NB, bool look) {
+static int factor_steps(LuCtx<T>& c, const PanelWidths& pw, bool look) {
   qd_lu_state* s = c.s;
   const long long N = c.N;
   c.sc = &s->sc_main;
@@ -1627,16 +1658,21 @@ static int factor_steps(LuCtx<T>& c, int NB, bool look) {
   // then solves the next step's U12 (beyond the panel after next) beside the
   // rest of the update, once the next panel is factored.
   const bool split = look && c.diag && c.f16;
-  LuCtx<T> ca = cp;  // the next U12 solve on the aux stream
-  ca.sc_solve = &s->sc_solve;
+  // The next U12 solve on the aux stream.  It can still run when the main
+  // stream starts the next step's solve for the panel after next (whose
+  // inner FP16 updates, with panels wider than the strip solve takes, use
+  // sc_solve), so it has scratch of its own.
+  LuCtx<T> ca = cp;
+  ca.sc_solve = &s->sc_u12;
   bool u12_ready = false;  // this step's U12 beyond the next panel is solved (ev_u12)
   QD_HIP(c.h, hipMemsetAsync(s->piv_state.p, 0, 2 * sizeof(int), c.st));
   int rc;
-  if ((rc = panel_step(c, 0, (int)std::min<long long>(NB, N)))) return rc;
-  for (long long j = 0; j < N; j += NB) {
-    const int jb = (int)std::min<long long>(NB, N - j);
+  if ((rc = panel_step(c, 0, pw.at(0, N)))) return rc;
+  for (long long j = 0, step = 0; j < N; j += step) {
+    const int jb = pw.at(j, N);
+    step = jb;
     const long long nj = j + jb;
-    const int njb = nj < N ? (int)std::min<long long>(NB, N - nj) : 0;
+    const int njb = nj < N ? pw.at(nj, N) : 0;
     // interchanges on the columns left and right of the panel (none with
     // diagonal pivots)
     if (j > 0 && !c.diag)
@@ -1677,7 +1713,7 @@ static int factor_steps(LuCtx<T>& c, int NB, bool look) {
           // this branch too: its rows are final once the first launch above
           // is done, and its L11 once the panel is
           const long long nj2 = nj + njb;
-          const int njb2 = nj2 < N ? (int)std::min<long long>(NB, N - nj2) : 0;
+          const int njb2 = nj2 < N ? pw.at(nj2, N) : 0;
           if (nj2 < N && nj2 + njb2 < N) {
             QD_HIP(c.h, hipStreamWaitEvent(ca.st, s->ev_top, 0));
             if ((rc = trsm_lower(ca, nj, njb, nj2 + njb2, N))) return rc;
@@ -2361,16 +2397,27 @@ static int ensure_factors(qd_context* h, int precision, qd_solve_report* rep) {
   int rc;
   // Panel width: the FP16 trailing update streams its FP32 C once per panel,
   // so it takes a wider panel than the MFMA f64/f32 updates need.  With the
-  // look-ahead, 1024 measured best on MI355X at 76,806 DOF (768-2048 tried).
-  int NB = precision == QD_FP16_IR ? 1024 : 512;
-  if (const char* e = std::getenv("QUADESTH_NB")) NB = std::max(64, std::atoi(e) / 32 * 32);
+  // look-ahead, measured on MI355X at 76,806 DOF: 2048 columns while more
+  // than 30,000 remain (the update hides the panel there), 1024 after; a
+  // factor 8% shorter than 1024 throughout, 1% shorter than 2048 throughout.
+  PanelWidths pw;
+  pw.nb = precision == QD_FP16_IR ? 1024 : 512;
+  if (precision == QD_FP16_IR) {
+    pw.wide = 2048;
+    pw.wide_rem = 30000;
+  }
+  if (const char* e = std::getenv("QUADESTH_NB")) pw.nb = std::max(64, std::atoi(e) / 32 * 32);
+  if (precision == QD_FP16_IR) {
+    if (const char* e = std::getenv("QUADESTH_NB_WIDE")) pw.wide = std::max(0, std::atoi(e) / 32 * 32);
+    if (const char* e = std::getenv("QUADESTH_NB_WIDE_REM")) pw.wide_rem = std::max(0LL, std::atoll(e));
+  }
   if (precision == QD_FP64) {
     s->W32.release();
     QD_ALLOC(h, s->W64, (size_t)ld * (size_t)N);
     hipLaunchKernelGGL(k_transpose_in<double>, tg, dim3(256), 0, h->stream, N, h->A.p, h->lda, s->W64.p, ld);
     QD_HIP(h, hipGetLastError());
     LuCtx<double> c{h, s, s->W64.p, ld, N, false, false, h->stream, &s->sc_main};
-    rc = factor(c, NB);
+    rc = factor(c, pw);
   } else {
     s->W64.release();
     QD_ALLOC(h, s->W32, (size_t)ld * (size_t)N);
@@ -2384,7 +2431,7 @@ static int ensure_factors(qd_context* h, int precision, qd_solve_report* rep) {
       QD_HIP(h, hipGetLastError());
       LuCtx<float> c{h, s, s->W32.p, ld, N, precision == QD_FP16_IR, true, h->stream, &s->sc_main};
       c.diag = diag;
-      rc = factor(c, NB);
+      rc = factor(c, pw);
       if (rc || !diag) break;
       unsigned gb = 0;
       QD_HIP(h, hipMemcpyAsync(&gb, s->growth.p, sizeof(gb), hipMemcpyDeviceToHost, h->stream));
@@ -2804,7 +2851,9 @@ int dgesv_device(qd_context* h, int n, int nrhs, double* a, long long lda, int* 
   hipLaunchKernelGGL(k_transpose_in<double>, tg, dim3(256), 0, h->stream, N, a, lda, s->W64.p, ld);
   QD_HIP(h, hipGetLastError());
   LuCtx<double> c{h, s, s->W64.p, ld, N, false, false, h->stream, &s->sc_main};
-  int rc = factor(c, 512);
+  PanelWidths pw;
+  pw.nb = 512;
+  int rc = factor(c, pw);
   if (rc) return rc;
   std::vector<int> ip(N);
   int st[2];

@@ -166,21 +166,32 @@ HBM_PEAK_GBS = 8000.0
 
 def trailing_shapes(N, precision, diagonal=True):
     """(M, Nn, K) of every outer trailing-update launch of one factorisation:
-    the library's panel width (FP16 1024, else 512; QUADESTH_NB overrides) and
-    its look-ahead split (the next panel's columns, then the rest).  With
-    diagonal pivots the FP16 update of the rest is two launches, the next
-    panel's rows first."""
+    the library's panel widths (FP16: 2048 while more than 30,000 columns
+    remain, then 1024; else 512; QUADESTH_NB, QUADESTH_NB_WIDE and
+    QUADESTH_NB_WIDE_REM override) and its look-ahead split (the next panel's columns, then the
+    rest).  With diagonal pivots the FP16 update of the rest is two launches,
+    the next panel's rows first."""
     nb = int(os.environ.get("QUADESTH_NB", "1024" if precision == "fp16" else "512"))
     nb = max(64, nb // 32 * 32)
+    wide = rem = 0
+    if precision == "fp16":
+        wide = max(0, int(os.environ.get("QUADESTH_NB_WIDE", "2048")) // 32 * 32)
+        rem = max(0, int(os.environ.get("QUADESTH_NB_WIDE_REM", "30000")))
+
+    def width(j):
+        return min(wide if wide > nb and N - j > rem else nb, N - j)
+
     look = os.environ.get("QUADESTH_LOOKAHEAD", "1")[:1] != "0"
     split = look and diagonal and precision == "fp16"
     out = []
-    for j in range(0, N, nb):
-        jb = min(nb, N - j)
+    j = 0
+    while j < N:
+        jb = width(j)
         nj = j + jb
+        j = nj
         if nj >= N:
             break
-        njb = min(nb, N - nj)
+        njb = width(nj)
         if look and nj + njb < N:
             out.append((N - nj, njb, jb))
             if split:
@@ -369,7 +380,7 @@ def main():
                          "c4: adjoint gradient on the perturbed cube")
     ap.add_argument("--fd-dirs", type=int, default=64, help="c4: directions of the finite-difference check")
     ap.add_argument("--trailing-update-only", action="store_true",
-                    help="only the FP16 trailing-update launches (223) of one 76,806-DOF factorisation "
+                    help="only the FP16 trailing-update launches (154) of one 76,806-DOF factorisation "
                          "(qd_fp16_update on synthetic operands), for hardware counter passes")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")

@@ -1,0 +1,37 @@
+"""Panel-width schedules of the mixed-precision factorisation on c1 (18,438 DOF).
+
+With the look-ahead, the aux stream's U12 solve can still run while the main
+stream starts the next step's solve for the panel after next.  Panels wider
+than the strip solve takes (2048) give that main-stream solve inner FP16
+updates, so the two solves need separate scratch: shared scratch corrupts the
+factors (the refinement then needs tens of sweeps instead of a handful).  The
+schedules below cover 1024 throughout, 2048 throughout, and 2048 then 1024."""
+import os
+
+import pytest
+
+from conftest import golden, problem_from_golden, rel_err
+
+pytestmark = pytest.mark.gpu
+
+SCHEDULES = {
+    "narrow": {"QUADESTH_NB": "1024", "QUADESTH_NB_WIDE": "0"},
+    "wide": {"QUADESTH_NB": "2048", "QUADESTH_NB_WIDE": "0"},
+    "wide_then_narrow": {"QUADESTH_NB": "1024", "QUADESTH_NB_WIDE": "2048", "QUADESTH_NB_WIDE_REM": "8192"},
+}
+
+
+@pytest.mark.parametrize("name", list(SCHEDULES))
+def test_fp16_schedule_matches_golden(name, monkeypatch):
+    from bem3d_elastostatic_amd.quadesth import Quadesth
+    for k, v in SCHEDULES[name].items():
+        monkeypatch.setenv(k, v)
+    g = golden("c1")
+    q = Quadesth(problem_from_golden(g))
+    q.ghmat8()
+    print(f"[schedule] c1 fp16 {name}: {SCHEDULES[name]}", flush=True)
+    X, rep = q.solve_linear_system(precision="fp16", tol=1e-14)
+    print(f"[schedule] {name}: {rep}", flush=True)
+    assert rep["converged"] == 1 and rep["gmres"] == 0, rep
+    assert rep["iterations"] <= 10, rep
+    assert rel_err(X, g["X"]) < 1e-10
