@@ -1952,12 +1952,17 @@ __global__ __launch_bounds__(256) void k_block_step(const T* __restrict__ W, lon
 
 // A whole triangular solve with one right-hand side in one launch ("decoupled
 // look-back").  Block row b (b = i forward, nblk-1-i backward for the i-th
-// group of kLbParts workgroups) is shared by kLbParts workgroups: part p
-// accumulates L[b, j] x_j over the dependencies j with j % kLbParts == p, as
-// each x_j is published (a per-block flag set to this call's epoch).  Parts
+// group of kLbParts workgroups) is shared by kLbParts workgroups, which split
+// its dependencies (the d-th earlier block in solve order, d = 0 .. ndep-1)
+// round robin from the newest down: part p takes the d with
+// (ndep-1-d) % kLbParts == kLbParts-1-p, so the last part holds d = ndep-1,
+// the block solved just before.  Each part accumulates L[b, j] x_j as each x_j
+// is published (a per-block flag set to this call's epoch).  Parts
 // 0 .. kLbParts-2 publish their partial sums (flag per part); the last part
-// waits for them, forms x_b = inv_b (y_b - sum) and publishes it.  The long
-// block rows near the end are thus streamed by several workgroups at once.
+// collects them before its final dependency (they depend only on older
+// blocks), then forms x_b = inv_b (y_b - sum) and publishes it: one
+// publication per block on the critical path.  The long block rows near the
+// end are streamed by several workgroups at once.
 // A publication is: the values' stores, a wait for their completion
 // (vmcnt(0): __syncthreads alone waits only for LDS), a barrier, the flag.
 // A workgroup waits only on workgroups with smaller indices (earlier blocks,
@@ -2032,7 +2037,29 @@ __global__ __launch_bounds__(256) void k_trsv_lookback(const T* __restrict__ W, 
     }
   }
   const long long ndep = lower ? b : nblk - 1 - b;
-  for (long long d = pidx; d < ndep; d += kLbParts) {
+  // the other parts' sums (last part, lanes with sub == 0: rows slot + 8 p)
+  T psum[kP];
+#pragma unroll
+  for (int p = 0; p < kP; ++p) psum[p] = (T)0;
+  auto gather = [&]() {
+    if (threadIdx.x == 0)
+      for (int q = 0; q < kLbParts - 1; ++q) lb_wait(pflags, b * kLbParts + q, epoch, err);
+    __syncthreads();
+    if (sub == 0) {
+#pragma unroll
+      for (int p = 0; p < kP; ++p) {
+        const int r = slot + 8 * p;
+        T sum = (T)0;
+        for (int q = 0; q < kLbParts - 1; ++q)
+          sum += __hip_atomic_load(&part[((size_t)b * kLbParts + q) * TB + r], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        psum[p] = sum;
+      }
+    }
+  };
+  bool gathered = false;
+  const long long d0 = ((ndep - kLbParts + pidx) % kLbParts + kLbParts) % kLbParts;
+  for (long long d = d0; d < ndep; d += kLbParts) {
     const long long j = lower ? d : nblk - 1 - d;
     const long long cj = j * TB;
     const int mj = (int)min((long long)TB, N - cj);
@@ -2062,6 +2089,10 @@ __global__ __launch_bounds__(256) void k_trsv_lookback(const T* __restrict__ W, 
             if (col + e < mj) w[p][e] = src[e];
         }
       }
+    }
+    if (last && d + kLbParts >= ndep) {
+      gather();
+      gathered = true;
     }
     if (threadIdx.x == 0) lb_wait(flags, j, epoch, err);
     __syncthreads();
@@ -2100,20 +2131,13 @@ __global__ __launch_bounds__(256) void k_trsv_lookback(const T* __restrict__ W, 
       __hip_atomic_store(&pflags[b * kLbParts + pidx], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return;
   }
-  // the last part: the other parts' sums, then the block's right-hand side
-  if (threadIdx.x == 0)
-    for (int q = 0; q < kLbParts - 1; ++q) lb_wait(pflags, b * kLbParts + q, epoch, err);
-  __syncthreads();
+  // the last part: the other parts' sums (collected before the final
+  // dependency, or here when there was none), then the block's right-hand side
+  if (!gathered) gather();
 #pragma unroll
   for (int p = 0; p < kP; ++p) {
     const int r = slot + 8 * p;
-    if (sub == 0) {
-      T sum = rs[p];
-      for (int q = 0; q < kLbParts - 1; ++q)
-        sum += __hip_atomic_load(&part[((size_t)b * kLbParts + q) * TB + r], __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_AGENT);
-      sres[r] = (r < m) ? y[rb + r] - sum : (T)0;
-    }
+    if (sub == 0) sres[r] = (r < m) ? y[rb + r] - (rs[p] + psum[p]) : (T)0;
   }
   __syncthreads();
   {
