@@ -78,6 +78,7 @@ struct qd_lu_state {
   qd::DevBuf<unsigned char> trs_tmp;  // intermediate vector of the block triangular solves
   qd::DevBuf<int> lb_flags;      // error word, then per-block and per-part flags of the look-back solves
   qd::DevBuf<unsigned char> lb_part;  // look-back solves: partial sums of the parts of each block row
+  qd::DevBuf<unsigned long long> lb_pub;  // FP32 look-back solves: (epoch << 32) | bits of each x value
   int lb_epoch = 0;              // value the flags take in the latest look-back solve
   qd::DevBuf<float> ts_linv;     // inverses of the leaves' unit lower blocks (32 x 32 each)
   qd::DevBuf<unsigned> growth;   // diagonal-pivot factorisations: largest |l| (float bits)
@@ -1963,8 +1964,11 @@ __global__ __launch_bounds__(256) void k_block_step(const T* __restrict__ W, lon
 // blocks), then forms x_b = inv_b (y_b - sum) and publishes it: one
 // publication per block on the critical path.  The long block rows near the
 // end are streamed by several workgroups at once.
-// A publication is: the values' stores, a wait for their completion
-// (vmcnt(0): __syncthreads alone waits only for LDS), a barrier, the flag.
+// A publication of partial sums is: the values' stores, a wait for their
+// completion (vmcnt(0): __syncthreads alone waits only for LDS), a barrier,
+// the flag.  For FP32, x_b is published in pub as 64-bit words (epoch in the
+// high half, the value's bits in the low half), which readers poll directly:
+// no flag, and no wait for the stores.  FP64 values take the flag route.
 // A workgroup waits only on workgroups with smaller indices (earlier blocks,
 // or earlier parts of its own block), which are dispatched before it.  The
 // waits are bounded: after kLbMaxSpins polls (about 0.1 s) one gives up and
@@ -1995,11 +1999,15 @@ __device__ __forceinline__ void lb_wait(const int* flags, long long i, int epoch
 template <typename T, int TB>
 __global__ __launch_bounds__(256) void k_trsv_lookback(const T* __restrict__ W, long long ld, long long N, int lower,
                                                        const T* __restrict__ inv, const T* __restrict__ y,
-                                                       T* out, T* part, int* flags, int epoch, int* err) {
+                                                       T* out, T* part, int* flags, unsigned long long* pub,
+                                                       int epoch, int* err) {
   constexpr int E = 16 / sizeof(T);  // elements of a lane's row piece
   constexpr int kP = TB / 8;          // rows per thread (8 rows per pass)
+  constexpr bool kPacked = sizeof(T) == 4;
   static_assert(E * 32 == TB, "row piece layout");
   __shared__ T sres[TB];
+  __shared__ T sxj[2][TB];  // packed route: dependency values, double-buffered
+  __shared__ T sy[TB];      // last part: the block's right-hand side, loaded at the start
   const long long nblk = (N + TB - 1) / TB;
   const long long grp = (long long)blockIdx.x / kLbParts;
   const int pidx = (int)(blockIdx.x % kLbParts);
@@ -2035,6 +2043,8 @@ __global__ __launch_bounds__(256) void k_trsv_lookback(const T* __restrict__ W, 
         pinv[k0 + 1] = q.y;
       }
     }
+    // y_b now, off the critical path (read after a later barrier)
+    if (threadIdx.x < TB) sy[threadIdx.x] = ((int)threadIdx.x < m) ? y[rb + threadIdx.x] : (T)0;
   }
   const long long ndep = lower ? b : nblk - 1 - b;
   // the other parts' sums (last part, lanes with sub == 0: rows slot + 8 p)
@@ -2059,6 +2069,7 @@ __global__ __launch_bounds__(256) void k_trsv_lookback(const T* __restrict__ W, 
   };
   bool gathered = false;
   const long long d0 = ((ndep - kLbParts + pidx) % kLbParts + kLbParts) % kLbParts;
+  int buf = 0;
   for (long long d = d0; d < ndep; d += kLbParts) {
     const long long j = lower ? d : nblk - 1 - d;
     const long long cj = j * TB;
@@ -2094,13 +2105,44 @@ __global__ __launch_bounds__(256) void k_trsv_lookback(const T* __restrict__ W, 
       gather();
       gathered = true;
     }
-    if (threadIdx.x == 0) lb_wait(flags, j, epoch, err);
-    __syncthreads();
     T xj[E];
+    if constexpr (kPacked) {
+      // the first 32 lanes poll x_j's words (bounded), then share them in LDS;
+      // the buffer alternates, so one barrier per dependency suffices
+      if (slot == 0) {
+        unsigned long long v[E];
+        int spins = 0;
+        for (;;) {
+          bool ok = true;
 #pragma unroll
-    for (int e = 0; e < E; ++e)
-      xj[e] = (col + e < mj) ? __hip_atomic_load(&out[cj + col + e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                             : (T)0;
+          for (int e = 0; e < E; ++e) {
+            v[e] = (col + e < mj) ? __hip_atomic_load(&pub[cj + col + e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                  : ((unsigned long long)(unsigned)epoch << 32);
+            ok = ok && (unsigned)(v[e] >> 32) == (unsigned)epoch;
+          }
+          if (ok) break;
+          if (++spins > kLbMaxSpins) {
+            atomicExch(err, epoch);
+            break;
+          }
+          if ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch) break;
+          __builtin_amdgcn_s_sleep(1);
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e) sxj[buf][col + e] = (col + e < mj) ? __uint_as_float((unsigned)v[e]) : 0.0f;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int e = 0; e < E; ++e) xj[e] = sxj[buf][col + e];
+      buf ^= 1;
+    } else {
+      if (threadIdx.x == 0) lb_wait(flags, j, epoch, err);
+      __syncthreads();
+#pragma unroll
+      for (int e = 0; e < E; ++e)
+        xj[e] = (col + e < mj) ? __hip_atomic_load(&out[cj + col + e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                               : (T)0;
+    }
 #pragma unroll
     for (int p = 0; p < kP; ++p)
 #pragma unroll
@@ -2137,7 +2179,7 @@ __global__ __launch_bounds__(256) void k_trsv_lookback(const T* __restrict__ W, 
 #pragma unroll
   for (int p = 0; p < kP; ++p) {
     const int r = slot + 8 * p;
-    if (sub == 0) sres[r] = (r < m) ? y[rb + r] - (rs[p] + psum[p]) : (T)0;
+    if (sub == 0) sres[r] = (r < m) ? sy[r] - (rs[p] + psum[p]) : (T)0;
   }
   __syncthreads();
   {
@@ -2146,7 +2188,17 @@ __global__ __launch_bounds__(256) void k_trsv_lookback(const T* __restrict__ W, 
     for (int k = 0; k < kIp; ++k) xb += pinv[k] * sres[ipart * kIp + k];
 #pragma unroll
     for (int off = 1; off < kParts; off <<= 1) xb += __shfl_xor(xb, off);
-    if (ipart == 0 && irow < m) __hip_atomic_store(&out[rb + irow], xb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if constexpr (kPacked) {
+      if (ipart == 0 && irow < m) {
+        const unsigned long long w = ((unsigned long long)(unsigned)epoch << 32) | __float_as_uint(xb);
+        __hip_atomic_store(&pub[rb + irow], w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        out[rb + irow] = xb;
+      }
+      return;
+    } else {
+      if (ipart == 0 && irow < m)
+        __hip_atomic_store(&out[rb + irow], xb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
   __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): x_b's stores have completed
   __syncthreads();
@@ -2194,16 +2246,24 @@ static int tri_solve_inv(qd_context* h, const T* W, long long ld, long long N, i
       QD_HIP(h, hipMemsetAsync(s->lb_flags.p, 0, s->lb_flags.count * sizeof(int), h->stream));
     }
     QD_ALLOC(h, s->lb_part, (size_t)nblk * kLbParts * TB * sizeof(T));
+    unsigned long long* pub = nullptr;
+    if (sizeof(T) == 4) {
+      if ((long long)s->lb_pub.count < nblk * TB) {
+        QD_ALLOC(h, s->lb_pub, (size_t)(nblk * TB));
+        QD_HIP(h, hipMemsetAsync(s->lb_pub.p, 0, s->lb_pub.count * sizeof(unsigned long long), h->stream));
+      }
+      pub = s->lb_pub.p;
+    }
     T* part = reinterpret_cast<T*>(s->lb_part.p);
     int* err = s->lb_flags.p;  // the epoch of a call in which a wait gave up
     int* flags = s->lb_flags.p + 1;
     int ep = ++s->lb_epoch;
     const dim3 g((unsigned)(nblk * kLbParts));
     hipLaunchKernelGGL((k_trsv_lookback<T, TB>), g, dim3(256), 0, h->stream, W, ld, N, 1, invL, (const T*)x, tmp, part,
-                       flags, ep, err);
+                       flags, pub, ep, err);
     ep = ++s->lb_epoch;
     hipLaunchKernelGGL((k_trsv_lookback<T, TB>), g, dim3(256), 0, h->stream, W, ld, N, 0, invU, (const T*)tmp, x, part,
-                       flags, ep, err);
+                       flags, pub, ep, err);
     QD_HIP(h, hipGetLastError());
     return QD_OK;
   }
