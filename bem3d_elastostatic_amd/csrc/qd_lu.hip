@@ -2421,6 +2421,41 @@ __global__ __launch_bounds__(256) void k_resid_part(long long N, const double* _
   }
   for (int v = 0; v < nrhs; ++v) part[((long long)blockIdx.y * nrhs + v) * N + i] = s[v];
 }
+// One right-hand side: two rows per lane (16-byte loads of an even row pair,
+// A 16-byte aligned with an even lda) and eight columns in flight.  Each row
+// sums its columns in the same order as k_resid_part.
+__global__ __launch_bounds__(256) void k_resid1_part(long long N, const double* __restrict__ A, long long lda,
+                                                     const double* __restrict__ x, double* __restrict__ part) {
+  const long long i = ((long long)blockIdx.x * 256 + threadIdx.x) * 2;
+  const long long c0 = (long long)blockIdx.y * kResSlice;
+  const long long c1 = std::min<long long>(N, c0 + kResSlice);
+  if (i >= N) return;
+  const bool pair = i + 1 < N;
+  double s0 = 0.0, s1 = 0.0;
+  long long c = c0;
+  for (; c + 8 <= c1; c += 8) {
+    double2 a[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const double* src = A + (c + u) * lda + i;
+      a[u] = pair ? *reinterpret_cast<const double2*>(src) : make_double2(src[0], 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const double xc = x[c + u];
+      s0 += a[u].x * xc;
+      s1 += a[u].y * xc;
+    }
+  }
+  for (; c < c1; ++c) {
+    const double* src = A + c * lda + i;
+    const double xc = x[c];
+    s0 += src[0] * xc;
+    if (pair) s1 += src[1] * xc;
+  }
+  part[(long long)blockIdx.y * N + i] = s0;
+  if (pair) part[(long long)blockIdx.y * N + i + 1] = s1;
+}
 // transposed: one wave per output i over a column of A (contiguous)
 __global__ __launch_bounds__(256) void k_residT_part(long long N, const double* __restrict__ A, long long lda,
                                                      int nrhs, const double* __restrict__ x, long long ldx,
@@ -2564,7 +2599,10 @@ static int residual(qd_context* h, bool trans, int nrhs, const double* x, long l
   const long long N = s->N;
   const int nslice = trans ? 1 : (int)((N + kResSlice - 1) / kResSlice);
   QD_ALLOC(h, s->rowpart, (size_t)nslice * nrhs * N);
-  if (!trans)
+  if (!trans && nrhs == 1 && h->lda % 2 == 0 && (reinterpret_cast<uintptr_t>(h->A.p) & 15) == 0)
+    hipLaunchKernelGGL(k_resid1_part, dim3((unsigned)((N + 511) / 512), nslice), dim3(256), 0, h->stream, N, h->A.p,
+                       h->lda, x, s->rowpart.p);
+  else if (!trans)
     hipLaunchKernelGGL(k_resid_part, dim3((unsigned)((N + 255) / 256), nslice), dim3(256), 0, h->stream, N, h->A.p,
                        h->lda, nrhs, x, ldx, s->rowpart.p);
   else
