@@ -112,9 +112,10 @@ typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 // layout conversion: col-major double A (lda) -> row-major T W (ldw)
 template <typename T>
 __global__ __launch_bounds__(256) void k_transpose_in(long long N, const double* __restrict__ A,
-                                                      long long lda, T* __restrict__ W, long long ldw) {
+                                                      long long lda, T* __restrict__ W, long long ldw,
+                                                      long long ct0 = 0) {
   __shared__ double tile[32][33];
-  const long long c0 = (long long)blockIdx.x * 32, r0 = (long long)blockIdx.y * 32;
+  const long long c0 = ((long long)blockIdx.x + ct0) * 32, r0 = (long long)blockIdx.y * 32;
   const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;  // 32 x 8
   for (int k = ty; k < 32; k += 8) {
     long long r = r0 + tx, c = c0 + k;
@@ -1256,6 +1257,8 @@ struct LuCtx {
   qd_lu_state::F16Scratch* sc = nullptr;      // FP16 operand copies of this context
   qd_lu_state::F16Scratch* sc_solve = nullptr;  // those of trsm_lower's inner updates (nullptr: sc)
   bool diag = false;  // with calu: the leaves take their diagonal pivots instead (checked by growth)
+  const double* src = nullptr;  // column-major FP64 A that the factorisation first converts into W
+  long long src_ld = 0;
 };
 
 // The FP16 update C -= (A16 B16^T) * colscale on stream st; max_wgs > 0
@@ -1566,11 +1569,14 @@ struct PanelWidths {
   int nb = 1024;
   int wide = 0;
   long long wide_rem = 0;
+  int tail = 0;  // columns of the panels once at most tail_rem remain (0: nb)
+  long long tail_rem = 0;
   int at(long long j, long long N) const {
-    const int w = (wide > nb && N - j > wide_rem) ? wide : nb;
-    return (int)std::min<long long>(w, N - j);
+    const long long rem = N - j;
+    const int w = (wide > nb && rem > wide_rem) ? wide : (tail > 0 && rem <= tail_rem) ? tail : nb;
+    return (int)std::min<long long>(w, rem);
   }
-  int widest() const { return std::max(nb, wide); }
+  int widest() const { return std::max(std::max(nb, wide), tail); }
 };
 
 template <typename T>
@@ -1668,7 +1674,31 @@ static int factor_steps(LuCtx<T>& c, const PanelWidths& pw, bool look) {
   bool u12_ready = false;  // this step's U12 beyond the next panel is solved (ev_u12)
   QD_HIP(c.h, hipMemsetAsync(s->piv_state.p, 0, 2 * sizeof(int), c.st));
   int rc;
-  if ((rc = panel_step(c, 0, pw.at(0, N)))) return rc;
+  const int jb0 = pw.at(0, N);
+  if (c.src) {
+    // W from A: the first panel's columns, then the rest beside that panel
+    // (on the aux stream with the look-ahead; the two touch disjoint columns)
+    const unsigned nt = (unsigned)((N + 31) / 32), ct = std::min(nt, (unsigned)((jb0 + 31) / 32));
+    hipLaunchKernelGGL(k_transpose_in<T>, dim3(ct, nt), dim3(256), 0, c.st, N, c.src, c.src_ld, c.W, c.ld, 0LL);
+    QD_HIP(c.h, hipGetLastError());
+    if (look) {
+      QD_HIP(c.h, hipEventRecord(s->ev_cols, c.st));
+      QD_HIP(c.h, hipStreamWaitEvent(cp.st, s->ev_cols, 0));
+      if ((rc = panel_step(cp, 0, jb0))) return rc;
+      QD_HIP(c.h, hipEventRecord(s->ev_panel, cp.st));
+    }
+    if (nt > ct)
+      hipLaunchKernelGGL(k_transpose_in<T>, dim3(nt - ct, nt), dim3(256), 0, c.st, N, c.src, c.src_ld, c.W, c.ld,
+                         (long long)ct);
+    QD_HIP(c.h, hipGetLastError());
+    if (look) {
+      QD_HIP(c.h, hipStreamWaitEvent(c.st, s->ev_panel, 0));
+    } else if ((rc = panel_step(c, 0, jb0))) {
+      return rc;
+    }
+  } else if ((rc = panel_step(c, 0, jb0))) {
+    return rc;
+  }
   for (long long j = 0, step = 0; j < N; j += step) {
     const int jb = pw.at(j, N);
     step = jb;
@@ -2529,6 +2559,8 @@ static int ensure_factors(qd_context* h, int precision, qd_solve_report* rep) {
   if (precision == QD_FP16_IR) {
     if (const char* e = std::getenv("QUADESTH_NB_WIDE")) pw.wide = std::max(0, std::atoi(e) / 32 * 32);
     if (const char* e = std::getenv("QUADESTH_NB_WIDE_REM")) pw.wide_rem = std::max(0LL, std::atoll(e));
+    if (const char* e = std::getenv("QUADESTH_NB_TAIL")) pw.tail = std::max(0, std::atoi(e) / 32 * 32);
+    if (const char* e = std::getenv("QUADESTH_NB_TAIL_REM")) pw.tail_rem = std::max(0LL, std::atoll(e));
   }
   if (precision == QD_FP64) {
     s->W32.release();
@@ -2546,10 +2578,11 @@ static int ensure_factors(qd_context* h, int precision, qd_solve_report* rep) {
     // the factorisation is redone with tournament pivoting.
     bool diag = diag_pivots_allowed();
     for (int attempt = 0; attempt < 2; ++attempt) {
-      hipLaunchKernelGGL(k_transpose_in<float>, tg, dim3(256), 0, h->stream, N, h->A.p, h->lda, s->W32.p, ld);
-      QD_HIP(h, hipGetLastError());
+      // factor_steps converts A into W32, beside the first panel
       LuCtx<float> c{h, s, s->W32.p, ld, N, precision == QD_FP16_IR, true, h->stream, &s->sc_main};
       c.diag = diag;
+      c.src = h->A.p;
+      c.src_ld = h->lda;
       rc = factor(c, pw);
       if (rc || !diag) break;
       unsigned gb = 0;
