@@ -177,9 +177,16 @@ def trailing_shapes(N, precision, diagonal=True):
     if precision == "fp16":
         wide = max(0, int(os.environ.get("QUADESTH_NB_WIDE", "2048")) // 32 * 32)
         rem = max(0, int(os.environ.get("QUADESTH_NB_WIDE_REM", "30000")))
+    tail = tail_rem = 0
+    if precision == "fp16":
+        tail = max(0, int(os.environ.get("QUADESTH_NB_TAIL", "0")) // 32 * 32)
+        tail_rem = max(0, int(os.environ.get("QUADESTH_NB_TAIL_REM", "0")))
 
     def width(j):
-        return min(wide if wide > nb and N - j > rem else nb, N - j)
+        left = N - j
+        if wide > nb and left > rem:
+            return min(wide, left)
+        return min(tail if tail > 0 and left <= tail_rem else nb, left)
 
     look = os.environ.get("QUADESTH_LOOKAHEAD", "1")[:1] != "0"
     split = look and diagonal and precision == "fp16"
