@@ -2114,7 +2114,9 @@ __global__ __launch_bounds__(256) void k_trsv_lookback(const T* __restrict__ W, 
       if (r < m) {
         if (col + E <= mj) {
           if constexpr (sizeof(T) == 4) {
-            const float4 q = *reinterpret_cast<const float4*>(src);
+            // each L/U entry is read once per sweep: non-temporal
+            typedef float f4v __attribute__((ext_vector_type(4)));
+            const f4v q = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(src));
             w[p][0] = q.x;
             w[p][1] = q.y;
             w[p][2] = q.z;
@@ -2453,7 +2455,8 @@ __global__ __launch_bounds__(256) void k_resid_part(long long N, const double* _
 }
 // One right-hand side: two rows per lane (16-byte loads of an even row pair,
 // A 16-byte aligned with an even lda) and eight columns in flight.  Each row
-// sums its columns in the same order as k_resid_part.
+// sums its columns in the same order as k_resid_part.  A is read once per
+// sweep, so its loads are non-temporal.
 __global__ __launch_bounds__(256) void k_resid1_part(long long N, const double* __restrict__ A, long long lda,
                                                      const double* __restrict__ x, double* __restrict__ part) {
   const long long i = ((long long)blockIdx.x * 256 + threadIdx.x) * 2;
@@ -2468,7 +2471,13 @@ __global__ __launch_bounds__(256) void k_resid1_part(long long N, const double* 
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const double* src = A + (c + u) * lda + i;
-      a[u] = pair ? *reinterpret_cast<const double2*>(src) : make_double2(src[0], 0.0);
+      if (pair) {
+        typedef double d2v __attribute__((ext_vector_type(2)));
+        const d2v v = __builtin_nontemporal_load(reinterpret_cast<const d2v*>(src));
+        a[u] = make_double2(v.x, v.y);
+      } else {
+        a[u] = make_double2(__builtin_nontemporal_load(src), 0.0);
+      }
     }
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
