@@ -550,19 +550,18 @@ __device__ __forceinline__ void dw_load_row(const float* __restrict__ W, long lo
 // One step (column K) of the top block's elimination: lane K's row is final
 // and goes to LDS (slot K & 1, so one barrier per step suffices: a lane
 // writing slot K & 1 again at K + 2 has passed the barrier of step K + 1,
-// which every lane reaches only after reading slot K & 1) and to lu_out.
+// which every lane reaches only after reading slot K & 1).  Rows go to memory
+// after the last step, all lanes at once (row stores inside the steps, 16 per
+// step from one lane, fill the store queue and stall the elimination).
 template <int K>
-__device__ __forceinline__ void dw_step(float (&a)[kDW], int w, float (*srow)[kDW], float* __restrict__ lu_out,
-                                        float& gmax) {
+__device__ __forceinline__ void dw_step(float (&a)[kDW], int w, float (*srow)[kDW], float& gmax) {
   if (K < w) {
     const int lane = threadIdx.x;
     if (lane == K) {
 #pragma unroll
-      for (int j4 = 0; j4 < kDW / 4; ++j4) {
-        const float4 v = make_float4(a[4 * j4], a[4 * j4 + 1], a[4 * j4 + 2], a[4 * j4 + 3]);
-        *reinterpret_cast<float4*>(&srow[K & 1][4 * j4]) = v;
-        *reinterpret_cast<float4*>(lu_out + K * kDW + 4 * j4) = v;
-      }
+      for (int j4 = 0; j4 < kDW / 4; ++j4)
+        *reinterpret_cast<float4*>(&srow[K & 1][4 * j4]) =
+            make_float4(a[4 * j4], a[4 * j4 + 1], a[4 * j4 + 2], a[4 * j4 + 3]);
     }
     __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the row is in LDS
     __builtin_amdgcn_wave_barrier();
@@ -589,8 +588,8 @@ __device__ __forceinline__ void dw_step(float (&a)[kDW], int w, float (*srow)[kD
 
 template <int... Ks>
 __device__ __forceinline__ void dw_all(std::integer_sequence<int, Ks...>, float (&a)[kDW], int w,
-                                       float (*srow)[kDW], float* __restrict__ lu_out, float& gmax) {
-  (dw_step<Ks>(a, w, srow, lu_out, gmax), ...);
+                                       float (*srow)[kDW], float& gmax) {
+  (dw_step<Ks>(a, w, srow, gmax), ...);
 }
 
 // The top w x w block (w <= 64) of a diagonal-pivot leaf, one wave: its LU
@@ -612,7 +611,13 @@ __global__ __launch_bounds__(64) void k_leaf_diag(const float* __restrict__ W, l
 #pragma unroll
     for (int j = 0; j < kDW; ++j) a[j] = 0.0f;
   float gmax = 0.0f;
-  dw_all(std::make_integer_sequence<int, kDW>{}, a, w, srow, lu_out, gmax);
+  dw_all(std::make_integer_sequence<int, kDW>{}, a, w, srow, gmax);
+  // lane r's registers hold row r of the LU (multipliers left of the diagonal)
+  if (lane < w)
+#pragma unroll
+    for (int j4 = 0; j4 < kDW / 4; ++j4)
+      *reinterpret_cast<float4*>(lu_out + lane * kDW + 4 * j4) =
+          make_float4(a[4 * j4], a[4 * j4 + 1], a[4 * j4 + 2], a[4 * j4 + 3]);
   ts_growth_max(gmax, growth);
   if (lane < w) ipiv[c0 + lane] = (int)c0 + lane;
   // the unit lower factor (identity beyond w) for the inverses: lane r's
