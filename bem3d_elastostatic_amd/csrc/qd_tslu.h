@@ -548,48 +548,33 @@ __device__ __forceinline__ void dw_load_row(const float* __restrict__ W, long lo
 }
 
 // One step (column K) of the top block's elimination: lane K's row is final
-// and goes to LDS (slot K & 1, so one barrier per step suffices: a lane
-// writing slot K & 1 again at K + 2 has passed the barrier of step K + 1,
-// which every lane reaches only after reading slot K & 1).  Rows go to memory
-// after the last step, all lanes at once (row stores inside the steps, 16 per
-// step from one lane, fill the store queue and stall the elimination).
+// and reaches the other lanes by readlane broadcasts (no LDS round trip per
+// step).  Rows go to memory after the last step, all lanes at once.
 template <int K>
-__device__ __forceinline__ void dw_step(float (&a)[kDW], int w, float (*srow)[kDW], float& gmax) {
+__device__ __forceinline__ void dw_step(float (&a)[kDW], int w, float& gmax) {
   if (K < w) {
     const int lane = threadIdx.x;
-    if (lane == K) {
-#pragma unroll
-      for (int j4 = 0; j4 < kDW / 4; ++j4)
-        *reinterpret_cast<float4*>(&srow[K & 1][4 * j4]) =
-            make_float4(a[4 * j4], a[4 * j4 + 1], a[4 * j4 + 2], a[4 * j4 + 3]);
-    }
-    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the row is in LDS
-    __builtin_amdgcn_wave_barrier();
-    const float piv = srow[K & 1][K];
+    const float piv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a[K]), K));
     if (piv == 0.0f) gmax = __int_as_float(0x7f800000);
     const float rp = (piv != 0.0f) ? __builtin_amdgcn_rcpf(piv) : 0.0f;
-    if (lane > K && lane < w) {
-      const float l = a[K] * rp;
+    const bool act = lane > K && lane < w;
+    const float l = a[K] * rp;
+    if (act) {
       a[K] = l;
       gmax = fmaxf(gmax, fabsf(l));
-      const float4* r4 = reinterpret_cast<const float4*>(&srow[K & 1][0]);
+    }
 #pragma unroll
-      for (int j4 = (K + 1) / 4; j4 < kDW / 4; ++j4) {
-        const float4 v = r4[j4];
-        if (4 * j4 > K) a[4 * j4] -= l * v.x;
-        if (4 * j4 + 1 > K) a[4 * j4 + 1] -= l * v.y;
-        if (4 * j4 + 2 > K) a[4 * j4 + 2] -= l * v.z;
-        if (4 * j4 + 3 > K) a[4 * j4 + 3] -= l * v.w;
-      }
+    for (int j = K + 1; j < kDW; ++j) {
+      const float u = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a[j]), K));
+      if (act) a[j] -= l * u;
     }
   }
   __builtin_amdgcn_sched_barrier(0);
 }
 
 template <int... Ks>
-__device__ __forceinline__ void dw_all(std::integer_sequence<int, Ks...>, float (&a)[kDW], int w,
-                                       float (*srow)[kDW], float& gmax) {
-  (dw_step<Ks>(a, w, srow, gmax), ...);
+__device__ __forceinline__ void dw_all(std::integer_sequence<int, Ks...>, float (&a)[kDW], int w, float& gmax) {
+  (dw_step<Ks>(a, w, gmax), ...);
 }
 
 // The top w x w block (w <= 64) of a diagonal-pivot leaf, one wave: its LU
@@ -601,7 +586,6 @@ __global__ __launch_bounds__(64) void k_leaf_diag(const float* __restrict__ W, l
                                                   long long c0, int w, float* __restrict__ lu_out,
                                                   float* __restrict__ linv_out, int* __restrict__ ipiv,
                                                   unsigned* __restrict__ growth) {
-  __shared__ __attribute__((aligned(16))) float srow[2][kDW];
   __shared__ float sL[kDW][kDW + 1];
   __builtin_amdgcn_s_setprio(3);
   const int lane = threadIdx.x;
@@ -611,7 +595,7 @@ __global__ __launch_bounds__(64) void k_leaf_diag(const float* __restrict__ W, l
 #pragma unroll
     for (int j = 0; j < kDW; ++j) a[j] = 0.0f;
   float gmax = 0.0f;
-  dw_all(std::make_integer_sequence<int, kDW>{}, a, w, srow, gmax);
+  dw_all(std::make_integer_sequence<int, kDW>{}, a, w, gmax);
   // lane r's registers hold row r of the LU (multipliers left of the diagonal)
   if (lane < w)
 #pragma unroll
