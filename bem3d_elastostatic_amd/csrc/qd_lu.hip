@@ -1681,17 +1681,17 @@ static int factor_steps(LuCtx<T>& c, const PanelWidths& pw, bool look) {
     const unsigned nt = (unsigned)((N + 31) / 32), ct = std::min(nt, (unsigned)((jb0 + 31) / 32));
     hipLaunchKernelGGL(k_transpose_in<T>, dim3(ct, nt), dim3(256), 0, c.st, N, c.src, c.src_ld, c.W, c.ld, 0LL);
     QD_HIP(c.h, hipGetLastError());
-    if (look) {
-      QD_HIP(c.h, hipEventRecord(s->ev_cols, c.st));
-      QD_HIP(c.h, hipStreamWaitEvent(cp.st, s->ev_cols, 0));
-      if ((rc = panel_step(cp, 0, jb0))) return rc;
-      QD_HIP(c.h, hipEventRecord(s->ev_panel, cp.st));
-    }
+    if (look) QD_HIP(c.h, hipEventRecord(s->ev_cols, c.st));
+    // the rest is enqueued before the panel's many small launches, which
+    // would hold the host up while the main stream idles
     if (nt > ct)
       hipLaunchKernelGGL(k_transpose_in<T>, dim3(nt - ct, nt), dim3(256), 0, c.st, N, c.src, c.src_ld, c.W, c.ld,
                          (long long)ct);
     QD_HIP(c.h, hipGetLastError());
     if (look) {
+      QD_HIP(c.h, hipStreamWaitEvent(cp.st, s->ev_cols, 0));
+      if ((rc = panel_step(cp, 0, jb0))) return rc;
+      QD_HIP(c.h, hipEventRecord(s->ev_panel, cp.st));
       QD_HIP(c.h, hipStreamWaitEvent(c.st, s->ev_panel, 0));
     } else if ((rc = panel_step(c, 0, jb0))) {
       return rc;
