@@ -86,6 +86,8 @@ struct qd_lu_state {
   qd::DevBuf<float> inv32;       // inverted diagonal blocks of L and U
   qd::DevBuf<double> inv64;
   double anorm = -1.0;           // ||A||_inf of the FP64 matrix
+  qd::DevBuf<double> rowabs;     // row sums of |A| per 512-column slice, from A's conversion into W32
+  int rowabs_nsl = 0;            // slices in rowabs for the held factors (0: none)
   ~qd_lu_state() {
     if (ev_cols) (void)hipEventDestroy(ev_cols);
     if (ev_panel) (void)hipEventDestroy(ev_panel);
@@ -126,6 +128,56 @@ __global__ __launch_bounds__(256) void k_transpose_in(long long N, const double*
     long long r = r0 + k, c = c0 + tx;
     if (r < N && c < N) W[r * ldw + c] = (T)tile[tx][k];
   }
+}
+
+// k_transpose_in over whole 512-column slices (kAbsTiles tiles of 32 columns
+// per workgroup, from column tile ct0, a multiple of kAbsTiles), also writing
+// each row's sum of |A| over the slice to rowabs[slice * N + row]: the
+// refinement's ||A||_inf without a pass of its own over A.
+constexpr int kAbsTiles = 16;
+template <typename T>
+__global__ __launch_bounds__(256) void k_transpose_abs(long long N, const double* __restrict__ A, long long lda,
+                                                       T* __restrict__ W, long long ldw, long long ct0,
+                                                       double* __restrict__ rowabs) {
+  __shared__ double tile[32][33];
+  __shared__ double red[8][33];
+  const long long r0 = (long long)blockIdx.y * 32;
+  const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;  // 32 x 8
+  const long long t0 = ct0 + (long long)blockIdx.x * kAbsTiles;
+  double acc = 0.0;
+  for (int t = 0; t < kAbsTiles; ++t) {
+    const long long c0 = (t0 + t) * 32;
+    if (c0 >= N) break;
+    for (int k = ty; k < 32; k += 8) {
+      const long long r = r0 + tx, c = c0 + k;
+      const double v = (r < N && c < N) ? A[c * lda + r] : 0.0;
+      tile[k][tx] = v;  // tile[col][row]
+      acc += fabs(v);
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+      const long long r = r0 + k, c = c0 + tx;
+      if (r < N && c < N) W[r * ldw + c] = (T)tile[tx][k];
+    }
+    __syncthreads();
+  }
+  red[ty][tx] = acc;
+  __syncthreads();
+  if (ty == 0 && r0 + tx < N) {
+    double sum = 0.0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) sum += red[j][tx];
+    rowabs[(t0 / kAbsTiles) * N + r0 + tx] = sum;
+  }
+}
+
+// rowabs slices summed: out[r] = sum over slices of rowabs[slice * N + r].
+__global__ void k_rowabs_sum(long long N, int nsl, const double* __restrict__ rowabs, double* __restrict__ out) {
+  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= N) return;
+  double s = 0.0;
+  for (int q = 0; q < nsl; ++q) s += rowabs[(long long)q * N + r];
+  out[r] = s;
 }
 
 // row-major double W -> col-major double A (LAPACK output of dgesv_)
@@ -1678,15 +1730,22 @@ static int factor_steps(LuCtx<T>& c, const PanelWidths& pw, bool look) {
   if (c.src) {
     // W from A: the first panel's columns, then the rest beside that panel
     // (on the aux stream with the look-ahead; the two touch disjoint columns)
-    const unsigned nt = (unsigned)((N + 31) / 32), ct = std::min(nt, (unsigned)((jb0 + 31) / 32));
-    hipLaunchKernelGGL(k_transpose_in<T>, dim3(ct, nt), dim3(256), 0, c.st, N, c.src, c.src_ld, c.W, c.ld, 0LL);
+    // (also the row sums of |A| per 512-column slice: the first launch
+    // covers whole slices, at least the first panel's columns)
+    const unsigned nt = (unsigned)((N + 31) / 32);
+    const unsigned ct = std::min(nt, (unsigned)((jb0 + 32 * kAbsTiles - 1) / (32 * kAbsTiles)) * kAbsTiles);
+    const unsigned nsl = (nt + kAbsTiles - 1) / kAbsTiles;
+    QD_ALLOC(c.h, s->rowabs, (size_t)nsl * N + N);
+    auto wg = [](unsigned tiles) { return (tiles + kAbsTiles - 1) / kAbsTiles; };
+    hipLaunchKernelGGL(k_transpose_abs<T>, dim3(wg(ct), nt), dim3(256), 0, c.st, N, c.src, c.src_ld, c.W, c.ld, 0LL,
+                       s->rowabs.p);
     QD_HIP(c.h, hipGetLastError());
     if (look) QD_HIP(c.h, hipEventRecord(s->ev_cols, c.st));
     // the rest is enqueued before the panel's many small launches, which
     // would hold the host up while the main stream idles
     if (nt > ct)
-      hipLaunchKernelGGL(k_transpose_in<T>, dim3(nt - ct, nt), dim3(256), 0, c.st, N, c.src, c.src_ld, c.W, c.ld,
-                         (long long)ct);
+      hipLaunchKernelGGL(k_transpose_abs<T>, dim3(wg(nt - ct), nt), dim3(256), 0, c.st, N, c.src, c.src_ld, c.W, c.ld,
+                         (long long)ct, s->rowabs.p);
     QD_HIP(c.h, hipGetLastError());
     if (look) {
       QD_HIP(c.h, hipStreamWaitEvent(cp.st, s->ev_cols, 0));
@@ -2550,6 +2609,7 @@ static int ensure_factors(qd_context* h, int precision, qd_solve_report* rep) {
   s->ld = ld;
   s->valid = false;
   s->anorm = -1.0;
+  s->rowabs_nsl = 0;
   const double t0 = now_ms();
   dim3 tg((unsigned)((N + 31) / 32), (unsigned)((N + 31) / 32));
   int rc;
@@ -2604,6 +2664,7 @@ static int ensure_factors(qd_context* h, int precision, qd_solve_report* rep) {
       diag = false;
     }
     s->diag_pivots = diag;
+    if (!rc) s->rowabs_nsl = (int)(((N + 31) / 32 + kAbsTiles - 1) / kAbsTiles);
   }
   if (rc) return rc;
   {
@@ -2765,6 +2826,15 @@ int lu_solve(qd_context* h, int precision, int nrhs, double* b, long long ldb, d
     R->converged = 1;
     R->iterations = 0;
   } else {
+    if (s->anorm < 0.0 && s->rowabs_nsl > 0) {
+      // the row sums that A's conversion into the factors' matrix left
+      hipLaunchKernelGGL(k_rowabs_sum, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, h->stream, N, s->rowabs_nsl,
+                         s->rowabs.p, s->rowabs.p + (size_t)s->rowabs_nsl * N);
+      QD_HIP(h, hipGetLastError());
+      std::vector<double> nn;
+      host_norms(h, 1, N, s->rowabs.p + (size_t)s->rowabs_nsl * N, N, nn);
+      s->anorm = nn[0];
+    }
     if (s->anorm < 0.0) {
       const int nsl = (int)((N + kResSlice - 1) / kResSlice);
       DevBuf<double> ra;
