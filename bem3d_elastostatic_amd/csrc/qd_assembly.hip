@@ -154,12 +154,21 @@ __global__ __launch_bounds__(64) void k_hpass(AsmArgs a, const int* __restrict__
 #pragma unroll
         for (int jj = 0; jj < 3; ++jj) acc[kk][ii][jj] = 0.0;
     const double* gp = asmtab + (size_t)e * kGP * kAsmStride;
+    // a Gauss point's data (position, normal, weights) are loaded one point
+    // ahead, in flight during the previous point's arithmetic
+    double cur[6 + NC];
+#pragma unroll
+    for (int q = 0; q < 6 + NC; ++q) cur[q] = gp[q];
 #pragma unroll 1
-    for (int g = 0; g < NG; ++g, gp += kAsmStride) {
-      const double r1 = gp[0] - xp, r2 = gp[1] - yp, r3 = gp[2] - zp;
+    for (int g = 0; g < NG; ++g) {
+      double nxt[6 + NC];
+      const double* gq = gp + (size_t)(g + 1 < NG ? g + 1 : g) * kAsmStride;
+#pragma unroll
+      for (int q = 0; q < 6 + NC; ++q) nxt[q] = gq[q];
+      const double r1 = cur[0] - xp, r2 = cur[1] - yp, r3 = cur[2] - zp;
       const double rr = r1 * r1 + r2 * r2 + r3 * r3;
       const double ir = inv_sqrt(rr);
-      const double eta[3] = {gp[3], gp[4], gp[5]};
+      const double eta[3] = {cur[3], cur[4], cur[5]};
       const double rd[3] = {r1 * ir, r2 * ir, r3 * ir};
       const double rdn = (r1 * eta[0] + r2 * eta[1] + r3 * eta[2]) * ir;
       const double c5r2 = c5 * (ir * ir);
@@ -173,12 +182,14 @@ __global__ __launch_bounds__(64) void k_hpass(AsmArgs a, const int* __restrict__
         }
 #pragma unroll
       for (int kk = 0; kk < NC; ++kk) {
-        const double p = gp[6 + kk];
+        const double p = cur[6 + kk];
 #pragma unroll
         for (int ii = 0; ii < 3; ++ii)
 #pragma unroll
           for (int jj = 0; jj < 3; ++jj) acc[kk][ii][jj] += t[jj][ii] * p;
       }
+#pragma unroll
+      for (int q = 0; q < 6 + NC; ++q) cur[q] = nxt[q];
     }
     if (valid && !sing) {
       const int kb = inf->kode, k0 = inf->k0mask;
