@@ -242,7 +242,23 @@ __global__ __launch_bounds__(64) void k_somi_far(int l, int ne, int slice, const
     const double dx = px - c[0], dy = py - c[1], dz = pz - c[2];
     if (dx * dx + dy * dy + dz * dz > c[3]) {
       const int g0 = far_start[e], g1 = far_start[e + 1];
-      for (int g = g0; g < g1; ++g) somi_gp(tab + (size_t)g * kSomiStride, px, py, pz, k, u, s);
+      if (g1 > g0) {
+        // each Gauss point's row loaded one point ahead, in flight during the
+        // previous point's arithmetic
+        double cur[kSomiStride];
+#pragma unroll
+        for (int q = 0; q < kSomiStride; ++q) cur[q] = tab[(size_t)g0 * kSomiStride + q];
+#pragma unroll 1
+        for (int g = g0; g < g1; ++g) {
+          double nxt[kSomiStride];
+          const double* tq = tab + (size_t)(g + 1 < g1 ? g + 1 : g) * kSomiStride;
+#pragma unroll
+          for (int q = 0; q < kSomiStride; ++q) nxt[q] = tq[q];
+          somi_gp(cur, px, py, pz, k, u, s);
+#pragma unroll
+          for (int q = 0; q < kSomiStride; ++q) cur[q] = nxt[q];
+        }
+      }
     }
   }
   if (valid) {
