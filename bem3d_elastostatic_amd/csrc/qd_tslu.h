@@ -657,11 +657,22 @@ __global__ __launch_bounds__(256) void k_finish_diag(float* __restrict__ W, long
   __shared__ float sR[kDW];
   __builtin_amdgcn_s_setprio(3);
   const int tid = threadIdx.x;
-  for (int i = tid; i < kDW * kDW; i += 256) {
-    const int r = i / kDW, j = i % kDW;
-    const float v = (r < w && j < w) ? lu_in[i] : 0.0f;
-    sU[r][j] = (j >= r) ? v : 0.0f;
-    if (blockIdx.x == 0 && r < w && j < w) W[(c0 + r) * ld + c0 + j] = v;
+  // this thread's row first, then the block's LU (all loads in flight before
+  // the LDS stores and barriers)
+  const long long r = c0 + w + (long long)blockIdx.x * 256 + tid;
+  float x[kDW];
+  if (r < N) dw_load_row(W, ld, r, c0, w, x);
+  float lv[kDW * kDW / 256];
+#pragma unroll
+  for (int q = 0; q < kDW * kDW / 256; ++q) {
+    const int i = tid + 256 * q, ri = i / kDW, j = i % kDW;
+    lv[q] = (ri < w && j < w) ? lu_in[i] : 0.0f;
+  }
+#pragma unroll
+  for (int q = 0; q < kDW * kDW / 256; ++q) {
+    const int i = tid + 256 * q, ri = i / kDW, j = i % kDW;
+    sU[ri][j] = (j >= ri) ? lv[q] : 0.0f;
+    if (blockIdx.x == 0 && ri < w && j < w) W[(c0 + ri) * ld + c0 + j] = lv[q];
   }
   __syncthreads();
   if (tid < kDW) {
@@ -669,11 +680,8 @@ __global__ __launch_bounds__(256) void k_finish_diag(float* __restrict__ W, long
     sR[tid] = (tid < w && d != 0.0f) ? 1.0f / d : 0.0f;
   }
   __syncthreads();
-  const long long r = c0 + w + (long long)blockIdx.x * 256 + tid;
   float g = 0.0f;
   if (r < N) {
-    float x[kDW];
-    dw_load_row(W, ld, r, c0, w, x);
     dwf_all(std::make_integer_sequence<int, kDW>{}, x, w, sU, sR);
 #pragma unroll
     for (int j = 0; j < kDW; ++j) g = fmaxf(g, fabsf(x[j]));
