@@ -416,7 +416,22 @@ __global__ __launch_bounds__(256) void k_trsm_strip(float* __restrict__ W, long 
   const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lg = lane >> 4;
   const long long col = c0 + (long long)blockIdx.x * kSW + c;
   const bool cv = col < c1;
-  for (int r = slot; r < m; r += kSlots) xs[r * kSW + c] = cv ? W[(r0 + r) * ld + col] : 0.0f;
+  {
+    // the strip's rows: every load in flight before the LDS stores (m is at
+    // most 1024 rows for the narrow strips, 256 for the wide ones)
+    constexpr int kLd = (kSW == 8 ? 1024 : 256) / kSlots;
+    float v[kLd];
+#pragma unroll
+    for (int q = 0; q < kLd; ++q) {
+      const int r = slot + q * kSlots;
+      v[q] = (cv && r < m) ? W[(r0 + r) * ld + col] : 0.0f;
+    }
+#pragma unroll
+    for (int q = 0; q < kLd; ++q) {
+      const int r = slot + q * kSlots;
+      if (r < m) xs[r * kSW + c] = v[q];
+    }
+  }
   const float* inv0 = linv + (r0 / 32) * 1024;
   auto swz = [](int r) { return r & 7; };
   // a chunk's L rows as MFMA A operands, straight from memory (L is shared by
