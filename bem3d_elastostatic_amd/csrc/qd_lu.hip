@@ -789,10 +789,18 @@ __global__ __launch_bounds__(256) void k_gemm_narrow(long long M, int N, int K, 
         for (int t = 0; t < 8; ++t) a[i][t] = (row < M && kb + t < K) ? A[row * lda + kb + t] : 0.0f;
       }
     }
+    // this slice of B: every load in flight before the LDS stores
+    float bv[32 * NW / 256];
+#pragma unroll
+    for (int q = 0; q < 32 * NW / 256; ++q) {
+      const int e = tid + 256 * q, kr = e / NW, c = e % NW;
+      bv[q] = (k0 + kr < K && c < N) ? B[(long long)(k0 + kr) * ldb + c] : 0.0f;
+    }
     if (k0 > 0) __syncthreads();  // the previous slice of B is consumed
-    for (int e = tid; e < 32 * NW; e += 256) {
-      const int kr = e / NW, c = e % NW;
-      sB[kr][c] = (k0 + kr < K && c < N) ? B[(long long)(k0 + kr) * ldb + c] : 0.0f;
+#pragma unroll
+    for (int q = 0; q < 32 * NW / 256; ++q) {
+      const int e = tid + 256 * q;
+      sB[e / NW][e % NW] = bv[q];
     }
     __syncthreads();
 #pragma unroll
