@@ -198,6 +198,18 @@ __global__ __launch_bounds__(64) void k_hpass(AsmArgs a, const int* __restrict__
 #pragma unroll
       for (int k = 0; k < NC; ++k) {
         const int ik = inf->node[k];
+        // the node's A columns: all nine loads in flight before the adds (the
+        // branches below are uniform; a column that is not updated is read
+        // and left alone)
+        double old[3][3];
+        if (a.write_A) {
+#pragma unroll
+          for (int ld = 0; ld < 3; ++ld) {
+            const double* col = Aout + (long long)(3 * ik + ld) * a.lda + 3LL * in;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) old[ld][q] = col[q];
+          }
+        }
 #pragma unroll
         for (int ld = 0; ld < 3; ++ld) {
           const int bit = 3 * k + ld;
@@ -210,9 +222,9 @@ __global__ __launch_bounds__(64) void k_hpass(AsmArgs a, const int* __restrict__
           } else if (((k0 >> bit) & 1) == 0) {
             if (a.write_A) {
               double* col = Aout + (long long)(3 * ik + ld) * a.lda + 3LL * in;
-              col[0] += h0;
-              col[1] += h1;
-              col[2] += h2;
+              col[0] = old[ld][0] + h0;
+              col[1] = old[ld][1] + h1;
+              col[2] = old[ld][2] + h2;
             }
           } else {
             const double v = evp[bit];
