@@ -195,21 +195,29 @@ __global__ __launch_bounds__(64) void k_hpass(AsmArgs a, const int* __restrict__
       const int kb = inf->kode, k0 = inf->k0mask;
       const double* ebc = ebc_all + (size_t)e * 27;
       const double* evp = evp_all + (size_t)e * 27;
+      // a node's A columns: its nine loads issued one node ahead, in flight
+      // during the previous node's updates (the branches below are uniform; a
+      // column that is not updated is read and left alone)
+      auto load_cols = [&](int k, double (&o)[3][3]) {
+        const int ik = inf->node[k];
+#pragma unroll
+        for (int ld = 0; ld < 3; ++ld) {
+          const double* col = Aout + (long long)(3 * ik + ld) * a.lda + 3LL * in;
+#pragma unroll
+          for (int q = 0; q < 3; ++q) o[ld][q] = col[q];
+        }
+      };
+      double nxt[3][3] = {};
+      if (a.write_A) load_cols(0, nxt);
 #pragma unroll
       for (int k = 0; k < NC; ++k) {
         const int ik = inf->node[k];
-        // the node's A columns: all nine loads in flight before the adds (the
-        // branches below are uniform; a column that is not updated is read
-        // and left alone)
         double old[3][3];
-        if (a.write_A) {
 #pragma unroll
-          for (int ld = 0; ld < 3; ++ld) {
-            const double* col = Aout + (long long)(3 * ik + ld) * a.lda + 3LL * in;
+        for (int ld = 0; ld < 3; ++ld)
 #pragma unroll
-            for (int q = 0; q < 3; ++q) old[ld][q] = col[q];
-          }
-        }
+          for (int q = 0; q < 3; ++q) old[ld][q] = nxt[ld][q];
+        if (a.write_A && k + 1 < NC) load_cols(k + 1, nxt);
 #pragma unroll
         for (int ld = 0; ld < 3; ++ld) {
           const int bit = 3 * k + ld;
