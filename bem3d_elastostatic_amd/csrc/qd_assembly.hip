@@ -307,6 +307,17 @@ __global__ __launch_bounds__(64) void k_gpass(AsmArgs a, const int* __restrict__
 #pragma unroll
       for (int k = 0; k < NC; ++k) {
         const int ik = inf->node[k];
+        // the node's A columns: all nine loads in flight before the updates
+        // (the branches below are uniform; a column not updated is left alone)
+        double old[3][3] = {};
+        if (a.write_A && (kb & 7) != 7) {
+#pragma unroll
+          for (int ld = 0; ld < 3; ++ld) {
+            const double* col = Aout + (long long)(3 * ik + ld) * a.lda + 3LL * in;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) old[ld][q] = col[q];
+          }
+        }
 #pragma unroll
         for (int ld = 0; ld < 3; ++ld) {
           // g(jj, 3k+ld) = U(jj, ld)
@@ -317,9 +328,9 @@ __global__ __launch_bounds__(64) void k_gpass(AsmArgs a, const int* __restrict__
           if (((kb >> ld) & 1) == 0) {
             if (a.write_A) {
               double* col = Aout + (long long)(3 * ik + ld) * a.lda + 3LL * in;
-              col[0] -= g0 * mu;
-              col[1] -= g1 * mu;
-              col[2] -= g2 * mu;
+              col[0] = old[ld][0] - g0 * mu;
+              col[1] = old[ld][1] - g1 * mu;
+              col[2] = old[ld][2] - g2 * mu;
             }
           } else {
             const double b = ebc[3 * k + ld];
