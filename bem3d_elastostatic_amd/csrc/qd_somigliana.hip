@@ -300,7 +300,23 @@ __global__ __launch_bounds__(64) void k_somi_near(int l, int ne, const double* _
       mask &= mask - 1;
       const int en = base + b;
       const double* t = tab + (size_t)en * kNear * kSomiStride;
-      for (int g = lane; g < kNear; g += 64) somi_gp(t + (size_t)g * kSomiStride, px, py, pz, k, u, s);
+      if (lane < kNear) {
+        // each lane's next Gauss point loaded one ahead, in flight during the
+        // current one's arithmetic
+        double cur[kSomiStride];
+#pragma unroll
+        for (int q = 0; q < kSomiStride; ++q) cur[q] = t[(size_t)lane * kSomiStride + q];
+#pragma unroll 1
+        for (int g = lane; g < kNear; g += 64) {
+          double nxt[kSomiStride];
+          const double* tq = t + (size_t)(g + 64 < kNear ? g + 64 : g) * kSomiStride;
+#pragma unroll
+          for (int q = 0; q < kSomiStride; ++q) nxt[q] = tq[q];
+          somi_gp(cur, px, py, pz, k, u, s);
+#pragma unroll
+          for (int q = 0; q < kSomiStride; ++q) cur[q] = nxt[q];
+        }
+      }
       near_gp += kNear;
       skipped_far += (nconec[en] == 9) ? 36 : 16;
     }
