@@ -2789,7 +2789,10 @@ static int host_norms(qd_context* h, int nrhs, long long N, const double* dev, l
     QD_HIP(h, hipMemcpyAsync(tmp.data(), dev + (long long)v * ld, N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     QD_HIP(h, hipStreamSynchronize(h->stream));
     double m = 0.0;
-    for (long long i = 0; i < N; ++i) m = std::max(m, std::fabs(tmp[i]));
+    for (long long i = 0; i < N; ++i) {
+      const double a = std::fabs(tmp[i]);
+      m = (a <= m) ? m : a;  // keeps a NaN
+    }
     out[v] = m;
   }
   return QD_OK;
@@ -2797,6 +2800,59 @@ static int host_norms(qd_context* h, int nrhs, long long N, const double* dev, l
 
 static int gmres_ir(qd_context* h, bool trans, double* x, double* b, long long ldb, double tol, int max_iter,
                     int* iters, double* relres);
+
+// ||A||_inf of the FP64 matrix, once per held factorisation: from the row
+// sums that A's conversion into W32 left, else one pass over A.
+static int ensure_anorm(qd_context* h) {
+  qd_lu_state* s = h->lu;
+  const long long N = s->N;
+  if (s->anorm >= 0.0) return QD_OK;
+  int rc;
+  std::vector<double> nn;
+  if (s->rowabs_nsl > 0) {
+    hipLaunchKernelGGL(k_rowabs_sum, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, h->stream, N, s->rowabs_nsl,
+                       s->rowabs.p, s->rowabs.p + (size_t)s->rowabs_nsl * N);
+    QD_HIP(h, hipGetLastError());
+    if ((rc = host_norms(h, 1, N, s->rowabs.p + (size_t)s->rowabs_nsl * N, N, nn))) return rc;
+  } else {
+    const int nsl = (int)((N + kResSlice - 1) / kResSlice);
+    DevBuf<double> ra;
+    QD_ALLOC(h, ra, (size_t)nsl * N + N);
+    hipLaunchKernelGGL(k_rowabs_part, dim3((unsigned)((N + 255) / 256), nsl), dim3(256), 0, h->stream, N, h->A.p,
+                       h->lda, ra.p);
+    DevBuf<double> zero;
+    QD_ALLOC(h, zero, (size_t)N);
+    QD_HIP(h, hipMemsetAsync(zero.p, 0, N * sizeof(double), h->stream));
+    // sum the slices (k_resid_final computes 0 - sum; the norm takes |.|)
+    hipLaunchKernelGGL(k_resid_final, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, h->stream, N, nsl, 1, ra.p,
+                       zero.p, N, ra.p + (size_t)nsl * N);
+    QD_HIP(h, hipGetLastError());
+    if ((rc = host_norms(h, 1, N, ra.p + (size_t)nsl * N, N, nn))) return rc;
+    // For A^T the bound would be ||A||_1; the row norm is kept for both (it
+    // only scales the stopping test, and the two are within a factor N).
+  }
+  s->anorm = nn[0];
+  return QD_OK;
+}
+
+// Largest normwise backward error over the right-hand sides:
+// ||r_v||_inf / (||A||_inf ||x_v||_inf + ||b_v||_inf); NaN anywhere gives NaN.
+static int backward_error(qd_context* h, int nrhs, const double* Rr, const double* X, const std::vector<double>& bn,
+                          double* worst) {
+  qd_lu_state* s = h->lu;
+  const long long N = s->N;
+  std::vector<double> rn, xn;
+  int rc;
+  if ((rc = host_norms(h, nrhs, N, Rr, N, rn))) return rc;
+  if ((rc = host_norms(h, nrhs, N, X, N, xn))) return rc;
+  double w = 0.0;
+  for (int v = 0; v < nrhs; ++v) {
+    const double e = rn[v] / (s->anorm * xn[v] + bn[v] + 1e-300);
+    w = (e <= w) ? w : e;  // keeps a NaN
+  }
+  *worst = w;
+  return QD_OK;
+}
 
 int lu_solve(qd_context* h, int precision, int nrhs, double* b, long long ldb, double tol, int max_iter,
              bool trans, qd_solve_report* rep) {
@@ -2843,43 +2899,18 @@ int lu_solve(qd_context* h, int precision, int nrhs, double* b, long long ldb, d
   for (int v = 0; v < nrhs; ++v)
     QD_HIP(h, hipMemcpyAsync(Bc + (long long)v * N, b + (long long)v * ldb, N * sizeof(double),
                              hipMemcpyDeviceToDevice, h->stream));
+  if ((rc = ensure_anorm(h))) return rc;
+  std::vector<double> bn;
+  if ((rc = host_norms(h, nrhs, N, Bc, N, bn))) return rc;
   if (precision == QD_FP64) {
     rc = apply_inverse(h, trans, nrhs, Bc, X);
     if (rc) return rc;
+    // the direct solve's own backward error against the FP64 A (one residual)
+    if ((rc = residual(h, trans, nrhs, X, N, Bc, N, Rr))) return rc;
+    if ((rc = backward_error(h, nrhs, Rr, X, bn, &R->residual))) return rc;
     R->converged = 1;
     R->iterations = 0;
   } else {
-    if (s->anorm < 0.0 && s->rowabs_nsl > 0) {
-      // the row sums that A's conversion into the factors' matrix left
-      hipLaunchKernelGGL(k_rowabs_sum, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, h->stream, N, s->rowabs_nsl,
-                         s->rowabs.p, s->rowabs.p + (size_t)s->rowabs_nsl * N);
-      QD_HIP(h, hipGetLastError());
-      std::vector<double> nn;
-      host_norms(h, 1, N, s->rowabs.p + (size_t)s->rowabs_nsl * N, N, nn);
-      s->anorm = nn[0];
-    }
-    if (s->anorm < 0.0) {
-      const int nsl = (int)((N + kResSlice - 1) / kResSlice);
-      DevBuf<double> ra;
-      QD_ALLOC(h, ra, (size_t)nsl * N + N);
-      hipLaunchKernelGGL(k_rowabs_part, dim3((unsigned)((N + 255) / 256), nsl), dim3(256), 0, h->stream, N, h->A.p,
-                         h->lda, ra.p);
-      DevBuf<double> zero;
-      QD_ALLOC(h, zero, (size_t)N);
-      QD_HIP(h, hipMemsetAsync(zero.p, 0, N * sizeof(double), h->stream));
-      // sum the slices (k_resid_final computes 0 - sum, so negate the norm)
-      hipLaunchKernelGGL(k_resid_final, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, h->stream, N, nsl, 1, ra.p,
-                         zero.p, N, ra.p + (size_t)nsl * N);
-      std::vector<double> nn;
-      host_norms(h, 1, N, ra.p + (size_t)nsl * N, N, nn);
-      s->anorm = nn[0];
-      if (trans) {
-        // ||A^T||_inf = ||A||_1: column sums, but the bound only scales the
-        // stopping test; keep the row norm (both are within a factor N).
-      }
-    }
-    std::vector<double> bn;
-    host_norms(h, nrhs, N, Bc, N, bn);
     QD_HIP(h, hipMemsetAsync(X, 0, (size_t)nrhs * N * sizeof(double), h->stream));
     QD_HIP(h, hipMemcpyAsync(Rr, Bc, (size_t)nrhs * N * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
     double prev = 1e300;
@@ -2895,17 +2926,15 @@ int lu_solve(qd_context* h, int precision, int nrhs, double* b, long long ldb, d
       ++it;
       rc = residual(h, trans, nrhs, X, N, Bc, N, Rr);
       if (rc) return rc;
-      std::vector<double> rn, xn;
-      host_norms(h, nrhs, N, Rr, N, rn);
-      host_norms(h, nrhs, N, X, N, xn);
       double worst = 0.0;
-      for (int v = 0; v < nrhs; ++v) worst = std::max(worst, rn[v] / (s->anorm * xn[v] + bn[v] + 1e-300));
+      if ((rc = backward_error(h, nrhs, Rr, X, bn, &worst))) return rc;
       R->residual = worst;
       if (worst <= tl) {
         done = true;
         break;
       }
       if (!(worst < 0.5 * prev)) {
+        // no progress, or a non-finite residual (GMRES stops on that too)
         stalled = true;
         break;
       }
@@ -2972,17 +3001,22 @@ __global__ void k_axpby(long long n, double a, const double* __restrict__ x, dou
   if (i < n) y[i] = a * x[i] + b * y[i];
 }
 
-static double dot(qd_context* h, long long n, const double* a, const double* b) {
+// *out = a . b; an error code when the device reduction fails, QD_ERR_NOCONV
+// when the value is not finite (the Arnoldi process cannot continue).
+static int dot(qd_context* h, long long n, const double* a, const double* b, double* out) {
   const int nb = 512;
   DevBuf<double>& part = h->lu->rowpart;
-  if (part.ensure(std::max<size_t>(part.count, nb)) != hipSuccess) return NAN;
+  QD_ALLOC(h, part, std::max<size_t>(part.count, nb));
   hipLaunchKernelGGL(k_dot_part, dim3(nb), dim3(256), 0, h->stream, n, a, b, part.p);
+  QD_HIP(h, hipGetLastError());
   double hp[nb];
-  if (hipMemcpyAsync(hp, part.p, sizeof(hp), hipMemcpyDeviceToHost, h->stream) != hipSuccess) return NAN;
-  if (hipStreamSynchronize(h->stream) != hipSuccess) return NAN;
+  QD_HIP(h, hipMemcpyAsync(hp, part.p, sizeof(hp), hipMemcpyDeviceToHost, h->stream));
+  QD_HIP(h, hipStreamSynchronize(h->stream));
   double s = 0.0;
   for (int i = 0; i < nb; ++i) s += hp[i];
-  return s;
+  if (!std::isfinite(s)) return set_error(h, QD_ERR_NOCONV, "GMRES: non-finite inner product");
+  *out = s;
+  return QD_OK;
 }
 
 static int gmres_ir(qd_context* h, bool trans, double* x, double* b, long long /*ldb*/, double tol, int max_restart,
@@ -2996,23 +3030,20 @@ static int gmres_ir(qd_context* h, bool trans, double* x, double* b, long long /
   double* r = V + (size_t)(m + 2) * N;
   const unsigned nb = (unsigned)((N + 255) / 256);
   QD_HIP(h, hipMemsetAsync(V + (size_t)m * N, 0, N * sizeof(double), h->stream));
-  const double bnorm_inf = [&] {
-    std::vector<double> o;
-    host_norms(h, 1, N, b, N, o);
-    return o[0];
-  }();
+  std::vector<double> bn;
+  int rc;
+  if ((rc = host_norms(h, 1, N, b, N, bn))) return rc;
   *iters = 0;
   for (int restart = 0; restart < std::max(1, max_restart); ++restart) {
-    int rc = residual(h, trans, 1, x, N, b, N, r);
-    if (rc) return rc;
-    std::vector<double> rn, xn;
-    host_norms(h, 1, N, r, N, rn);
-    host_norms(h, 1, N, x, N, xn);
-    *relres = rn[0] / (s->anorm * xn[0] + bnorm_inf + 1e-300);
+    if ((rc = residual(h, trans, 1, x, N, b, N, r))) return rc;
+    if ((rc = backward_error(h, 1, r, x, bn, relres))) return rc;
     if (*relres <= tol) return QD_OK;
+    if (!std::isfinite(*relres)) return set_error(h, QD_ERR_NOCONV, "GMRES: non-finite residual");
     rc = apply_inverse(h, trans, 1, r, w);
     if (rc) return rc;
-    const double beta = std::sqrt(dot(h, N, w, w));
+    double ww;
+    if ((rc = dot(h, N, w, w, &ww))) return rc;
+    const double beta = std::sqrt(ww);
     std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1, 0.0);
     g[0] = beta;
     hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, h->stream, N, 1.0 / beta, w, 0.0, V);
@@ -3027,11 +3058,14 @@ static int gmres_ir(qd_context* h, bool trans, double* x, double* b, long long /
       if (rc) return rc;
       hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, h->stream, N, 0.0, w, -1.0, w);
       for (int j = 0; j <= k; ++j) {
-        const double hij = dot(h, N, w, V + (size_t)j * N);
+        double hij;
+        if ((rc = dot(h, N, w, V + (size_t)j * N, &hij))) return rc;
         H[(size_t)j * m + k] = hij;
         hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, h->stream, N, -hij, V + (size_t)j * N, 1.0, w);
       }
-      const double hn = std::sqrt(dot(h, N, w, w));
+      double hh;
+      if ((rc = dot(h, N, w, w, &hh))) return rc;
+      const double hn = std::sqrt(hh);
       H[(size_t)(k + 1) * m + k] = hn;
       if (k + 1 < m) hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, h->stream, N, 1.0 / hn, w, 0.0, V + (size_t)(k + 1) * N);
       for (int j = 0; j < k; ++j) {
@@ -3063,13 +3097,8 @@ static int gmres_ir(qd_context* h, bool trans, double* x, double* b, long long /
       hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, h->stream, N, y[i], V + (size_t)i * N, 1.0, x);
     QD_HIP(h, hipGetLastError());
   }
-  int rc = residual(h, trans, 1, x, N, b, N, r);
-  if (rc) return rc;
-  std::vector<double> rn, xn;
-  host_norms(h, 1, N, r, N, rn);
-  host_norms(h, 1, N, x, N, xn);
-  *relres = rn[0] / (s->anorm * xn[0] + bnorm_inf + 1e-300);
-  return QD_OK;
+  if ((rc = residual(h, trans, 1, x, N, b, N, r))) return rc;
+  return backward_error(h, 1, r, x, bn, relres);
 }
 
 void lu_free(qd_context* h) {

@@ -367,6 +367,13 @@ __global__ __launch_bounds__(64 * V) void k_tslu_tree(const float* __restrict__ 
   }
 }
 
+// Running maximum of |v| that keeps NaN: for non-negative floats the unsigned
+// order of the bit patterns is the numeric order with NaN above +inf
+// (fmaxf would drop a NaN operand).
+__device__ __forceinline__ float abs_max_keep_nan(float g, float v) {
+  return __uint_as_float(max(__float_as_uint(g), __float_as_uint(fabsf(v))));
+}
+
 // Largest magnitude of a float as its bit pattern (NaN above +inf), folded
 // into *growth; called by whole waves.
 __device__ __forceinline__ void ts_growth_max(float v, unsigned* growth) {
@@ -555,13 +562,13 @@ __device__ __forceinline__ void dw_step(float (&a)[kDW], int w, float& gmax) {
   if (K < w) {
     const int lane = threadIdx.x;
     const float piv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a[K]), K));
-    if (piv == 0.0f) gmax = __int_as_float(0x7f800000);
+    if (piv == 0.0f) gmax = abs_max_keep_nan(gmax, __int_as_float(0x7f800000));
     const float rp = (piv != 0.0f) ? __builtin_amdgcn_rcpf(piv) : 0.0f;
     const bool act = lane > K && lane < w;
     const float l = a[K] * rp;
     if (act) {
       a[K] = l;
-      gmax = fmaxf(gmax, fabsf(l));
+      gmax = abs_max_keep_nan(gmax, l);
     }
 #pragma unroll
     for (int j = K + 1; j < kDW; ++j) {
@@ -684,7 +691,7 @@ __global__ __launch_bounds__(256) void k_finish_diag(float* __restrict__ W, long
   if (r < N) {
     dwf_all(std::make_integer_sequence<int, kDW>{}, x, w, sU, sR);
 #pragma unroll
-    for (int j = 0; j < kDW; ++j) g = fmaxf(g, fabsf(x[j]));
+    for (int j = 0; j < kDW; ++j) g = abs_max_keep_nan(g, x[j]);
     float* dst = W + r * ld + c0;
     if (w == kDW) {
 #pragma unroll

@@ -17,12 +17,8 @@ pytestmark = pytest.mark.gpu
 
 
 def _hip():
-    """The HIP runtime this process already uses (the library is linked to it)."""
-    with open("/proc/self/maps") as f:
-        for line in f:
-            if "libamdhip64" in line:
-                return C.CDLL(line.split()[-1])
-    raise RuntimeError("HIP runtime not loaded")
+    from conftest import hip_runtime
+    return hip_runtime()
 
 
 @pytest.mark.parametrize("precision", ["fp16", "fp32"])
@@ -59,3 +55,35 @@ def test_reversed_rows_fall_back_to_tournament(precision):
     assert rep["pivoting"] == 0, rep
     assert rep["converged"] == 1 and rep["residual"] <= 1e-14
     assert rel_err(X, g["X"]) < 1e-10
+
+
+@pytest.mark.parametrize("precision", ["fp16", "fp32"])
+@pytest.mark.parametrize("row", [10, 300])
+def test_nan_multiplier_falls_back(precision, row):
+    """A NaN entry of A in column 5 gives a NaN multiplier in the first leaf:
+    row 10 inside the leaf's 64 x 64 top block (the one-wave elimination),
+    row 300 below it (the L21 rows).  The largest-multiplier word must rank
+    the NaN above every bound, so the factors are redone with tournament
+    pivoting (pivoting == 0); the refinement then reports QD_ERR_NOCONV
+    instead of a result."""
+    import torch
+    from bem3d_elastostatic_amd import _lib
+    from bem3d_elastostatic_amd.quadesth import Quadesth
+    g = golden("cube8")
+    q = Quadesth(problem_from_golden(g))
+    q.ghmat8()
+    _, FI = q.system(with_matrix=False)
+    N = FI.shape[0]
+    pa, lda, _ = q.system_device()
+    nan = np.array([np.nan])
+    rc = _hip().hipMemcpy(C.c_void_p(pa + 8 * (5 * lda + row)), nan.ctypes.data_as(C.c_void_p), C.c_size_t(8),
+                          C.c_int(1))
+    assert rc == 0
+    b = torch.tensor(FI, dtype=torch.float64, device="cuda")
+    rep = _lib.SolveReport()
+    print(f"[pivoting] cube8 NaN at A[{row}, 5] {precision}", flush=True)
+    rc = q.lib.qd_solve_device(q.h, _lib.PRECISIONS[precision], 1, C.c_void_p(b.data_ptr()), N, 1e-14, 30,
+                               C.byref(rep))
+    print(f"[pivoting] rc {rc} {rep.as_dict()}", flush=True)
+    assert rep.pivoting == 0, rep.as_dict()
+    assert rc == -6 and rep.converged == 0
