@@ -60,6 +60,7 @@ _SIGNATURES = {
     "qd_system_device": (_i, [_vp, C.POINTER(_pd), _pll, C.POINTER(_pd)]),
     "qd_solve": (_i, [_vp, _i, _i, _pd, _i, _d, _i, C.POINTER(SolveReport)]),
     "qd_solve_device": (_i, [_vp, _i, _i, _vp, _ll, _d, _i, C.POINTER(SolveReport)]),
+    "qd_solve_device_noalloc": (_i, [_vp, _i, _i, _vp, _ll, _d, _i, C.POINTER(SolveReport)]),
     "qd_solve_transpose_device": (_i, [_vp, _i, _i, _vp, _ll, _d, _i, C.POINTER(SolveReport)]),
     "qd_reorganize_solution": (_i, [_vp, _pd, _pd, _pd]),
     "qd_internal_points": (_i, [_vp, _i, _pd, _pd, _pd, _pd, _pd, _i, _pd, _pd, _pd, _pd, _pll]),

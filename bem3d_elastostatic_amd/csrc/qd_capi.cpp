@@ -190,6 +190,11 @@ int qd_create(qd_handle* out, int device) {
   if (hipSetDevice(device) != hipSuccess) return create_failed(QD_ERR_HIP, "qd_create: hipSetDevice failed");
   qd_context* h = new qd_context();
   h->device = device;
+  std::string serr;
+  if (noalloc_init(device, &serr) != QD_OK) {
+    delete h;
+    return create_failed(QD_ERR_HIP, "qd_create: " + serr);
+  }
   std::vector<SubTri> st;
   build_subtri(st);
   if (h->subtri.ensure(st.size()) != hipSuccess ||
@@ -563,6 +568,18 @@ int qd_solve_device(qd_handle h, int precision, int nrhs, double* b_dev, long lo
     return set_error(h, QD_ERR_ARG, "nrhs=%d ldb=%lld", nrhs, ldb);
   QD_HIP(h, hipSetDevice(h->device));
   return lu_solve(h, precision, nrhs, b_dev, ldb, tol, max_iter, false, rep);
+}
+
+int qd_solve_device_noalloc(qd_handle h, int precision, int nrhs, double* b_dev, long long ldb, double tol,
+                            int max_iter, qd_solve_report* rep) {
+  if (!h) return QD_ERR_ARG;
+  if (!h->have_sys) return set_error(h, QD_ERR_STATE, "qd_solve_device_noalloc before qd_ghmat8");
+  if (precision < QD_FP64 || precision > QD_FP16_IR)
+    return set_error(h, QD_ERR_ARG, "precision %d", precision);
+  if (nrhs < 1 || !b_dev || ldb < 3LL * h->n)
+    return set_error(h, QD_ERR_ARG, "nrhs=%d ldb=%lld", nrhs, ldb);
+  QD_HIP(h, hipSetDevice(h->device));
+  return lu_solve_noalloc(h, precision, nrhs, b_dev, ldb, tol, max_iter, rep);
 }
 
 int qd_solve_transpose_device(qd_handle h, int precision, int nrhs, double* c_dev, long long ldc,

@@ -29,26 +29,31 @@ inline MemStats& mem_stats() {
   return s;
 }
 
-// Device buffer owned by the handle.
+// Device buffer owned by the handle, or a view of memory it does not own
+// (a slice of the allocation-free entry point's static workspace).
 template <typename T>
 struct DevBuf {
   T* p = nullptr;
   size_t count = 0;
+  bool view = false;
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() { release(); }
   void release() {
-    if (p) {
+    if (p && !view) {
       (void)hipFree(p);
       mem_stats().add(-(long long)(count * sizeof(T)));
     }
     p = nullptr;
     count = 0;
+    view = false;
   }
-  // (Re)allocates when the size grows; contents are not preserved.
+  // (Re)allocates when the size grows; contents are not preserved.  A view
+  // never allocates: too small a view is an error.
   hipError_t ensure(size_t n) {
     if (n <= count && p) return hipSuccess;
+    if (view) return hipErrorOutOfMemory;
     release();
     if (n == 0) return hipSuccess;
     hipError_t e = hipMalloc((void**)&p, n * sizeof(T));
@@ -103,6 +108,9 @@ struct qd_context {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
+  // set for the duration of an allocation-free call: QD_ALLOC then fails
+  // instead of growing a buffer
+  bool forbid_alloc = false;
 
   // problem data (host copies)
   int n = 0, ne = 0;
@@ -201,6 +209,9 @@ int set_error(qd_context* h, int code, const char* fmt, ...);
   } while (0)
 #define QD_ALLOC(h, buf, n)                                                                    \
   do {                                                                                         \
+    if ((h)->forbid_alloc && ((buf).count < (size_t)(n) || !(buf).p))                           \
+      return qd::set_error((h), QD_ERR_NOMEM, "%s:%d: %zu elements needed in an allocation-free call", \
+                           __FILE__, __LINE__, (size_t)(n));                                    \
     hipError_t e_ = (buf).ensure(n);                                                           \
     if (e_ != hipSuccess)                                                                      \
       return qd::set_error((h), QD_ERR_NOMEM, "%s:%d allocating %zu elements: %s", __FILE__, \
@@ -227,6 +238,12 @@ int boundary_stress_run(qd_context* h, const double* fi_dev, const double* dfi_d
                         double* se_dev, double* sn_dev, double* prin_dev, double* vm_dev);
 int lu_solve(qd_context* h, int precision, int nrhs, double* b_dev, long long ldb, double tol,
              int max_iter, bool transpose, qd_solve_report* rep);
+// lu_solve with the held factors and every work vector in the library's
+// static device workspace (no allocation); QD_ERR_STATE without such factors.
+int lu_solve_noalloc(qd_context* h, int precision, int nrhs, double* b_dev, long long ldb, double tol,
+                     int max_iter, qd_solve_report* rep);
+// Once per device, with that device current: the workspace's device address.
+int noalloc_init(int device, std::string* err);
 void lu_free(qd_context* h);
 int fp16_update(qd_context* h, long long M, long long N, int Kp, const void* A16, const void* B16,
                 const float* colscale, float* C, long long ldc);

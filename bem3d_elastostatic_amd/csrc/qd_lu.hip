@@ -34,6 +34,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <cmath>
+#include <mutex>
 #include <type_traits>
 #include <utility>
 #include <vector>
@@ -70,6 +71,7 @@ struct qd_lu_state {
   qd::DevBuf<float> vecf;
   qd::DevBuf<double> gm;         // GMRES basis
   qd::DevBuf<double> rowpart;    // residual slice partials
+  qd::DevBuf<double> nrm;        // block maxima of the inf-norms
   qd::DevBuf<int> ts_cand;       // tournament candidate lists, all levels
   qd::DevBuf<unsigned> ts_cnt;   // tournament group counters (zero between leaves)
   qd::DevBuf<int> ts_meta;       // interchanges of a leaf
@@ -2740,13 +2742,14 @@ static int residual(qd_context* h, bool trans, int nrhs, const double* x, long l
   return QD_OK;
 }
 
-// d = op(A)^-1 r with the held factors; r, d: nrhs vectors of stride N (FP64).
-static int apply_inverse(qd_context* h, bool trans, int nrhs, const double* r, double* d) {
+// d = op(A)^-1 r with the held factors; r, d: nrhs vectors of stride N (FP64);
+// t64: nrhs * N doubles of work space for the FP64 factors.
+static int apply_inverse(qd_context* h, bool trans, int nrhs, const double* r, double* d, double* t64) {
   qd_lu_state* s = h->lu;
   const long long N = s->N;
   const unsigned nb = (unsigned)((N + 255) / 256);
   if (s->precision == QD_FP64) {
-    double* t = s->vec.p + (size_t)4 * 8 * N;  // scratch slot
+    double* t = t64;
     if (!trans) {
       hipLaunchKernelGGL((k_gather<double, double>), dim3(nb), dim3(256), 0, h->stream, N, nrhs, s->perm_dev.p, r, N, t);
       int rc = tri_solve_inv<double>(h, s->W64.p, s->ld, N, nrhs, t, s->inv64.p);
@@ -2781,24 +2784,53 @@ __global__ void k_axpy_inplace(long long n, const double* __restrict__ d, double
   if (i < n) x[i] += d[i];
 }
 
-// inf-norms per rhs, reduced on the host (vectors are O(N))
-static int host_norms(qd_context* h, int nrhs, long long N, const double* dev, long long ld, std::vector<double>& out) {
-  std::vector<double> tmp((size_t)N);
-  out.assign(nrhs, 0.0);
+__global__ void k_fill_zero(long long n, double* __restrict__ x) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = 0.0;
+}
+
+// max(a, b) that keeps a NaN in either operand
+__host__ __device__ __forceinline__ double max_keep_nan(double a, double b) { return (a != a || a > b) ? a : b; }
+
+// Block maxima of |v| (NaN kept) for nrhs vectors of stride ld: part[v * kNormBlocks + block].
+constexpr int kNormBlocks = 64;
+__global__ __launch_bounds__(256) void k_absmax_part(long long N, const double* __restrict__ v, long long ld,
+                                                     double* __restrict__ part) {
+  __shared__ double red[256];
+  const double* x = v + (long long)blockIdx.y * ld;
+  double m = 0.0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < N; i += (long long)kNormBlocks * 256)
+    m = max_keep_nan(fabs(x[i]), m);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] = max_keep_nan(red[threadIdx.x + k], red[threadIdx.x]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[(long long)blockIdx.y * kNormBlocks + blockIdx.x] = red[0];
+}
+
+// inf-norms of nrhs (<= 8) vectors: block maxima on the device, the last
+// kNormBlocks per vector on the host
+static int host_norms(qd_context* h, int nrhs, long long N, const double* dev, long long ld, double* out) {
+  qd_lu_state* s = h->lu;
+  if (nrhs < 1 || nrhs > 8) return set_error(h, QD_ERR_ARG, "host_norms: nrhs=%d", nrhs);
+  QD_ALLOC(h, s->nrm, (size_t)8 * kNormBlocks);
+  hipLaunchKernelGGL(k_absmax_part, dim3(kNormBlocks, nrhs), dim3(256), 0, h->stream, N, dev, ld, s->nrm.p);
+  QD_HIP(h, hipGetLastError());
+  double part[8 * kNormBlocks];
+  QD_HIP(h, hipMemcpyAsync(part, s->nrm.p, (size_t)nrhs * kNormBlocks * sizeof(double), hipMemcpyDeviceToHost,
+                           h->stream));
+  QD_HIP(h, hipStreamSynchronize(h->stream));
   for (int v = 0; v < nrhs; ++v) {
-    QD_HIP(h, hipMemcpyAsync(tmp.data(), dev + (long long)v * ld, N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    QD_HIP(h, hipStreamSynchronize(h->stream));
     double m = 0.0;
-    for (long long i = 0; i < N; ++i) {
-      const double a = std::fabs(tmp[i]);
-      m = (a <= m) ? m : a;  // keeps a NaN
-    }
+    for (int i = 0; i < kNormBlocks; ++i) m = max_keep_nan(part[v * kNormBlocks + i], m);
     out[v] = m;
   }
   return QD_OK;
 }
 
-static int gmres_ir(qd_context* h, bool trans, double* x, double* b, long long ldb, double tol, int max_iter,
+static int gmres_ir(qd_context* h, bool trans, double* x, double* b, double* t64, double tol, int max_iter,
                     int* iters, double* relres);
 
 // ||A||_inf of the FP64 matrix, once per held factorisation: from the row
@@ -2808,48 +2840,43 @@ static int ensure_anorm(qd_context* h) {
   const long long N = s->N;
   if (s->anorm >= 0.0) return QD_OK;
   int rc;
-  std::vector<double> nn;
+  double nn = 0.0;
   if (s->rowabs_nsl > 0) {
     hipLaunchKernelGGL(k_rowabs_sum, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, h->stream, N, s->rowabs_nsl,
                        s->rowabs.p, s->rowabs.p + (size_t)s->rowabs_nsl * N);
     QD_HIP(h, hipGetLastError());
-    if ((rc = host_norms(h, 1, N, s->rowabs.p + (size_t)s->rowabs_nsl * N, N, nn))) return rc;
+    if ((rc = host_norms(h, 1, N, s->rowabs.p + (size_t)s->rowabs_nsl * N, N, &nn))) return rc;
   } else {
+    // slice partials of the row sums of |A|, then their sums (in the residual
+    // partials' buffer)
     const int nsl = (int)((N + kResSlice - 1) / kResSlice);
-    DevBuf<double> ra;
-    QD_ALLOC(h, ra, (size_t)nsl * N + N);
+    QD_ALLOC(h, s->rowpart, (size_t)(nsl + 1) * N);
+    double* ra = s->rowpart.p;
     hipLaunchKernelGGL(k_rowabs_part, dim3((unsigned)((N + 255) / 256), nsl), dim3(256), 0, h->stream, N, h->A.p,
-                       h->lda, ra.p);
-    DevBuf<double> zero;
-    QD_ALLOC(h, zero, (size_t)N);
-    QD_HIP(h, hipMemsetAsync(zero.p, 0, N * sizeof(double), h->stream));
-    // sum the slices (k_resid_final computes 0 - sum; the norm takes |.|)
-    hipLaunchKernelGGL(k_resid_final, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, h->stream, N, nsl, 1, ra.p,
-                       zero.p, N, ra.p + (size_t)nsl * N);
+                       h->lda, ra);
+    hipLaunchKernelGGL(k_rowabs_sum, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, h->stream, N, nsl, ra,
+                       ra + (size_t)nsl * N);
     QD_HIP(h, hipGetLastError());
-    if ((rc = host_norms(h, 1, N, ra.p + (size_t)nsl * N, N, nn))) return rc;
+    if ((rc = host_norms(h, 1, N, ra + (size_t)nsl * N, N, &nn))) return rc;
     // For A^T the bound would be ||A||_1; the row norm is kept for both (it
     // only scales the stopping test, and the two are within a factor N).
   }
-  s->anorm = nn[0];
+  s->anorm = nn;
   return QD_OK;
 }
 
 // Largest normwise backward error over the right-hand sides:
 // ||r_v||_inf / (||A||_inf ||x_v||_inf + ||b_v||_inf); NaN anywhere gives NaN.
-static int backward_error(qd_context* h, int nrhs, const double* Rr, const double* X, const std::vector<double>& bn,
+static int backward_error(qd_context* h, int nrhs, const double* Rr, const double* X, const double* bn,
                           double* worst) {
   qd_lu_state* s = h->lu;
   const long long N = s->N;
-  std::vector<double> rn, xn;
+  double rn[8], xn[8];
   int rc;
   if ((rc = host_norms(h, nrhs, N, Rr, N, rn))) return rc;
   if ((rc = host_norms(h, nrhs, N, X, N, xn))) return rc;
   double w = 0.0;
-  for (int v = 0; v < nrhs; ++v) {
-    const double e = rn[v] / (s->anorm * xn[v] + bn[v] + 1e-300);
-    w = (e <= w) ? w : e;  // keeps a NaN
-  }
+  for (int v = 0; v < nrhs; ++v) w = max_keep_nan(rn[v] / (s->anorm * xn[v] + bn[v] + 1e-300), w);
   *worst = w;
   return QD_OK;
 }
@@ -2890,20 +2917,22 @@ int lu_solve(qd_context* h, int precision, int nrhs, double* b, long long ldb, d
   const long long N = s->N;
   const double t0 = now_ms();
   const int lb_epoch0 = s->lb_epoch;
-  QD_ALLOC(h, s->vec, (size_t)5 * 8 * N);
-  QD_ALLOC(h, s->vecf, (size_t)8 * N);
+  const long long S = (long long)nrhs * N;
+  QD_ALLOC(h, s->vec, (size_t)5 * S);
+  QD_ALLOC(h, s->vecf, (size_t)S);
   double* X = s->vec.p;                 // solutions
-  double* Rr = s->vec.p + 8 * N;        // residuals
-  double* D = s->vec.p + 16 * N;        // corrections
-  double* Bc = s->vec.p + 24 * N;       // compact copy of b
+  double* Rr = s->vec.p + S;            // residuals
+  double* D = s->vec.p + 2 * S;         // corrections
+  double* Bc = s->vec.p + 3 * S;        // compact copy of b
+  double* T64 = s->vec.p + 4 * S;       // work vectors of the FP64 substitutions
   for (int v = 0; v < nrhs; ++v)
     QD_HIP(h, hipMemcpyAsync(Bc + (long long)v * N, b + (long long)v * ldb, N * sizeof(double),
                              hipMemcpyDeviceToDevice, h->stream));
   if ((rc = ensure_anorm(h))) return rc;
-  std::vector<double> bn;
+  double bn[8];
   if ((rc = host_norms(h, nrhs, N, Bc, N, bn))) return rc;
   if (precision == QD_FP64) {
-    rc = apply_inverse(h, trans, nrhs, Bc, X);
+    rc = apply_inverse(h, trans, nrhs, Bc, X, T64);
     if (rc) return rc;
     // the direct solve's own backward error against the FP64 A (one residual)
     if ((rc = residual(h, trans, nrhs, X, N, Bc, N, Rr))) return rc;
@@ -2911,15 +2940,15 @@ int lu_solve(qd_context* h, int precision, int nrhs, double* b, long long ldb, d
     R->converged = 1;
     R->iterations = 0;
   } else {
-    QD_HIP(h, hipMemsetAsync(X, 0, (size_t)nrhs * N * sizeof(double), h->stream));
-    QD_HIP(h, hipMemcpyAsync(Rr, Bc, (size_t)nrhs * N * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, h->stream, S, X);
+    QD_HIP(h, hipMemcpyAsync(Rr, Bc, (size_t)S * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
     double prev = 1e300;
     int it = 0;
     bool done = false, stalled = false;
     const int maxit = max_iter > 0 ? max_iter : 30;
     const double tl = tol > 0.0 ? tol : 1e-14;
     while (it < maxit) {
-      rc = apply_inverse(h, trans, nrhs, Rr, D);
+      rc = apply_inverse(h, trans, nrhs, Rr, D, T64);
       if (rc) return rc;
       hipLaunchKernelGGL(k_axpy_inplace, dim3((unsigned)((nrhs * N + 255) / 256)), dim3(256), 0, h->stream,
                          (long long)nrhs * N, D, X);
@@ -2950,7 +2979,7 @@ int lu_solve(qd_context* h, int precision, int nrhs, double* b, long long ldb, d
       for (int v = 0; v < nrhs; ++v) {
         int its = 0;
         double rr = 0.0;
-        rc = gmres_ir(h, trans, X + (long long)v * N, Bc + (long long)v * N, N, tl, maxit, &its, &rr);
+        rc = gmres_ir(h, trans, X + (long long)v * N, Bc + (long long)v * N, T64, tl, maxit, &its, &rr);
         if (rc < 0) return rc;
         all = all && rr <= tl;
         worst = std::max(worst, rr);
@@ -3019,18 +3048,19 @@ static int dot(qd_context* h, long long n, const double* a, const double* b, dou
   return QD_OK;
 }
 
-static int gmres_ir(qd_context* h, bool trans, double* x, double* b, long long /*ldb*/, double tol, int max_restart,
+constexpr int kGmresM = 20;  // Krylov vectors per restart
+static int gmres_ir(qd_context* h, bool trans, double* x, double* b, double* t64, double tol, int max_restart,
                     int* iters, double* relres) {
   qd_lu_state* s = h->lu;
   const long long N = s->N;
-  const int m = 20;
+  constexpr int m = kGmresM;
   QD_ALLOC(h, s->gm, (size_t)(m + 3) * N);
   double* V = s->gm.p;
   double* w = V + (size_t)(m + 1) * N;
   double* r = V + (size_t)(m + 2) * N;
   const unsigned nb = (unsigned)((N + 255) / 256);
-  QD_HIP(h, hipMemsetAsync(V + (size_t)m * N, 0, N * sizeof(double), h->stream));
-  std::vector<double> bn;
+  hipLaunchKernelGGL(k_fill_zero, dim3(nb), dim3(256), 0, h->stream, N, V + (size_t)m * N);
+  double bn[1];
   int rc;
   if ((rc = host_norms(h, 1, N, b, N, bn))) return rc;
   *iters = 0;
@@ -3039,12 +3069,12 @@ static int gmres_ir(qd_context* h, bool trans, double* x, double* b, long long /
     if ((rc = backward_error(h, 1, r, x, bn, relres))) return rc;
     if (*relres <= tol) return QD_OK;
     if (!std::isfinite(*relres)) return set_error(h, QD_ERR_NOCONV, "GMRES: non-finite residual");
-    rc = apply_inverse(h, trans, 1, r, w);
+    rc = apply_inverse(h, trans, 1, r, w, t64);
     if (rc) return rc;
     double ww;
     if ((rc = dot(h, N, w, w, &ww))) return rc;
     const double beta = std::sqrt(ww);
-    std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1, 0.0);
+    double H[(m + 1) * m] = {}, cs[m] = {}, sn[m] = {}, g[m + 1] = {};
     g[0] = beta;
     hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, h->stream, N, 1.0 / beta, w, 0.0, V);
     int k = 0;
@@ -3054,7 +3084,7 @@ static int gmres_ir(qd_context* h, bool trans, double* x, double* b, long long /
       if (rc) return rc;
       // residual gives b - A v; use b = 0 slot: compute A v = -(0 - A v)
       // (V[m] is zeroed below before first use)
-      rc = apply_inverse(h, trans, 1, r, w);
+      rc = apply_inverse(h, trans, 1, r, w, t64);
       if (rc) return rc;
       hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, h->stream, N, 0.0, w, -1.0, w);
       for (int j = 0; j <= k; ++j) {
@@ -3087,7 +3117,7 @@ static int gmres_ir(qd_context* h, bool trans, double* x, double* b, long long /
         break;
       }
     }
-    std::vector<double> y(k, 0.0);
+    double y[m] = {};
     for (int i = k - 1; i >= 0; --i) {
       double t = g[i];
       for (int j = i + 1; j < k; ++j) t -= H[(size_t)i * m + j] * y[j];
@@ -3104,6 +3134,160 @@ static int gmres_ir(qd_context* h, bool trans, double* x, double* b, long long /
 void lu_free(qd_context* h) {
   delete h->lu;
   h->lu = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free solves (qd_solve_device_noalloc).  Every work vector of a
+// solve with held factors comes from this one static array: each GPU the
+// library runs on holds its own copy of it until the process ends.  Host code
+// reaches a GPU's copy only through the address hipGetSymbolAddress gave for
+// it (the array's name is a host-side stand-in, not a device address).  The
+// array keeps the last call's contents.  Its first part holds only the
+// look-back solves' flags (with the error word) and published words, at the
+// same offsets in every call: they carry a per-device epoch that only grows,
+// so a stale word never matches a later call; the array is cleared once at
+// set-up.  The work vectors follow, sliced per call by N and the number of
+// right-hand sides.  Calls that overlap on one GPU share the array and race.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+constexpr size_t kScratchFlags = 16384;    // ints: error word, block and part flags
+constexpr size_t kScratchPub = 131072;     // 64-bit published words of the FP32 look-back solves
+constexpr size_t kScratchVectors = kScratchFlags * 4 + kScratchPub * 8;  // offset of the work vectors
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_ptr[kMaxDevices];
+static int g_scratch_epoch[kMaxDevices];
+
+__global__ void k_fill_zero_u32(long long n, unsigned* __restrict__ x) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    x[i] = 0u;
+}
+
+int noalloc_init(int device, std::string* err) {
+  if (device < 0 || device >= kMaxDevices) {
+    *err = "device index beyond the allocation-free workspace table";
+    return QD_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return QD_OK;
+  void* ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    *err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    return QD_ERR_HIP;
+  }
+  hipLaunchKernelGGL(k_fill_zero_u32, dim3(1024), dim3(256), 0, nullptr, (long long)(kScratchBytes / 4),
+                     static_cast<unsigned*>(ptr));
+  if ((e = hipGetLastError()) == hipSuccess) e = hipStreamSynchronize(nullptr);
+  if (e != hipSuccess) {
+    *err = std::string("clearing the allocation-free workspace: ") + hipGetErrorString(e);
+    return QD_ERR_HIP;
+  }
+  g_scratch_ptr[device] = static_cast<unsigned char*>(ptr);
+  g_scratch_epoch[device] = 0;
+  return QD_OK;
+}
+
+namespace {
+// Points a handle buffer at a slice of the workspace for one call, restoring
+// the buffer afterwards.
+template <typename T>
+struct ViewIn {
+  DevBuf<T>& b;
+  T* p0;
+  size_t c0;
+  bool v0;
+  ViewIn(DevBuf<T>& buf, unsigned char* ptr, size_t cnt) : b(buf), p0(buf.p), c0(buf.count), v0(buf.view) {
+    b.p = reinterpret_cast<T*>(ptr);
+    b.count = cnt;
+    b.view = true;
+  }
+  ~ViewIn() {
+    b.p = p0;
+    b.count = c0;
+    b.view = v0;
+  }
+};
+struct ForbidAlloc {
+  qd_context* h;
+  explicit ForbidAlloc(qd_context* hh) : h(hh) { h->forbid_alloc = true; }
+  ~ForbidAlloc() { h->forbid_alloc = false; }
+};
+}  // namespace
+
+int lu_solve_noalloc(qd_context* h, int precision, int nrhs, double* b, long long ldb, double tol, int max_iter,
+                     qd_solve_report* rep) {
+  qd_lu_state* s = h->lu;
+  if (!s || !s->valid || s->precision != precision)
+    return set_error(h, QD_ERR_STATE,
+                     "qd_solve_device_noalloc needs the factors of an earlier qd_solve of precision %d on this handle",
+                     precision);
+  if (h->device < 0 || h->device >= kMaxDevices) return set_error(h, QD_ERR_ARG, "device %d", h->device);
+  unsigned char* base;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    base = g_scratch_ptr[h->device];
+  }
+  if (!base) return set_error(h, QD_ERR_STATE, "allocation-free workspace not set up on device %d", h->device);
+  // the slices one call of lu_solve (at most 8 right-hand sides at a time)
+  // asks for: QD_ALLOC finds each big enough and allocates nothing
+  const long long N = s->N;
+  const long long r = std::min(nrhs, 8);
+  const bool f64 = precision == QD_FP64;
+  const long long TB = f64 ? TriTB<double>::v : TriTB<float>::v;
+  const long long esz = f64 ? 8 : 4;
+  const long long nblk = (N + TB - 1) / TB;
+  const long long nslice = (N + kResSlice - 1) / kResSlice;
+  const size_t n_vec = (size_t)(5 * r * N), n_vecf = (size_t)(r * N);
+  const size_t n_rowpart = (size_t)std::max<long long>((nslice * r + 1) * N, 512);
+  const size_t n_nrm = (size_t)8 * kNormBlocks, n_trs = (size_t)(N * r * 8);
+  const size_t n_flags = (size_t)(1 + nblk * (1 + kLbParts)), n_part = (size_t)(nblk * kLbParts * TB * esz);
+  const size_t n_pub = f64 ? 0 : (size_t)(nblk * TB), n_gm = (size_t)(kGmresM + 3) * N;
+  // flags and published words at fixed offsets; the vectors after them
+  unsigned char* p_flags = base;
+  unsigned char* p_pub = base + kScratchFlags * 4;
+  size_t off = kScratchVectors;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off += (bytes + 255) / 256 * 256;
+    return base + o;
+  };
+  unsigned char* p_vec = take(n_vec * 8);
+  unsigned char* p_vecf = take(n_vecf * 4);
+  unsigned char* p_rowpart = take(n_rowpart * 8);
+  unsigned char* p_nrm = take(n_nrm * 8);
+  unsigned char* p_trs = take(n_trs);
+  unsigned char* p_part = take(n_part);
+  unsigned char* p_gm = take(n_gm * 8);
+  if (off > kScratchBytes || n_flags > kScratchFlags || n_pub > kScratchPub)
+    return set_error(h, QD_ERR_ARG,
+                     "qd_solve_device_noalloc: N=%lld with %lld right-hand sides at a time needs %zu bytes of "
+                     "workspace (%zu flags, %zu published words); the static workspace holds %zu (%zu, %zu)",
+                     N, r, off, n_flags, n_pub, kScratchBytes, kScratchFlags, kScratchPub);
+  ViewIn<double> v1(s->vec, p_vec, n_vec);
+  ViewIn<float> v2(s->vecf, p_vecf, n_vecf);
+  ViewIn<double> v3(s->rowpart, p_rowpart, n_rowpart);
+  ViewIn<double> v4(s->nrm, p_nrm, n_nrm);
+  ViewIn<unsigned char> v5(s->trs_tmp, p_trs, n_trs);
+  ViewIn<int> v6(s->lb_flags, p_flags, n_flags);
+  ViewIn<unsigned char> v7(s->lb_part, p_part, n_part);
+  ViewIn<unsigned long long> v8(s->lb_pub, p_pub, n_pub);
+  ViewIn<double> v9(s->gm, p_gm, n_gm);
+  ForbidAlloc guard(h);
+  // the workspace's flags follow the device's epochs, not the handle's
+  const int own_epoch = s->lb_epoch;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    s->lb_epoch = g_scratch_epoch[h->device];
+  }
+  const int rc = lu_solve(h, precision, nrhs, b, ldb, tol, max_iter, false, rep);
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    g_scratch_epoch[h->device] = std::max(g_scratch_epoch[h->device], s->lb_epoch);
+  }
+  s->lb_epoch = own_epoch;
+  return rc;
 }
 
 int fp16_update(qd_context* h, long long M, long long N, int Kp, const void* A16, const void* B16,

@@ -119,6 +119,17 @@ int qd_solve(qd_handle h, int precision, int nrhs, double* b, int ldb, double to
 /* Same with device right-hand sides (overwritten by the solutions). */
 int qd_solve_device(qd_handle h, int precision, int nrhs, double* b_dev, long long ldb,
                     double tol, int max_iter, qd_solve_report* rep);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (serving loops that solve new right-hand sides against factors
+ * already held, code that must not call hipMalloc): qd_solve_device with the
+ * factors of an earlier qd_solve / qd_solve_device of the same precision on
+ * this handle (QD_ERR_STATE otherwise), and every work vector in one static
+ * 64 MiB device workspace per GPU, so the call allocates nothing.  The
+ * workspace holds N up to about 100,000 solved one right-hand side at a time,
+ * about 36,000 with eight; beyond that the call returns QD_ERR_ARG.  Calls
+ * that overlap on one GPU share the workspace and race. */
+int qd_solve_device_noalloc(qd_handle h, int precision, int nrhs, double* b_dev, long long ldb,
+                            double tol, int max_iter, qd_solve_report* rep);
 /* Adjoint solve A^T y = c with the factors of the last qd_solve of this
  * precision (no reference counterpart: the gradient of the solve). */
 int qd_solve_transpose_device(qd_handle h, int precision, int nrhs, double* c_dev,
