@@ -66,6 +66,11 @@ _SIGNATURES = {
     "qd_internal_points": (_i, [_vp, _i, _pd, _pd, _pd, _pd, _pd, _i, _pd, _pd, _pd, _pd, _pll]),
     "qd_internal_points_device": (_i, [_vp, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _pll]),
     "qd_boundary_stresses": (_i, [_vp, _pd, _pd, _pd, _pd, _pd, _pd]),
+    "qd_write_input_echo": (_i, [_vp, C.c_char_p, C.c_char_p, _i, _pd, _pd, _pd, _i]),
+    "qd_output_data": (_i, [_vp, C.c_char_p, _pd, _pd, _i, _pd]),
+    "qd_output_boundary_stresses": (_i, [_vp, C.c_char_p, C.c_char_p, _pd, _pd, _pd, _pd]),
+    "qd_output_internal_points": (_i, [_vp, C.c_char_p, C.c_char_p, _i, _pd, _pd, _pd, _pd, _pd, _pd, _pd]),
+    "qd_save_disp_trac": (_i, [_vp, C.c_char_p, _i, _i, _pd, _pd]),
     "qd_set_coordinates": (_i, [_vp, _pd, _pd, _pd]),
     "qd_solve_transpose": (_i, [_vp, _i, _i, _pd, _i, _d, _i, C.POINTER(SolveReport)]),
     "qd_ghmat8_vjp": (_i, [_vp, _pd, _pd, _pd, _pd, _pd, _pd, _pd]),

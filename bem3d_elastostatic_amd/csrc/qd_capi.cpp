@@ -255,21 +255,24 @@ int qd_set_mesh(qd_handle h, int n, int ne, const double* x, const double* y, co
   h->hkconec.assign(kconec, kconec + 9 * (size_t)ne);
   for (int e = 0; e < ne; ++e)
     for (int k = h->hnconec[e]; k < 9; ++k) h->hkconec[e * 9 + k] = 0;
-  // io_mod.f90:368-378
+  // nesim and the elements of each node in ascending order (io_mod.f90:124-135).
+  // The reference keeps the first 8 in ke(8, n) and its kode0 search and
+  // reorganize_solution then read past ke for a node in more than 8
+  // elements (assembly_mod.f90:150-151, io_mod.f90:170-171); the list here
+  // holds every element, so those searches see them all.
   h->nesim.assign(n, 0);
-  h->ke.assign(8 * (size_t)n, 0);
   for (int e = 0; e < ne; ++e)
-    for (int j = 0; j < h->hnconec[e]; ++j) {
-      int nd = h->hkconec[e * 9 + j] - 1;
-      h->nesim[nd] += 1;
-      if (h->nesim[nd] <= 8) h->ke[nd * 8 + h->nesim[nd] - 1] = e + 1;
-    }
+    for (int j = 0; j < h->hnconec[e]; ++j) h->nesim[h->hkconec[e * 9 + j] - 1] += 1;
+  h->ke_ptr.assign(n + 1, 0);
   for (int i = 0; i < n; ++i) {
-    if (h->nesim[i] > 8)
-      return set_error(h, QD_ERR_MESH,
-                       "node %d is shared by %d elements; the reference's ke(8, n) holds 8 "
-                       "(io_mod.f90:131-133)", i + 1, h->nesim[i]);
     if (h->nesim[i] == 0) return set_error(h, QD_ERR_MESH, "node %d belongs to no element", i + 1);
+    h->ke_ptr[i + 1] = h->ke_ptr[i] + h->nesim[i];
+  }
+  h->ke.assign((size_t)h->ke_ptr[n], 0);
+  {
+    std::vector<int> fill(h->ke_ptr.begin(), h->ke_ptr.end() - 1);
+    for (int e = 0; e < ne; ++e)
+      for (int j = 0; j < h->hnconec[e]; ++j) h->ke[fill[h->hkconec[e * 9 + j] - 1]++] = e + 1;
   }
   // singular pairs, node-major, elements ascending (the ke order)
   h->h_pairs.clear();
@@ -277,7 +280,7 @@ int qd_set_mesh(qd_handle h, int n, int ne, const double* x, const double* y, co
   for (int i = 0; i < n; ++i) {
     h->h_pair_ptr[i] = (int)h->h_pairs.size();
     for (int j = 0; j < h->nesim[i]; ++j) {
-      int e = h->ke[i * 8 + j] - 1;
+      int e = h->ke[h->ke_ptr[i] + j] - 1;
       int nodo = 0;
       for (int k = 0; k < h->hnconec[e]; ++k)
         if (h->hkconec[e * 9 + k] - 1 == i) {
@@ -352,7 +355,7 @@ int qd_set_material(qd_handle h, double mu, double nu) {
 // kode0 / vp search of assembly_mod.f90:149-161 for node nd (0-based).
 static int find_vp(qd_context* h, const int* kode, const double* bc, int nd, int ld, double* vp) {
   for (int i = 0; i < h->nesim[nd]; ++i) {
-    int e = h->ke[nd * 8 + i] - 1;
+    int e = h->ke[h->ke_ptr[nd] + i] - 1;
     if (kode[e * 3 + ld] == 0) {
       for (int m = 0; m < h->hnconec[e]; ++m)
         if (h->hkconec[e * 9 + m] - 1 == nd) {
@@ -639,7 +642,7 @@ int qd_reorganize_solution(qd_handle h, const double* x, double* fi, double* dfi
   for (int in = 0; in < n; ++in)
     for (int ld = 0; ld < 3; ++ld)
       for (int j = 0; j < h->nesim[in]; ++j) {
-        int e = h->ke[in * 8 + j] - 1;
+        int e = h->ke[h->ke_ptr[in] + j] - 1;
         if (kode[e * 3 + ld] == 0) {
           for (int m = 0; m < h->hnconec[e]; ++m)
             if (h->hkconec[e * 9 + m] - 1 == in) {

@@ -117,7 +117,8 @@ struct qd_context {
   std::vector<double> hx, hy, hz;
   std::vector<int> hnconec, hkconec, hkode;
   std::vector<double> hbc;
-  std::vector<int> nesim, ke;  // ke(8, n) 1-based element numbers
+  std::vector<int> nesim;      // elements sharing each node
+  std::vector<int> ke_ptr, ke;  // elements of node i: ke[ke_ptr[i] .. ke_ptr[i+1]), 1-based, ascending
   qd::Material mat{};
   bool have_mesh = false, have_mat = false, have_bc = false, have_geom = false, have_sys = false;
 

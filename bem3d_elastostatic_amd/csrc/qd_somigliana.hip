@@ -570,16 +570,18 @@ __global__ void k_bstress_elem(int ne, const int* __restrict__ nconec, const int
   }
 }
 
-__global__ void k_bstress_node(int n, const int* __restrict__ nesim, const int* __restrict__ ke,
+// Nodal average over the first min(nesim, 8) elements of the node (in
+// ascending order), as stress_mod.f90:385-397 averages over ke(1:8, node).
+__global__ void k_bstress_node(int n, const int* __restrict__ ke_ptr, const int* __restrict__ ke,
                                const int* __restrict__ nconec, const int* __restrict__ kconec0,
                                const double* __restrict__ se, double* __restrict__ sn,
                                double* __restrict__ pr, double* __restrict__ vm) {
   const int in = blockIdx.x * blockDim.x + threadIdx.x;
   if (in >= n) return;
   double s[6] = {0, 0, 0, 0, 0, 0};
-  const int ns = min(nesim[in], 8);
+  const int ns = min(ke_ptr[in + 1] - ke_ptr[in], 8);
   for (int j = 0; j < ns; ++j) {
-    const int ie = ke[in * 8 + j] - 1;
+    const int ie = ke[ke_ptr[in] + j] - 1;
     for (int k = 0; k < nconec[ie]; ++k)
       if (kconec0[ie * 9 + k] == in) {
         for (int d = 0; d < 6; ++d) s[d] += se[(size_t)ie * 54 + 6 * k + d];
@@ -594,14 +596,14 @@ __global__ void k_bstress_node(int n, const int* __restrict__ nesim, const int* 
 
 int boundary_stress_run(qd_context* h, const double* fi, const double* dfi, double* se, double* sn,
                         double* pr, double* vm) {
-  DevBuf<int> dnesim, dke;
-  QD_ALLOC(h, dnesim, (size_t)h->n);
-  QD_ALLOC(h, dke, 8 * (size_t)h->n);
-  QD_HIP(h, hipMemcpy(dnesim.p, h->nesim.data(), h->n * sizeof(int), hipMemcpyHostToDevice));
-  QD_HIP(h, hipMemcpy(dke.p, h->ke.data(), 8 * (size_t)h->n * sizeof(int), hipMemcpyHostToDevice));
+  DevBuf<int> dptr, dke;
+  QD_ALLOC(h, dptr, (size_t)h->n + 1);
+  QD_ALLOC(h, dke, h->ke.size());
+  QD_HIP(h, hipMemcpy(dptr.p, h->ke_ptr.data(), (h->n + 1) * sizeof(int), hipMemcpyHostToDevice));
+  QD_HIP(h, hipMemcpy(dke.p, h->ke.data(), h->ke.size() * sizeof(int), hipMemcpyHostToDevice));
   hipLaunchKernelGGL(k_bstress_elem, dim3((h->ne + 63) / 64), dim3(64), 0, h->stream, h->ne, h->nconec.p,
                      h->kconec0.p, h->x.p, h->y.p, h->z.p, fi, dfi, h->mat.mu, h->mat.nu, se);
-  hipLaunchKernelGGL(k_bstress_node, dim3((h->n + 127) / 128), dim3(128), 0, h->stream, h->n, dnesim.p,
+  hipLaunchKernelGGL(k_bstress_node, dim3((h->n + 127) / 128), dim3(128), 0, h->stream, h->n, dptr.p,
                      dke.p, h->nconec.p, h->kconec0.p, se, sn, pr, vm);
   QD_HIP(h, hipGetLastError());
   QD_HIP(h, hipStreamSynchronize(h->stream));

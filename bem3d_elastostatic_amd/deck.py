@@ -8,10 +8,10 @@ Layout (list-directed Fortran reads, so commas and blanks both separate):
 3. ``n`` lines ``node x y z``;
 4. ``ne`` lines ``elem nconec k1 .. k_nconec``;
 5. ``3*ne`` lines ``elem kode v1 .. v_nconec`` (direction x, y, z per element,
-   io_mod.f90:342-349); legacy lines hold complex pairs ``(re, im)`` of which
-   the real part is kept (io_mod.f90:262-270);
-6. ``3*l`` reals: the internal points (read as one list, io_mod.f90:360);
-7. optionally the tracked node (io_mod.f90:381-384, default 1).
+   io_mod.f90:97-113); legacy lines hold complex pairs ``(re, im)`` of which
+   the real part is kept (io_mod.f90:18-27);
+6. ``3*l`` reals: the internal points (read as one list, io_mod.f90:115-117);
+7. optionally the tracked node (io_mod.f90:137-140, default 1).
 
 The writer emits full precision (``%.17g``): the reference's own preprocessor
 rounds coordinates to 4 decimals and mu/nu to 1/2 decimals, which is a

@@ -6,7 +6,7 @@ The solver consumes the same arrays the reference keeps as module globals
 ``nconec(ne)`` / ``kconec(9, ne)`` (1-based node numbers, Fortran column-major,
 so element ``e`` occupies ``kconec[e, :]`` in a C-ordered ``(ne, 9)`` array),
 per-element boundary-condition codes ``kode(3*ne)`` (0 = displacement
-prescribed, 1 = traction prescribed, io_mod.f90:339-353) and prescribed values
+prescribed, 1 = traction prescribed, io_mod.f90:97-113) and prescribed values
 ``bc(27, ne)`` stored as ``bc[e, 3*k + ld]``.
 
 Node orderings follow the reference's element definitions:
@@ -302,9 +302,79 @@ def config4(amplitude: float = 0.02, seed: int = 3, npoints: int = 4096) -> BemP
     return p
 
 
+def unit_cube_fan_top(m: int = 4, young: float = 1000.0, nu: float = 0.3, points=None) -> BemProblem:
+    """Unit cube with m x m quad9 per face, except the top face z=1, which is a
+    fan of 4m tri6 elements around its centre: the centre node belongs to 4m
+    elements (16 for m = 4), more than the reference's ke(8, n) holds
+    (io_mod.f90:131).  Boundary conditions are config 0's uniaxial case
+    (exact solution u = (-nu x, -nu y, z)/E) except that the second half of
+    the fan, in element order, prescribes u_z = 1/E instead of t_z = 1: the
+    centre's first displacement-prescribing element (assembly_mod.f90:149-161)
+    is then its (2m+1)-th."""
+    coords, nconec, kconec, face_of = box_mesh((0, 0, 0), (1, 1, 1), m, m, m)
+    keep = np.array([f != "z1" for f in face_of])
+    nconec, kconec = nconec[keep], kconec[keep]
+    face_of = [f for f, k in zip(face_of, keep) if k]
+    used = np.unique(kconec[kconec > 0])
+    # perimeter of the top face, counter-clockwise seen from +z: (vertex, mid) pairs
+    h = 1.0 / m
+    ring = []
+    for i in range(m):                      # y = 0, x increasing
+        ring.append(((i * h, 0.0), ((i + 0.5) * h, 0.0)))
+    for i in range(m):                      # x = 1, y increasing
+        ring.append(((1.0, i * h), (1.0, (i + 0.5) * h)))
+    for i in range(m):                      # y = 1, x decreasing
+        ring.append(((1.0 - i * h, 1.0), (1.0 - (i + 0.5) * h, 1.0)))
+    for i in range(m):                      # x = 0, y decreasing
+        ring.append(((0.0, 1.0 - i * h), (0.0, 1.0 - (i + 0.5) * h)))
+    pts = [tuple(coords[j - 1]) for j in used]
+    index = {}
+    for j, c in zip(used, pts):
+        index[(round(c[0], 12), round(c[1], 12), round(c[2], 12))] = len(index) + 1
+    old_to_new = {int(j): k + 1 for k, j in enumerate(used)}
+    kconec = np.vectorize(lambda v: old_to_new.get(int(v), 0))(kconec).astype(np.int32)
+    xyz = list(pts)
+
+    def node(x, y, z=1.0):
+        key = (round(x, 12), round(y, 12), round(z, 12))
+        if key not in index:
+            xyz.append((x, y, z))
+            index[key] = len(xyz)
+        return index[key]
+
+    centre = node(0.5, 0.5)
+    nk = len(ring)
+    fan_k = np.zeros((nk, 9), dtype=np.int32)
+    for k in range(nk):
+        (v0, mid), (v1, _) = ring[k], ring[(k + 1) % nk]
+        a, b = node(*v0), node(*v1)
+        sa = node(0.5 * (0.5 + v0[0]), 0.5 * (0.5 + v0[1]))
+        sb = node(0.5 * (0.5 + v1[0]), 0.5 * (0.5 + v1[1]))
+        # vertices 1-2-3 counter-clockwise from +z, then mid-sides 1-2, 2-3, 3-1
+        fan_k[k, :6] = (centre, a, b, sa, node(*mid), sb)
+    nconec = np.concatenate([nconec, np.full(nk, 6, dtype=np.int32)])
+    kconec = np.concatenate([kconec, fan_k])
+    face_of = face_of + ["z1"] * nk
+    coords = np.array(xyz, dtype=np.float64)
+    bcs = {"z0": ((1, 1, 0), (0.0, 0.0, 0.0)), "x0": ((0, 1, 1), (0.0, 0.0, 0.0)),
+           "y0": ((1, 0, 1), (0.0, 0.0, 0.0)), "z1": ((1, 1, 1), (0.0, 0.0, 1.0))}
+    kode, bc = _bc_arrays(nconec, face_of, bcs)
+    ne = nconec.shape[0]
+    for e in range(ne - nk // 2, ne):
+        kode[e] = (1, 1, 0)
+        bc[e, 2::3] = 0.0
+        bc[e, 2:3 * 6:3] = 1.0 / young
+    if points is None:
+        g = np.array([0.25, 0.5, 0.75])
+        points = np.array([[a, b, c] for a in g for b in g for c in g])
+    return BemProblem(f"UNIT CUBE {m}x{m} QUAD9, TOP FACE A FAN OF {nk} TRI6", coords[:, 0].copy(),
+                      coords[:, 1].copy(), coords[:, 2].copy(), nconec, kconec, kode, bc,
+                      _mu_from_e(young, nu), nu, points)
+
+
 def adjacency(p: BemProblem, cap: int = 8):
     """``nesim(n)`` and ``ke(n, cap)`` (1-based element numbers, ascending),
-    exactly as io_mod.f90:368-378 fills them (entries beyond ``cap`` dropped)."""
+    exactly as io_mod.f90:124-135 fills them (entries beyond ``cap`` dropped)."""
     nesim = np.zeros(p.n, dtype=np.int32)
     ke = np.zeros((p.n, cap), dtype=np.int32)
     for e in range(p.ne):

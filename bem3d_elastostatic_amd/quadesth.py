@@ -188,6 +188,46 @@ class Quadesth:
         return dict(sigma_elem=se.reshape(self.ne, 9, 6), sigma=sn.reshape(self.n, 6),
                     prin=pr.reshape(self.n, 3), vm=vm)
 
+    def write_outputs(self, directory, FI, DFI, internal=None, boundary=None, ntrack=None,
+                      fileout: str = "cube8_out.dat"):
+        """The reference's result files in ``directory``, in main.f90:75-80's
+        order: ``fileout`` (the deck echo of io_mod.f90:44-140, output_data,
+        the boundary- and internal-stress tables), stresses.dat,
+        internal_points.dat and disp_trac.dat.  ``internal`` is
+        compute_internal_points' dict (u, sigma, prin, vm) for the problem's
+        points, ``boundary`` compute_boundary_stresses' dict (computed here
+        when None).  Returns the paths written."""
+        import os
+        os.makedirs(directory, exist_ok=True)
+        p = self.p
+        out = os.path.join(directory, fileout).encode()
+        spath = os.path.join(directory, "stresses.dat").encode()
+        ipath = os.path.join(directory, "internal_points.dat").encode()
+        tpath = os.path.join(directory, "disp_trac.dat").encode()
+        l = p.l if internal is not None else 0
+        pts = _f64(p.points if l else np.zeros((1, 3)))
+        cx, cy, cz = _f64(pts[:, 0]), _f64(pts[:, 1]), _f64(pts[:, 2])
+        nt = int(p.ntrack if ntrack is None else ntrack)
+        check(self.h, self.lib.qd_write_input_echo(self.h, out, p.title.encode(), l, _ptr(cx), _ptr(cy), _ptr(cz), nt),
+              "write_input_echo")
+        fi, dfi = _f64(FI), _f64(DFI).reshape(-1)
+        u = _f64(internal["u"]).reshape(-1) if l else np.zeros(3)
+        check(self.h, self.lib.qd_output_data(self.h, out, _ptr(fi), _ptr(dfi), l, _ptr(u)), "output_data")
+        b = boundary if boundary is not None else self.compute_boundary_stresses(FI, DFI)
+        check(self.h, self.lib.qd_output_boundary_stresses(
+            self.h, out, spath, _ptr(_f64(b["sigma_elem"]).reshape(-1)), _ptr(_f64(b["sigma"]).reshape(-1)),
+            _ptr(_f64(b["prin"]).reshape(-1)), _ptr(_f64(b["vm"]))), "output_boundary_stresses")
+        paths = [out.decode(), spath.decode()]
+        if l:
+            check(self.h, self.lib.qd_output_internal_points(
+                self.h, out, ipath, l, _ptr(cx), _ptr(cy), _ptr(cz), _ptr(u),
+                _ptr(_f64(internal["sigma"]).reshape(-1)), _ptr(_f64(internal["prin"]).reshape(-1)),
+                _ptr(_f64(internal["vm"]))), "output_internal_points")
+            paths.append(ipath.decode())
+        check(self.h, self.lib.qd_save_disp_trac(self.h, tpath, 0, nt, _ptr(fi), _ptr(dfi)), "save_disp_trac")
+        paths.append(tpath.decode())
+        return paths
+
     def run(self, precision: str = "fp64", variant: int = 0, points=None, tol: float = 1e-14):
         """The whole hot path in main.f90's order; returns every result."""
         self.precompute_element_geometry()

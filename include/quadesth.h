@@ -33,10 +33,11 @@ enum {
   QD_ERR_ARG = -1,     /* bad argument */
   QD_ERR_HIP = -2,     /* HIP runtime error */
   QD_ERR_STATE = -3,   /* called before the phase it depends on */
-  QD_ERR_MESH = -4,    /* invalid mesh, e.g. a node shared by more than 8
-                          elements (overflows ke(8, n), io_mod.f90:131-133) */
+  QD_ERR_MESH = -4,    /* invalid mesh: nconec not 9 or 6, a node number out
+                          of range, a node in no element */
   QD_ERR_NOMEM = -5,   /* device allocation failed */
-  QD_ERR_NOCONV = -6   /* iterative refinement missed its tolerance */
+  QD_ERR_NOCONV = -6,  /* iterative refinement missed its tolerance */
+  QD_ERR_IO = -7       /* an output file could not be written */
 };
 
 /* Precision of the factorisation used by qd_solve.  Residuals of the
@@ -82,7 +83,7 @@ int qd_synchronize(qd_handle h);
 
 /* ---- problem data ---------------------------------------------------------
  * Replaces input_data's fill of the data_mod globals (io_mod.f90:36-146) and
- * the nesim/ke adjacency (io_mod.f90:368-378).  Copies to the device. */
+ * the nesim/ke adjacency (io_mod.f90:124-135).  Copies to the device. */
 int qd_set_mesh(qd_handle h, int n, int ne, const double* x, const double* y, const double* z,
                 const int* nconec, const int* kconec);
 /* update_material_constants (data_mod.f90:97-108): mu is the shear modulus
@@ -156,6 +157,36 @@ int qd_internal_points_device(qd_handle h, int l, const double* pts_dev, const d
  * (6, n), principal (3, n), von Mises (n); sigma_elem (6, 9, ne) nullable. */
 int qd_boundary_stresses(qd_handle h, const double* fi, const double* dfi, double* sigma_elem,
                          double* sigma_node, double* prin, double* vm);
+
+/* ---- text outputs --------------------------------------------------------
+ * The reference's result files, record for record in its FORMATs, from host
+ * arrays in its layouts and the handle's mesh, material and codes.  Paths
+ * are plain file names; NULL skips that file where noted. */
+/* The deck echo input_data writes at the top of the output file
+ * (io_mod.f90:44-140; the file is created or truncated).  title: the deck's
+ * first line; cx, cy, cz the l internal points; ntrack the tracked node. */
+int qd_write_input_echo(qd_handle h, const char* path, const char* title, int l, const double* cx,
+                        const double* cy, const double* cz, int ntrack);
+/* output_data (io_mod.f90:191-228), appended to path: boundary
+ * displacements fi (3n), element tractions dfi (27, ne), internal
+ * displacements dsol (3l). */
+int qd_output_data(qd_handle h, const char* path, const double* fi, const double* dfi, int l, const double* dsol);
+/* output_boundary_stresses (stress_mod.f90:412-468): the nodal and
+ * element-nodal tables appended to path (nullable) and stresses.dat written
+ * to stresses_path (nullable), from qd_boundary_stresses' outputs. */
+int qd_output_boundary_stresses(qd_handle h, const char* path, const char* stresses_path,
+                                const double* sigma_elem, const double* sigma_node, const double* prin,
+                                const double* vm);
+/* output_internal_points (stress_mod.f90:892-936): the stress table appended
+ * to path (nullable) and internal_points.dat written to points_path
+ * (nullable); nothing for l == 0. */
+int qd_output_internal_points(qd_handle h, const char* path, const char* points_path, int l, const double* cx,
+                              const double* cy, const double* cz, const double* u_int,
+                              const double* sigma_int, const double* prin, const double* vm);
+/* save_disp_trac (io_mod.f90:230-255): the tracked node's displacement and
+ * its traction in its first element, one record (appended when append != 0,
+ * else the file is created or truncated). */
+int qd_save_disp_trac(qd_handle h, const char* path, int append, int ntrack, const double* fi, const double* dfi);
 
 /* ---- derivatives (no reference counterpart: the reference computes no
  * gradients).  Together with qd_solve_transpose(_device) they give the adjoint

@@ -56,6 +56,8 @@ class Oracle:
         lib.qo_material.argtypes = [C.c_double, C.c_double, C.POINTER(_Material)]
         lib.qo_precompute.argtypes = [C.c_int, _ip, _ip, _dp, _dp, _dp] + [_dp] * 8
         lib.qo_adjacency.argtypes = [C.c_int, C.c_int, _ip, _ip, _ip, _ip]
+        lib.qo_set_ke_cap.argtypes = [C.c_int]
+        lib.qo_ke_cap.restype = C.c_int
         lib.qo_ghmat8.argtypes = ([C.c_int, C.c_int, _dp, _dp, _dp, _ip, _ip, _ip, _dp, _ip, _ip,
                                    C.POINTER(_Material)] + [_dp] * 8 + [_dp, _dp])
         lib.qo_ghmat8_rows.argtypes = ([C.c_int, C.c_int, _dp, _dp, _dp, _ip, _ip, _ip, _dp, _ip, _ip,
@@ -120,8 +122,15 @@ class Oracle:
         return [t[k] for k in ("eta1", "eta2", "eta3", "xja", "xg1", "xg2", "xg3", "ff")]
 
     def adjacency(self, p):
+        """nesim and ke.  ke is as wide as the most-shared node needs (at least
+        the reference's 8), so every element of every node is listed; the
+        width is set in the C oracle for the calls that follow."""
+        counts = np.bincount(np.concatenate([p.kconec[e, :int(p.nconec[e])] for e in range(p.ne)]) - 1,
+                             minlength=p.n)
+        cap = max(8, int(counts.max()))
+        self.lib.qo_set_ke_cap(cap)
         nesim = np.zeros(p.n, dtype=np.int32)
-        ke = np.zeros(p.n * 8, dtype=np.int32)
+        ke = np.zeros(p.n * cap, dtype=np.int32)
         self.lib.qo_adjacency(p.n, p.ne, _i32(p.nconec), _i32(p.kconec), nesim, ke)
         return nesim, ke
 
@@ -247,15 +256,20 @@ class Reference:
         lib.qref_reorganize.argtypes = [_dp, _dp, _dp]
         lib.qref_internal.argtypes = [_dp, _dp, _dp, _dp]
         lib.qref_boundary.argtypes = [_dp, _dp, _dp]
+        lib.qref_outputs.argtypes = [C.c_char_p]
 
-    def run(self, deck_path: str, solve=None, boundary=False):
+    def run(self, deck_path: str, solve=None, boundary=False, outputs=False):
         """Run the reference on a deck file.  ``solve(A, b) -> x`` defaults to
-        numpy's LAPACK gesv."""
+        numpy's LAPACK gesv.  With ``outputs`` (implies ``boundary``) the
+        reference's writers run too and their files come back as
+        ``out["files"]``: name -> bytes."""
+        boundary = boundary or outputs
+        td_out = tempfile.mkdtemp(dir="/tmp") if outputs else None
         with tempfile.TemporaryDirectory(dir="/tmp") as td:
             short = os.path.join(td, "d.dat")
             with open(deck_path, "rb") as src, open(short, "wb") as dst:
                 dst.write(src.read())
-            outp = os.path.join(td, "o.dat")
+            outp = os.path.join(td_out or td, "cube8_out.dat")
             self.lib.qref_load(short.encode(), outp.encode())
         n, ne, l = C.c_int(), C.c_int(), C.c_int()
         mu, nu = C.c_double(), C.c_double()
@@ -280,4 +294,15 @@ class Reference:
             s, pr, vm = np.zeros(6 * n), np.zeros(3 * n), np.zeros(n)
             self.lib.qref_boundary(s, pr, vm)
             out.update(bsigma=s.reshape(n, 6), bprin=pr.reshape(n, 3), bvm=vm)
+        if outputs:
+            import shutil
+            self.lib.qref_outputs(td_out.encode())
+            files = {}
+            for name in ("cube8_out.dat", "stresses.dat", "internal_points.dat", "disp_trac.dat"):
+                path = os.path.join(td_out, name)
+                if os.path.exists(path):
+                    with open(path, "rb") as fh:
+                        files[name] = fh.read()
+            shutil.rmtree(td_out)
+            out["files"] = files
         return out

@@ -356,15 +356,24 @@ void qo_locin8(const qo_material_t* m, const double xj1[9], const double xj2[9],
   }
 }
 
-/* io_mod.f90:368-378 (ke capped at 8 entries per node) */
+/* Entries per node of ke.  8 is the reference's ke(8, n) (data_mod.f90:79,
+ * io_mod.f90:131); a larger width lets the kode0 search and the
+ * reorganisation see every element of a node shared by more than 8, where
+ * the reference reads past ke(8, .) (assembly_mod.f90:150-151,
+ * io_mod.f90:170-171). */
+static int g_kecap = 8;
+void qo_set_ke_cap(int cap) { g_kecap = cap > 8 ? cap : 8; }
+int qo_ke_cap(void) { return g_kecap; }
+
+/* io_mod.f90:124-135 (ke holds the first g_kecap elements of each node) */
 void qo_adjacency(int n, int ne, const int* nconec, const int* kconec, int* nesim, int* ke) {
   for (int i = 0; i < n; ++i) nesim[i] = 0;
-  for (int i = 0; i < 8 * n; ++i) ke[i] = 0;
+  for (long i = 0; i < (long)g_kecap * n; ++i) ke[i] = 0;
   for (int e = 0; e < ne; ++e)
     for (int j = 0; j < nconec[e]; ++j) {
       int node = kconec[e * 9 + j] - 1;
       nesim[node] += 1;
-      if (nesim[node] <= 8) ke[node * 8 + nesim[node] - 1] = e + 1;
+      if (nesim[node] <= g_kecap) ke[(long)node * g_kecap + nesim[node] - 1] = e + 1;
     }
 }
 
@@ -374,9 +383,9 @@ void qo_adjacency(int n, int ne, const int* nconec, const int* kconec, int* nesi
  * search here stops at the stored entries. */
 static int find_vp(int ik, int ld, const int* nconec, const int* kconec, const int* kode,
                    const double* bc, const int* nesim, const int* ke, double* vp) {
-  int lim = nesim[ik] < 8 ? nesim[ik] : 8;
+  int lim = nesim[ik] < g_kecap ? nesim[ik] : g_kecap;
   for (int i = 0; i < lim; ++i) {
-    int e = ke[ik * 8 + i] - 1;
+    int e = ke[(long)ik * g_kecap + i] - 1;
     if (kode[e * 3 + ld] == 0) {
       for (int mm = 0; mm < nconec[e]; ++mm)
         if (kconec[e * 9 + mm] - 1 == ik) {
@@ -627,9 +636,9 @@ void qo_reorganize(int n, int ne, const int* nconec, const int* kconec, const in
     }
   for (int in = 0; in < n; ++in)
     for (int ld = 0; ld < 3; ++ld) {
-      int lim = nesim[in] < 8 ? nesim[in] : 8;
+      int lim = nesim[in] < g_kecap ? nesim[in] : g_kecap;
       for (int j = 0; j < lim; ++j) {
-        int e = ke[in * 8 + j] - 1;
+        int e = ke[(long)in * g_kecap + j] - 1;
         if (kode[e * 3 + ld] == 0) {
           for (int mm = 0; mm < nconec[e]; ++mm)
             if (kconec[e * 9 + mm] - 1 == in) {
@@ -1059,10 +1068,10 @@ void qo_boundary_stresses(int n, int ne, const double* x, const double* y, const
   for (int in = 0; in < n; ++in) {
     double* sn = sigma_node + 6 * in;
     for (int d = 0; d < 6; ++d) sn[d] = 0.0;
-    int ns = nesim[in] < 8 ? nesim[in] : 8;
+    int ns = nesim[in] < 8 ? nesim[in] : 8;  /* stress_mod.f90:385: min(nesim, 8) */
     if (ns > 0) {
       for (int j = 0; j < ns; ++j) {
-        int ie = ke[in * 8 + j] - 1;
+        int ie = ke[(long)in * g_kecap + j] - 1;
         for (int k = 0; k < nconec[ie]; ++k)
           if (kconec[ie * 9 + k] - 1 == in) {
             for (int d = 0; d < 6; ++d) sn[d] += sigma_elem[(size_t)ie * 54 + 6 * k + d];

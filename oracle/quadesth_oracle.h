@@ -8,8 +8,8 @@
  * Array conventions are the reference's module globals (data_mod.f90:38-52),
  * Fortran column-major, with 0-based C offsets:
  *   x,y,z[n]; nconec[ne]; kconec[e*9+m] (1-based node numbers);
- *   kode[e*3+ld]; bc[e*27 + 3*k + ld]; nesim[n]; ke[node*8 + i] (1-based
- *   element numbers); A[col*N + row], N = 3n; FI[N]; DFI[e*27 + 3*k + ld];
+ *   kode[e*3+ld]; bc[e*27 + 3*k + ld]; nesim[n]; ke[node*cap + i] (1-based
+ *   element numbers, cap = qo_ke_cap(), 8 by default); A[col*N + row], N = 3n; FI[N]; DFI[e*27 + 3*k + ld];
  *   Gauss tables g[e*36 + k] for eta1..3, xja, xg1..3 and ff[(e*36+k)*9 + m].
  */
 #ifndef QUADESTH_ORACLE_H
@@ -47,6 +47,9 @@ void qo_extin8(const qo_material_t* m, const double* eta1, const double* eta2, c
 void qo_locin8(const qo_material_t* m, const double xj1[9], const double xj2[9], const double xj3[9],
                double h[81], double g[81], int nodo, int ncone);
 
+/* Width of ke per node (default 8, the reference's ke(8, n)); ke[node*cap + i]. */
+void qo_set_ke_cap(int cap);
+int qo_ke_cap(void);
 void qo_adjacency(int n, int ne, const int* nconec, const int* kconec, int* nesim, int* ke);
 
 void qo_ghmat8(int n, int ne, const double* x, const double* y, const double* z,

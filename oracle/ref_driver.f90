@@ -89,6 +89,39 @@ contains
     vm_out = sigma_vm
   end subroutine qref_boundary
 
+  ! The reference's writers into directory dir (NUL-terminated, at most 40
+  ! characters), in main.f90:75-80's order: output_data (io_mod.f90:191-228)
+  ! continues the file qref_load's input_data echoed the deck into (fileout,
+  ! reopened for appending), then output_boundary_stresses (stress_mod.f90:
+  ! 412-468, also writing stresses.dat), output_internal_points
+  ! (stress_mod.f90:892-936, internal_points.dat) and save_disp_trac
+  ! (io_mod.f90:230-255, disp_trac.dat).  Needs qref_reorganize,
+  ! qref_internal and qref_boundary first.
+  subroutine qref_outputs(dir) bind(C, name="qref_outputs")
+    use stress_mod, only: output_boundary_stresses, output_internal_points, filestress, fileintpts
+    use io_mod, only: output_data, save_disp_trac
+    character(kind=c_char), intent(in) :: dir(*)
+    character(len=64) :: d
+    integer :: i
+    d = ' '
+    i = 1
+    do while (dir(i) /= c_null_char .and. i <= 40)
+      d(i:i) = dir(i)
+      i = i + 1
+    end do
+    filestress = trim(d) // '/stresses.dat'
+    fileintpts = trim(d) // '/internal_points.dat'
+    filetrack = trim(d) // '/disp_trac.dat'
+    open(unit=ipr, file=trim(fileout), status='old', position='append', action='write')
+    open(unit=iout_track, file=trim(filetrack), status='replace', action='write')
+    call output_data()
+    call output_boundary_stresses()
+    if (l > 0) call output_internal_points()
+    call save_disp_trac()
+    close(ipr)
+    close(iout_track)
+  end subroutine qref_outputs
+
   subroutine qref_free() bind(C, name="qref_free")
     call deallocate_stress_data()
     call deallocate_bem_data()

@@ -58,14 +58,59 @@ def mixed_small():
     return p
 
 
+LEGACY_DECK = os.path.join(OUT, "legacy_mixed.dat")
+
+
+def _fort_d(v):
+    """A real in Fortran D-exponent form, full precision."""
+    return ("%.17E" % v).replace("E", "D")
+
+
+def write_legacy_deck(p, path):
+    """The mixed_small problem in the legacy deck layout the reference still
+    reads (io_mod.f90:51-61): an 8-value header ``n, ne, l, nfr, mu, dam, ro,
+    nu`` plus one frequency line, and complex ``(re, im)`` prescribed values
+    (io_mod.f90:18-27) whose imaginary parts the reader drops.  Some values
+    carry D exponents; the imaginary parts are arbitrary non-zero numbers."""
+    rng = np.random.default_rng(31)
+    with open(path, "w") as fh:
+        fh.write("LEGACY FORMAT: " + p.title + "\n")
+        fh.write(f"{p.n}, {p.ne}, {p.l}, 2, {_fort_d(p.mu)}, 0.05, 2500.0, {float(p.nu)!r}\n")
+        fh.write("  0.5000D+01  0.1200D+02\n")
+        for i in range(p.n):
+            fh.write(f"{i + 1} {float(p.x[i])!r} {float(p.y[i])!r} {float(p.z[i])!r}\n")
+        for e in range(p.ne):
+            nc = int(p.nconec[e])
+            fh.write(f"{e + 1} {nc} " + " ".join(str(int(v)) for v in p.kconec[e, :nc]) + "\n")
+        for e in range(p.ne):
+            nc = int(p.nconec[e])
+            for k in range(3):
+                pairs = []
+                for j in range(nc):
+                    re_ = p.bc[e, 3 * j + k]
+                    im_ = float(rng.uniform(-5.0, 5.0))
+                    re_s = _fort_d(re_) if (e + j) % 2 else repr(float(re_))
+                    pairs.append(f"({re_s},{im_!r})")
+                fh.write(f"{e + 1} {int(p.kode[e, k])} " + " ".join(pairs) + "\n")
+        for i in range(p.l):
+            fh.write(" ".join(repr(float(v)) for v in p.points[i]) + "\n")
+        fh.write("7\n")
+
+
+def legacy_mixed():
+    write_legacy_deck(mixed_small(), LEGACY_DECK)
+    return deck.read_deck(LEGACY_DECK)
+
+
 CASES = {
     "c0": lambda: mesh.config0(),
+    "legacy_mixed": legacy_mixed,
     "mixed_small": mixed_small,
     "cantilever_h05": lambda: mesh.config3(npoints=64, h=0.5),
     "cube8": lambda: deck.read_deck("/root/reference/cube8.dat"),
     "c1": lambda: mesh.config1(),
 }
-FULL_A = {"c0", "mixed_small"}
+FULL_A = {"c0", "mixed_small", "legacy_mixed"}
 ROW_SAMPLES = 24
 
 
@@ -74,7 +119,8 @@ def run_case(name, ref):
     with tempfile.TemporaryDirectory(dir="/tmp") as td:
         path = os.path.join(td, "deck.dat")
         deck.write_deck(p, path)
-        r = ref.run(path, boundary=True)
+        # the legacy case goes through the reference's reader as written
+        r = ref.run(LEGACY_DECK if name == "legacy_mixed" else path, boundary=True)
     N = 3 * p.n
     out = dict(x=p.x, y=p.y, z=p.z, nconec=p.nconec, kconec=p.kconec, kode=p.kode, bc=p.bc,
                points=p.points, mu=np.float64(r["mu"]), nu=np.float64(r["nu"]),
@@ -91,11 +137,36 @@ def run_case(name, ref):
         out["A_colsum"] = r["A"].sum(axis=0)
     if p.l > 0:
         out.update(u=r["u"], sigma=r["sigma"], prin=r["prin"], vm=r["vm"])
+    if name == "legacy_mixed":
+        out["ntrack"] = np.int64(p.ntrack)
     if name == "cube8":
         # the output file the reference ships for this deck (E15.7 columns)
         out["shipped_internal_points"] = np.loadtxt("/root/reference/internal_points.dat", comments="#")
     np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **out)
     print(name, "n", p.n, "ne", p.ne, "l", p.l, "written")
+
+
+# The reference's own writers (io_mod.f90:191-255, stress_mod.f90:412-468,
+# 892-936, and the echo of io_mod.f90:44-140) on two cases, their four files
+# kept byte for byte in outputs.npz as "<case>/<file>" uint8 arrays.
+OUTPUT_CASES = ("cube8", "mixed_small")
+
+
+def run_outputs(ref):
+    out = {}
+    for name in OUTPUT_CASES:
+        p = CASES[name]()
+        with tempfile.TemporaryDirectory(dir="/tmp") as td:
+            path = os.path.join(td, "deck.dat")
+            if name == "cube8":
+                path = "/root/reference/cube8.dat"
+            else:
+                deck.write_deck(p, path)
+            r = ref.run(path, outputs=True)
+        for fname, data in r["files"].items():
+            out[f"{name}/{fname}"] = np.frombuffer(data, dtype=np.uint8)
+    np.savez_compressed(os.path.join(OUT, "outputs.npz"), **out)
+    print("outputs", {k: v.size for k, v in out.items()})
 
 
 # Directional derivatives of J = sum over the internal points of vm^2 (the
@@ -144,9 +215,11 @@ def run_grad_case(name, ref):
 def main(argv):
     build_reference()
     ref = Reference()
-    names = argv or (list(CASES) + list(GRAD_CASES))
+    names = argv or (list(CASES) + list(GRAD_CASES) + ["outputs"])
     for name in names:
-        if name in GRAD_CASES:
+        if name == "outputs":
+            run_outputs(ref)
+        elif name in GRAD_CASES:
             run_grad_case(name, ref)
         else:
             run_case(name, ref)
