@@ -15,6 +15,7 @@ QD_OK = 0
 QD_FP64, QD_FP32_IR, QD_FP16_IR = 0, 1, 2
 PRECISIONS = {"fp64": QD_FP64, "fp32": QD_FP32_IR, "fp16": QD_FP16_IR}
 SOMIGLIANA_REFERENCE, SOMIGLIANA_CORRECTED = 0, 1
+ASSEMBLY = {"fp64": 0, "far_fp32": 1, "far_fp32_factor": 2}
 PROF_KINDS = {"lu_trailing_gemm": 0, "lu_panel": 1, "assembly_h": 2, "assembly_g": 3,
               "somigliana_far": 4, "somigliana_near": 5}
 
@@ -56,6 +57,7 @@ _SIGNATURES = {
     "qd_precompute_element_geometry": (_i, [_vp] + [_pd] * 8),
     "qd_ghmat8": (_i, [_vp]),
     "qd_ghmat8_rhs": (_i, [_vp, _pd, _pd]),
+    "qd_set_assembly_precision": (_i, [_vp, _i]),
     "qd_get_system": (_i, [_vp, _pd, _i, _pd]),
     "qd_system_device": (_i, [_vp, C.POINTER(_pd), _pll, C.POINTER(_pd)]),
     "qd_solve": (_i, [_vp, _i, _i, _pd, _i, _d, _i, C.POINTER(SolveReport)]),

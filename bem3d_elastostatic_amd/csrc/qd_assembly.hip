@@ -36,11 +36,12 @@ __constant__ double c_omei[4] = {0.1739274226, 0.3260725774, 0.3260725774, 0.173
 __constant__ double c_omej[4] = {0.1355069134, 0.2034645680, 0.1298475476, 0.0311809709};
 
 // precompute_element_geometry / etaxja (assembly_mod.f90:28-51, 225-337):
-// one thread per (element, Gauss slot).
+// one thread per (element, Gauss slot).  asmtab32 holds the same point's
+// normal and weight products in FP32 for the far-field passes.
 __global__ void k_geometry(int ne, const int* __restrict__ nconec, const int* __restrict__ kconec0,
                            const double* __restrict__ x, const double* __restrict__ y,
                            const double* __restrict__ z, double* __restrict__ geo,
-                           double* __restrict__ asmtab) {
+                           double* __restrict__ asmtab, float* __restrict__ asmtab32) {
   const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= (long long)ne * kGP) return;
   const int e = (int)(tid / kGP), k = (int)(tid % kGP);
@@ -101,6 +102,48 @@ __global__ void k_geometry(int ne, const int* __restrict__ nconec, const int* __
   t[5] = out[2];
   for (int m = 0; m < 9; ++m) t[6 + m] = p12[m];
   t[15] = 0.0;
+  float* t32 = asmtab32 + slot * kAsmStride32;
+  for (int c = 0; c < 3; ++c) t32[c] = (float)out[c];
+  for (int m = 0; m < 9; ++m) t32[3 + m] = (float)p12[m];
+  for (int c = 12; c < kAsmStride32; ++c) t32[c] = 0.0f;
+}
+
+// Far-field test data per element: centroid (mean of the nodes) and the
+// squared distance beyond which a collocation node is more than two element
+// diameters (twice the largest node-to-centroid distance) away.
+__global__ void k_elem_far(int ne, const int* __restrict__ nconec, const int* __restrict__ kconec0,
+                           const double* __restrict__ x, const double* __restrict__ y,
+                           const double* __restrict__ z, double* __restrict__ far) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int nc = nconec[e];
+  double c0 = 0.0, c1 = 0.0, c2 = 0.0;
+  for (int m = 0; m < nc; ++m) {
+    const int nd = kconec0[e * 9 + m];
+    c0 += x[nd];
+    c1 += y[nd];
+    c2 += z[nd];
+  }
+  c0 /= nc;
+  c1 /= nc;
+  c2 /= nc;
+  double r2 = 0.0;
+  for (int m = 0; m < nc; ++m) {
+    const int nd = kconec0[e * 9 + m];
+    const double d0 = x[nd] - c0, d1 = y[nd] - c1, d2 = z[nd] - c2;
+    r2 = fmax(r2, d0 * d0 + d1 * d1 + d2 * d2);
+  }
+  // distance > 2 diameters = 4 radii
+  far[4 * e] = c0;
+  far[4 * e + 1] = c1;
+  far[4 * e + 2] = c2;
+  far[4 * e + 3] = 16.0 * r2;
+}
+
+// The collocation node (xp, yp, zp) is in element e's far field.
+__device__ __forceinline__ bool is_far(const double* __restrict__ far, int e, double xp, double yp, double zp) {
+  const double d0 = xp - far[4 * e], d1 = yp - far[4 * e + 1], d2 = zp - far[4 * e + 2];
+  return d0 * d0 + d1 * d1 + d2 * d2 > far[4 * e + 3];
 }
 
 struct AsmArgs {
@@ -119,11 +162,16 @@ struct AsmArgs {
   double* fpart;
   double mu, c3, c4, c5, c6;
   int write_A;
+  const float* asmtab32;  // far-field passes: FP32 normals and weight products
+  const double* elfar;    // far-field passes: k_elem_far's table
 };
 
 // H pass: h_elem of extin8 (assembly_mod.f90:339-384) with funkelvin's
-// traction kernel (618-650), then ghmat8's H terms (100-190).
-template <int NC>
+// traction kernel (618-650), then ghmat8's H terms (100-190).  With FAR32,
+// a lane whose collocation node is in the element's far field evaluates the
+// kernel and sums the element integral in FP32 (r itself is formed in FP64);
+// the other lanes, and everything after the integral, stay FP64.
+template <int NC, bool FAR32>
 __global__ __launch_bounds__(64) void k_hpass(AsmArgs a, const int* __restrict__ chunks, int coff,
                                                const double* __restrict__ asmtab, const ElemInfo* __restrict__ einfo,
                                                const double* __restrict__ ebc_all, const double* __restrict__ evp_all,
@@ -154,6 +202,50 @@ __global__ __launch_bounds__(64) void k_hpass(AsmArgs a, const int* __restrict__
 #pragma unroll
         for (int jj = 0; jj < 3; ++jj) acc[kk][ii][jj] = 0.0;
     const double* gp = asmtab + (size_t)e * kGP * kAsmStride;
+    if (FAR32 && is_far(a.elfar, e, xp, yp, zp)) {
+      float fa[NC][3][3];
+#pragma unroll
+      for (int kk = 0; kk < NC; ++kk)
+#pragma unroll
+        for (int ii = 0; ii < 3; ++ii)
+#pragma unroll
+          for (int jj = 0; jj < 3; ++jj) fa[kk][ii][jj] = 0.0f;
+      const float* g32 = a.asmtab32 + (size_t)e * kGP * kAsmStride32;
+      const float c5f = (float)c5, c6f = (float)c6;
+#pragma unroll 1
+      for (int g = 0; g < NG; ++g) {
+        const double* gd = gp + (size_t)g * kAsmStride;
+        const float* gf = g32 + (size_t)g * kAsmStride32;
+        const float r1 = (float)(gd[0] - xp), r2 = (float)(gd[1] - yp), r3 = (float)(gd[2] - zp);
+        const float ir = __builtin_amdgcn_rsqf(r1 * r1 + r2 * r2 + r3 * r3);
+        const float eta[3] = {gf[0], gf[1], gf[2]};
+        const float rd[3] = {r1 * ir, r2 * ir, r3 * ir};
+        const float rdn = (r1 * eta[0] + r2 * eta[1] + r3 * eta[2]) * ir;
+        const float c5r2 = c5f * (ir * ir);
+        float t[3][3];
+#pragma unroll
+        for (int ii = 0; ii < 3; ++ii)
+#pragma unroll
+          for (int jj = 0; jj < 3; ++jj) {
+            const float dl = (ii == jj) ? c6f : 0.0f;
+            t[jj][ii] = c5r2 * (rdn * (dl + 3.0f * rd[ii] * rd[jj]) + c6f * (eta[jj] * rd[ii] - eta[ii] * rd[jj]));
+          }
+#pragma unroll
+        for (int kk = 0; kk < NC; ++kk) {
+          const float p = gf[3 + kk];
+#pragma unroll
+          for (int ii = 0; ii < 3; ++ii)
+#pragma unroll
+            for (int jj = 0; jj < 3; ++jj) fa[kk][ii][jj] += t[jj][ii] * p;
+        }
+      }
+#pragma unroll
+      for (int kk = 0; kk < NC; ++kk)
+#pragma unroll
+        for (int ii = 0; ii < 3; ++ii)
+#pragma unroll
+          for (int jj = 0; jj < 3; ++jj) acc[kk][ii][jj] = (double)fa[kk][ii][jj];
+    } else {
     // a Gauss point's data (position, normal, weights) are loaded one point
     // ahead, in flight during the previous point's arithmetic
     double cur[6 + NC];
@@ -190,6 +282,7 @@ __global__ __launch_bounds__(64) void k_hpass(AsmArgs a, const int* __restrict__
       }
 #pragma unroll
       for (int q = 0; q < 6 + NC; ++q) cur[q] = nxt[q];
+    }
     }
     if (valid && !sing) {
       const int kb = inf->kode, k0 = inf->k0mask;
@@ -259,8 +352,9 @@ __global__ __launch_bounds__(64) void k_hpass(AsmArgs a, const int* __restrict__
 }
 
 // G pass: g_elem of extin8 with funkelvin's displacement kernel, then the
-// G terms of ghmat8 (assembly_mod.f90:131-133, 164-166, 179-185).
-template <int NC>
+// G terms of ghmat8 (assembly_mod.f90:131-133, 164-166, 179-185).  FAR32 as
+// in the H pass.
+template <int NC, bool FAR32>
 __global__ __launch_bounds__(64) void k_gpass(AsmArgs a, const int* __restrict__ gchunks, int coff,
                                                const double* __restrict__ asmtab, const ElemInfo* __restrict__ einfo,
                                                const double* __restrict__ ebc_all, double* __restrict__ Aout) {
@@ -286,6 +380,36 @@ __global__ __launch_bounds__(64) void k_gpass(AsmArgs a, const int* __restrict__
 #pragma unroll
       for (int c = 0; c < 6; ++c) acc[kk][c] = 0.0;
     const double* gp = asmtab + (size_t)e * kGP * kAsmStride;
+    if (FAR32 && is_far(a.elfar, e, xp, yp, zp)) {
+      float fa[NC][6];
+#pragma unroll
+      for (int kk = 0; kk < NC; ++kk)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) fa[kk][c] = 0.0f;
+      const float* g32 = a.asmtab32 + (size_t)e * kGP * kAsmStride32;
+      const float c3f = (float)c3, c4f = (float)c4;
+#pragma unroll 2
+      for (int g = 0; g < NG; ++g) {
+        const double* gd = gp + (size_t)g * kAsmStride;
+        const float* gf = g32 + (size_t)g * kAsmStride32;
+        const float r1 = (float)(gd[0] - xp), r2 = (float)(gd[1] - yp), r3 = (float)(gd[2] - zp);
+        const float ir = __builtin_amdgcn_rsqf(r1 * r1 + r2 * r2 + r3 * r3);
+        const float rd0 = r1 * ir, rd1 = r2 * ir, rd2 = r3 * ir;
+        const float c3r = c3f * ir;
+        const float u[6] = {(c4f + rd0 * rd0) * c3r, (c4f + rd1 * rd1) * c3r, (c4f + rd2 * rd2) * c3r,
+                            (rd0 * rd1) * c3r,       (rd1 * rd2) * c3r,       (rd0 * rd2) * c3r};
+#pragma unroll
+        for (int kk = 0; kk < NC; ++kk) {
+          const float p = gf[3 + kk];
+#pragma unroll
+          for (int c = 0; c < 6; ++c) fa[kk][c] += u[c] * p;
+        }
+      }
+#pragma unroll
+      for (int kk = 0; kk < NC; ++kk)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc[kk][c] = (double)fa[kk][c];
+    } else
 #pragma unroll 2
     for (int g = 0; g < NG; ++g, gp += kAsmStride) {
       const double r1 = gp[0] - xp, r2 = gp[1] - yp, r3 = gp[2] - zp;
@@ -533,25 +657,31 @@ int geometry_run(qd_context* h) {
   const size_t slots = (size_t)h->ne * kGP;
   QD_ALLOC(h, h->geo, slots * 16);
   QD_ALLOC(h, h->asmtab, slots * kAsmStride);
+  QD_ALLOC(h, h->asmtab32, slots * kAsmStride32);
+  QD_ALLOC(h, h->elfar, 4 * (size_t)h->ne);
   const int bs = 256;
   const int nb = (int)((slots + bs - 1) / bs);
   hipLaunchKernelGGL(k_geometry, dim3(nb), dim3(bs), 0, h->stream, h->ne, h->nconec.p, h->kconec0.p,
-                     h->x.p, h->y.p, h->z.p, h->geo.p, h->asmtab.p);
+                     h->x.p, h->y.p, h->z.p, h->geo.p, h->asmtab.p, h->asmtab32.p);
+  hipLaunchKernelGGL(k_elem_far, dim3((h->ne + 255) / 256), dim3(256), 0, h->stream, h->ne, h->nconec.p,
+                     h->kconec0.p, h->x.p, h->y.p, h->z.p, h->elfar.p);
   QD_HIP(h, hipGetLastError());
   return QD_OK;
 }
 
-int assembly_run(qd_context* h, const double* /*unused*/, double* fi_out_dev, bool write_A) {
+int assembly_run(qd_context* h, double* fi_out_dev, bool write_A, bool far32, bool into_pre) {
   const int n = h->n;
   const long long N = 3LL * n;
   const int nchunk = (int)h->h_chunk_start.size() - 1;
   const int ngchunk = (int)h->h_gchunks.size();
+  DevBuf<double>& Abuf = into_pre ? h->A_pre : h->A;
+  DevBuf<double>& FIbuf = into_pre ? h->FI_pre : h->FI;
   if (write_A) {
     long long lda = (N + 63) / 64 * 64;  // 512-byte aligned columns
     h->lda = lda;
-    QD_ALLOC(h, h->A, (size_t)lda * (size_t)N);
-    QD_ALLOC(h, h->FI, (size_t)N);
-    QD_HIP(h, hipMemsetAsync(h->A.p, 0, sizeof(double) * (size_t)lda * (size_t)N, h->stream));
+    QD_ALLOC(h, Abuf, (size_t)lda * (size_t)N);
+    QD_ALLOC(h, FIbuf, (size_t)N);
+    QD_HIP(h, hipMemsetAsync(Abuf.p, 0, sizeof(double) * (size_t)lda * (size_t)N, h->stream));
   }
   QD_ALLOC(h, h->dpart, (size_t)nchunk * n * 9);
   QD_ALLOC(h, h->fpart_h, (size_t)nchunk * n * 3);
@@ -570,8 +700,10 @@ int assembly_run(qd_context* h, const double* /*unused*/, double* fi_out_dev, bo
   a.ebc = h->ebc.p;
   a.evp = h->evp.p;
   a.chunk_start = h->chunk_start.p;
-  a.A = h->A.p;
+  a.A = Abuf.p;
   a.dpart = h->dpart.p;
+  a.asmtab32 = h->asmtab32.p;
+  a.elfar = h->elfar.p;
   a.mu = h->mat.mu;
   a.c3 = h->mat.c3;
   a.c4 = h->mat.c4;
@@ -605,20 +737,16 @@ int assembly_run(qd_context* h, const double* /*unused*/, double* fi_out_dev, bo
         ProfScope ps(h, pass == 0 ? QD_PROF_ASSEMBLY_H : QD_PROF_ASSEMBLY_G, gpe * fpg);
         if (pass == 0) {
           a.fpart = h->fpart_h.p;
-          if (kind == 9)
-            hipLaunchKernelGGL(k_hpass<9>, grid, dim3(64), 0, h->stream, a, h->colour_chunks.p, i, h->asmtab.p,
-                               h->einfo.p, h->ebc.p, h->evp.p, h->A.p);
-          else
-            hipLaunchKernelGGL(k_hpass<6>, grid, dim3(64), 0, h->stream, a, h->colour_chunks.p, i, h->asmtab.p,
-                               h->einfo.p, h->ebc.p, h->evp.p, h->A.p);
+          auto kh = kind == 9 ? (far32 ? k_hpass<9, true> : k_hpass<9, false>)
+                              : (far32 ? k_hpass<6, true> : k_hpass<6, false>);
+          hipLaunchKernelGGL(kh, grid, dim3(64), 0, h->stream, a, h->colour_chunks.p, i, h->asmtab.p, h->einfo.p,
+                             h->ebc.p, h->evp.p, Abuf.p);
         } else {
           a.fpart = h->fpart_g.p;
-          if (kind == 9)
-            hipLaunchKernelGGL(k_gpass<9>, grid, dim3(64), 0, h->stream, a, h->gchunks.p, i, h->asmtab.p,
-                               h->einfo.p, h->ebc.p, h->A.p);
-          else
-            hipLaunchKernelGGL(k_gpass<6>, grid, dim3(64), 0, h->stream, a, h->gchunks.p, i, h->asmtab.p,
-                               h->einfo.p, h->ebc.p, h->A.p);
+          auto kg = kind == 9 ? (far32 ? k_gpass<9, true> : k_gpass<9, false>)
+                              : (far32 ? k_gpass<6, true> : k_gpass<6, false>);
+          hipLaunchKernelGGL(kg, grid, dim3(64), 0, h->stream, a, h->gchunks.p, i, h->asmtab.p, h->einfo.p,
+                             h->ebc.p, Abuf.p);
         }
         QD_HIP(h, hipGetLastError());
         i = j;
@@ -646,8 +774,8 @@ int assembly_run(qd_context* h, const double* /*unused*/, double* fi_out_dev, bo
   f.node_k0 = h->node_k0.p;
   f.node_vp = h->node_vp.p;
   f.mu = h->mat.mu;
-  f.A = h->A.p;
-  f.FI = fi_out_dev ? fi_out_dev : h->FI.p;
+  f.A = Abuf.p;
+  f.FI = fi_out_dev ? fi_out_dev : FIbuf.p;
   f.write_A = write_A ? 1 : 0;
   hipLaunchKernelGGL(k_finalize, dim3((n + 127) / 128), dim3(128), 0, h->stream, f);
   QD_HIP(h, hipGetLastError());

@@ -472,9 +472,28 @@ int qd_ghmat8(qd_handle h) {
     h->have_geom = true;
   }
   lu_invalidate(h);
-  int rc = assembly_run(h, nullptr, nullptr, true);
+  int rc = assembly_run(h, nullptr, true, h->asm_mode == QD_ASSEMBLY_FAR_FP32);
   if (rc) return rc;
+  h->have_pre = false;
+  if (h->asm_mode == QD_ASSEMBLY_FAR_FP32_FACTOR) {
+    // the matrix the mixed-precision factorisations factor: far field in FP32
+    if ((rc = assembly_run(h, nullptr, true, true, true))) return rc;
+    h->have_pre = true;
+  } else {
+    h->A_pre.release();
+    h->FI_pre.release();
+  }
   h->have_sys = true;
+  return QD_OK;
+}
+
+int qd_set_assembly_precision(qd_handle h, int mode) {
+  if (!h) return QD_ERR_ARG;
+  if (mode != QD_ASSEMBLY_FP64 && mode != QD_ASSEMBLY_FAR_FP32 && mode != QD_ASSEMBLY_FAR_FP32_FACTOR)
+    return set_error(h, QD_ERR_ARG, "qd_set_assembly_precision: mode %d", mode);
+  h->asm_mode = mode;
+  h->have_sys = false;
+  lu_invalidate(h);
   return QD_OK;
 }
 
@@ -523,7 +542,7 @@ int qd_ghmat8_rhs(qd_handle h, const double* bc, double* fi) {
   rc = upload_i(h, dg, h->h_gchunks);
   if (!rc) {
     std::swap(h->gchunks.p, dg.p);
-    rc = assembly_run(h, nullptr, dfi.p, false);
+    rc = assembly_run(h, dfi.p, false, h->asm_mode == QD_ASSEMBLY_FAR_FP32);
     std::swap(h->gchunks.p, dg.p);
   }
   std::swap(h->ebc.p, dbc.p);

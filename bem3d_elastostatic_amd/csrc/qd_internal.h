@@ -94,6 +94,7 @@ struct SubTri {
 
 constexpr int kGP = 36;         // Gauss points per element slot (6x6; tri6 uses 16)
 constexpr int kAsmStride = 16;  // doubles per Gauss point in the assembly table
+constexpr int kAsmStride32 = 16;  // floats per Gauss point in the FP32 far-field table
 constexpr int kNearGL = 40;     // near-field rule of the internal points (stress_mod.f90:513)
 
 struct Material {
@@ -127,6 +128,8 @@ struct qd_context {
   qd::DevBuf<int> nconec, kconec0;  // kconec0: 0-based, (ne, 9)
   qd::DevBuf<double> geo;           // reference tables: 8 blocks of (36, ne) + ff (9, 36, ne)
   qd::DevBuf<double> asmtab;        // (ne*36, 16): xg(3) eta(3) p12(9) pad
+  qd::DevBuf<float> asmtab32;       // (ne*36, 16): eta(3) p12(9) pad, FP32
+  qd::DevBuf<double> elfar;         // (ne, 4): centroid, squared far-field distance
   qd::DevBuf<qd::ElemInfo> einfo;
   qd::DevBuf<double> ebc, evp;      // (ne, 27): bc and vp per element dof
   qd::DevBuf<int> node_k0;          // (n, 3) kode0 flag per node/direction
@@ -151,6 +154,11 @@ struct qd_context {
   // assembled system (column-major, leading dimension lda)
   qd::DevBuf<double> A, FI;
   long long lda = 0;
+  // QD_ASSEMBLY_*: far-field pairs in FP32 for A itself (1) or only for the
+  // matrix the mixed-precision factorisations factor (2, in A_pre)
+  int asm_mode = QD_ASSEMBLY_FP64;
+  qd::DevBuf<double> A_pre, FI_pre;
+  bool have_pre = false;
 
   // solver state (qd_lu.hip)
   qd_lu_state* lu = nullptr;
@@ -221,7 +229,8 @@ int set_error(qd_context* h, int code, const char* fmt, ...);
 
 // phases implemented in the .hip files
 int geometry_run(qd_context* h);
-int assembly_run(qd_context* h, const double* bc_dev_override, double* fi_out_dev, bool write_A);
+// A (or, with into_pre, A_pre) and FI (or fi_out_dev); far32: far-field pairs in FP32.
+int assembly_run(qd_context* h, double* fi_out_dev, bool write_A, bool far32 = false, bool into_pre = false);
 int somigliana_run(qd_context* h, int l, const double* pts_dev, const double* fi_dev,
                    const double* dfi_dev, int variant, double* u_dev, double* sig_dev,
                    double* prin_dev, double* vm_dev, long long* counts);

@@ -2675,7 +2675,9 @@ static int ensure_factors(qd_context* h, int precision, qd_solve_report* rep) {
       // factor_steps converts A into W32, beside the first panel
       LuCtx<float> c{h, s, s->W32.p, ld, N, precision == QD_FP16_IR, true, h->stream, &s->sc_main};
       c.diag = diag;
-      c.src = h->A.p;
+      // QD_ASSEMBLY_FAR_FP32_FACTOR: the factors come from the FP32 far-field
+      // matrix; the refinement's residual keeps the FP64 A
+      c.src = h->have_pre ? h->A_pre.p : h->A.p;
       c.src_ld = h->lda;
       rc = factor(c, pw);
       if (rc || !diag) break;
@@ -2689,7 +2691,9 @@ static int ensure_factors(qd_context* h, int precision, qd_solve_report* rep) {
       diag = false;
     }
     s->diag_pivots = diag;
-    if (!rc) s->rowabs_nsl = (int)(((N + 31) / 32 + kAbsTiles - 1) / kAbsTiles);
+    // the conversion's row sums are of the matrix it converted: ||A||_inf from
+    // them only when that is A itself
+    if (!rc && !h->have_pre) s->rowabs_nsl = (int)(((N + 31) / 32 + kAbsTiles - 1) / kAbsTiles);
   }
   if (rc) return rc;
   {

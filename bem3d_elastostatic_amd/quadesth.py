@@ -108,6 +108,13 @@ class Quadesth:
         check(self.h, self.lib.qd_precompute_element_geometry(self.h, *[_ptr(out[k]) for k in order]))
         return {k: (v.reshape(ne, 36, 9) if k == "ff" else v.reshape(ne, 36)) for k, v in out.items()}
 
+    def set_assembly_precision(self, mode: str = "fp64"):
+        """Arithmetic of the regular element integrals for the next ghmat8:
+        "fp64" (the reference's), "far_fp32" (pairs beyond two element
+        diameters in FP32, A itself) or "far_fp32_factor" (A in FP64; only the
+        mixed-precision factorisations factor the FP32 far-field matrix)."""
+        check(self.h, self.lib.qd_set_assembly_precision(self.h, _lib.ASSEMBLY[mode]), "set_assembly_precision")
+
     def ghmat8(self):
         """assembly_mod.f90:53-223 into device memory."""
         check(self.h, self.lib.qd_ghmat8(self.h), "ghmat8")

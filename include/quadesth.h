@@ -49,6 +49,25 @@ enum {
                        per-block scaling, FP64 refinement (GMRES fallback) */
 };
 
+/* Arithmetic of the regular element integrals of ghmat8 (extin8,
+ * assembly_mod.f90:339-384).  A pair is far when the collocation node lies
+ * more than two element diameters (twice the largest node-to-centroid
+ * distance) from the element's centroid; far pairs may run in FP32 (the
+ * Kelvin kernels and the Gauss sums; r itself in FP64).  Near and adjacent
+ * pairs, the singular elements (locin8), the rigid-body diagonal and every
+ * sum into A and FI stay FP64. */
+enum {
+  QD_ASSEMBLY_FP64 = 0,            /* every integral FP64 (the reference's) */
+  QD_ASSEMBLY_FAR_FP32 = 1,        /* A itself with FP32 far pairs: entries
+                                      off by ~1e-7 relative, so every solve
+                                      solves that A */
+  QD_ASSEMBLY_FAR_FP32_FACTOR = 2  /* A in FP64; the FP32/FP16 factorisations
+                                      factor a second matrix assembled with
+                                      FP32 far pairs, while the refinement's
+                                      residual keeps the FP64 A (solutions as
+                                      with QD_ASSEMBLY_FP64) */
+};
+
 /* Somigliana displacement identity variant (stress_mod.f90:838-843). */
 enum {
   QD_SOMIGLIANA_REFERENCE = 0,  /* the reference's sign of the T* term */
@@ -102,6 +121,9 @@ int qd_precompute_element_geometry(qd_handle h, double* eta1, double* eta2, doub
 /* ghmat8 (assembly_mod.f90:53-223): assembles A (3n x 3n) and FI (3n) into
  * device memory owned by the handle. */
 int qd_ghmat8(qd_handle h);
+/* QD_ASSEMBLY_* for the next qd_ghmat8 (no reference counterpart: the
+ * reference integrates in FP64 only). */
+int qd_set_assembly_precision(qd_handle h, int mode);
 /* FI only, for another set of prescribed values with the same kode
  * (assembly_mod.f90:126-218 restricted to FI); A is untouched. */
 int qd_ghmat8_rhs(qd_handle h, const double* bc, double* fi /* host, 3n */);
