@@ -446,7 +446,8 @@ int somigliana_run(qd_context* h, int l, const double* pts, const double* fi, co
   k.sg = variant ? -1.0 : 1.0;
   double* part = h->somi_part.p;
   double* nearp = part + (size_t)nslice * l * 9;
-  // ~250 flops per Gauss-point evaluation of accumulate_somigliana_range
+  // ~250 flops per Gauss-point evaluation of accumulate_somigliana_range; the
+  // work of both passes is set below from the evaluation counters
   {
     ProfScope ps(h, QD_PROF_SOMIGLIANA_FAR, 250.0 * (double)tot * l);
     hipLaunchKernelGGL(k_somi_far, dim3((l + 63) / 64, nslice), dim3(64), 0, h->stream, l, ne, slice, pts,
@@ -460,14 +461,30 @@ int somigliana_run(qd_context* h, int l, const double* pts, const double* fi, co
   hipLaunchKernelGGL(k_somi_final, dim3((l + 127) / 128), dim3(128), 0, h->stream, l, nslice, part, nearp,
                      u, sig, prin, vm);
   QD_HIP(h, hipGetLastError());
-  if (counts) {
+  const bool prof = ((h->prof_mask >> QD_PROF_SOMIGLIANA_FAR) & 1) || ((h->prof_mask >> QD_PROF_SOMIGLIANA_NEAR) & 1);
+  if (counts || prof) {
     long long c[2];
     QD_HIP(h, hipMemcpyAsync(c, h->counters.p, sizeof(c), hipMemcpyDeviceToHost, h->stream));
     QD_HIP(h, hipStreamSynchronize(h->stream));
     // far evaluations = all far points of all elements for all points, less
     // those of near elements
-    counts[0] = (long long)tot * l - c[0];
-    counts[1] = c[1];
+    const long long nfar = (long long)tot * l - c[0], nnear = c[1];
+    if (counts) {
+      counts[0] = nfar;
+      counts[1] = nnear;
+    }
+    // the profile records of this call: the evaluations each pass made
+    // (known only now), at ~250 flops each
+    bool far_done = false, near_done = false;
+    for (auto it = h->prof_pending.rbegin(); it != h->prof_pending.rend() && !(far_done && near_done); ++it) {
+      if (it->kind == QD_PROF_SOMIGLIANA_FAR && !far_done) {
+        it->work = 250.0 * (double)nfar;
+        far_done = true;
+      } else if (it->kind == QD_PROF_SOMIGLIANA_NEAR && !near_done) {
+        it->work = 250.0 * (double)nnear;
+        near_done = true;
+      }
+    }
   }
   return QD_OK;
 }

@@ -114,6 +114,38 @@ def _hip():
     return _HIP
 
 
+def host_description():
+    """CPU model, logical CPUs this process may run on, physical cores among
+    them, and the BLAS/LAPACK numpy calls."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    cpus = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(range(os.cpu_count() or 1))
+    cores = set()
+    for c in cpus:
+        try:
+            pkg = open(f"/sys/devices/system/cpu/cpu{c}/topology/physical_package_id").read().strip()
+            core = open(f"/sys/devices/system/cpu/cpu{c}/topology/core_id").read().strip()
+            cores.add((pkg, core))
+        except OSError:
+            cores.add(("?", str(c)))
+    blas = "unknown"
+    try:
+        from threadpoolctl import threadpool_info
+        info = [d for d in threadpool_info() if d.get("user_api") == "blas"]
+        if info:
+            blas = "%s %s (%s threads)" % (info[0].get("internal_api"), info[0].get("version"),
+                                            info[0].get("num_threads"))
+    except Exception:
+        pass
+    return dict(cpu_model=model, logical_cpus=len(cpus), physical_cores=len(cores), blas=blas)
+
+
 def cpu_baseline(p, FI, DFI, budget_s=25.0):
     """The oracle (a C/OpenMP restatement of the reference's loops) and numpy
     LAPACK timed on bounded samples of the same workload, extrapolated to one
@@ -121,6 +153,7 @@ def cpu_baseline(p, FI, DFI, budget_s=25.0):
     from oracle.pyoracle import Oracle
     o = Oracle()
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    host = host_description()
     t = o.geometry(p)
     rng = np.random.default_rng(0)
     # assembly: sampled collocation nodes, all elements
@@ -153,7 +186,7 @@ def cpu_baseline(p, FI, DFI, budget_s=25.0):
     fixed = max(t1 - slope * s1, 0.0)
     t_som = fixed + slope * p.l
     total = t_asm + t_lu + t_som
-    return dict(value=round(total, 3), unit="s/solve", cores=threads, kind="port",
+    return dict(value=round(total, 3), unit="s/solve", cores=threads, kind="port", host=host,
                 sample=(f"oracle ghmat8 rows of {k} of {p.n} collocation nodes ({ta:.2f} s), numpy LAPACK "
                         f"gesv N={ns} ({tl:.2f} s) scaled by (N/{ns})^3, oracle Somigliana at {s1} and {s2} of "
                         f"{p.l} points ({t1:.2f}+{t2:.2f} s); extrapolated phases s: assembly {t_asm:.1f}, "
@@ -376,6 +409,110 @@ def bench_gradient(args):
     s.close()
 
 
+def _c3_handle(args):
+    import torch
+    from bem3d_elastostatic_amd import mesh
+    from bem3d_elastostatic_amd.quadesth import Quadesth
+    p = mesh.config3() if args.config == "c3" else mesh.config1()
+    q = Quadesth(p)
+    if args.assembly != "fp64":
+        q.set_assembly_precision(args.assembly)
+    q.precompute_element_geometry()
+    torch.cuda.synchronize()
+    return p, q
+
+
+def bench_assembly_only(args):
+    """ghmat8 alone, --steps times after --warmup (no solve): the launches a
+    counter pass of the assembly kernels needs."""
+    p, q = _c3_handle(args)
+    for _ in range(args.warmup):
+        q.ghmat8()
+        q.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        q.ghmat8()
+        q.synchronize()
+        log(f"assembly {s} done")
+    t = (time.perf_counter() - t0) / args.steps
+    print(json.dumps({"mode": "assembly-only", "config": args.config, "assembly": args.assembly,
+                      "s_per_assembly": round(t, 5), "hg_entries_per_s": hg_entries(p) / t}), flush=True)
+
+
+def bench_internal_only(args):
+    """compute_internal_points alone at the configuration's points with a
+    seeded smooth synthetic boundary field (the work does not depend on the
+    values), --steps times after --warmup."""
+    import torch
+    from bem3d_elastostatic_amd import _lib
+    p, q = _c3_handle(args)
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(4)
+    fi = torch.tensor(1e-3 * np.sin(np.outer(np.arange(3 * p.n) % 97, [1.0]).ravel() + rng.normal(size=3 * p.n)),
+                      dtype=torch.float64, device=dev)
+    dfi = torch.tensor(rng.normal(size=27 * p.ne), dtype=torch.float64, device=dev)
+    pts = torch.tensor(np.ascontiguousarray(p.points), dtype=torch.float64, device=dev)
+    out = torch.empty(13 * p.l, dtype=torch.float64, device=dev)
+    base = out.data_ptr()
+    counts = np.zeros(2, dtype=np.int64)
+
+    def once():
+        rc = q.lib.qd_internal_points_device(q.h, p.l, C.c_void_p(pts.data_ptr()), C.c_void_p(fi.data_ptr()),
+                                             C.c_void_p(dfi.data_ptr()), 0, C.c_void_p(base),
+                                             C.c_void_p(base + 8 * 3 * p.l), C.c_void_p(base + 8 * 9 * p.l),
+                                             C.c_void_p(base + 8 * 12 * p.l),
+                                             counts.ctypes.data_as(C.POINTER(C.c_longlong)))
+        _lib.check(q.h, rc, "internal points")
+        q.synchronize()
+    for _ in range(args.warmup):
+        once()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        once()
+        log(f"internal points {s} done")
+    t = (time.perf_counter() - t0) / args.steps
+    print(json.dumps({"mode": "internal-only", "config": args.config, "s_per_call": round(t, 5),
+                      "gp_evals": {"far": int(counts[0]), "near": int(counts[1])}}), flush=True)
+
+
+def bench_cpu_full(args):
+    """The oracle's assembly (every collocation row, in chunks) and Somigliana
+    (every point, in chunks) of the 76,806-DOF cantilever on the host cores,
+    timed in full; the LU at N=8000 with numpy's LAPACK, scaled by N^3."""
+    from bem3d_elastostatic_amd import mesh
+    from oracle.pyoracle import Oracle
+    p = mesh.config3()
+    o = Oracle()
+    host = host_description()
+    log(f"host {host}")
+    t = o.geometry(p)
+    t0 = time.perf_counter()
+    for c0 in range(0, p.n, 2048):
+        o.ghmat8_rows(p, np.arange(c0, min(p.n, c0 + 2048)), t)
+        log(f"assembly rows of nodes {c0}..{min(p.n, c0 + 2048)} done, {time.perf_counter() - t0:.1f} s")
+    t_asm = time.perf_counter() - t0
+    rng = np.random.default_rng(4)
+    FI = 1e-3 * rng.normal(size=3 * p.n)
+    DFI = rng.normal(size=(p.ne, 27))
+    t0 = time.perf_counter()
+    for c0 in range(0, p.l, 8192):
+        o.internal_points(p, t, FI, DFI, points=p.points[c0:c0 + 8192])
+        log(f"internal points {c0}..{c0 + 8192} done, {time.perf_counter() - t0:.1f} s")
+    t_som = time.perf_counter() - t0
+    ns = 8000
+    A = rng.normal(size=(ns, ns))
+    t0 = time.perf_counter()
+    np.linalg.solve(A, rng.normal(size=ns))
+    tl = time.perf_counter() - t0
+    N = 3 * p.n
+    print(json.dumps({"mode": "cpu-full", "workload": "76806-DOF cantilever, 65536 points", "host": host,
+                      "omp_threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                      "assembly_s": round(t_asm, 2), "internal_points_s": round(t_som, 2),
+                      "lu_s_extrapolated": round(tl * (N / ns) ** 3, 1), "lu_sample": f"numpy gesv N={ns}: {tl:.2f} s",
+                      "note": "assembly and internal points timed in full (the near table is built per chunk of "
+                              "8192 points); the LU extrapolated by N^3"}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -393,7 +530,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (reference for the HBM byte counters)")
+    ap.add_argument("--assembly-only", action="store_true", help="only ghmat8, --steps times (no solve)")
+    ap.add_argument("--internal-only", action="store_true",
+                    help="only the internal points, --steps times, on a synthetic boundary field (no solve)")
+    ap.add_argument("--assembly", default="fp64", choices=["fp64", "far_fp32", "far_fp32_factor"],
+                    help="assembly precision of the --assembly-only / --internal-only modes")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="only the CPU baseline of the cantilever with assembly and internal points in full")
     args = ap.parse_args()
+    if args.cpu_full:
+        bench_cpu_full(args)
+        return
     if args.calibrate_copy:
         import torch
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")   # 1 GiB
@@ -418,6 +565,12 @@ def main():
         return
     if args.trailing_update_only:
         bench_trailing_update()
+        return
+    if args.assembly_only:
+        bench_assembly_only(args)
+        return
+    if args.internal_only:
+        bench_internal_only(args)
         return
     if args.config == "c3":
         p = mesh.config3()
@@ -503,6 +656,7 @@ def main():
         mask = np.arange(27)[None, :] < 3 * p.nconec[:, None]
         extra["fp64_path_s_per_solve"] = t64
         extra["fp64_factor_s"] = ph64[-1]["factor_ms"] / 1000.0
+        extra["fp64_backward_error"] = ph64[-1]["residual"]
         extra["fp64_lu_mfma_pct"] = 100.0 * lu_flops / (ph64[-1]["factor_ms"] / 1000.0) / (PEAK_TFLOPS["fp64"] * 1e12)
         extra["speedup_vs_fp64"] = t64 / (ms_per_step / 1000.0)
         extra["errors_vs_fp64"] = dict(u_boundary=rel(FI, FI64), t_boundary=rel(DFI[mask], DFI64[mask]),
