@@ -17,7 +17,7 @@ CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "libquadesth.so")
 SOURCES = ["qd_capi.cpp", "qd_assembly.hip", "qd_somigliana.hip", "qd_lu.hip", "qd_lapack.cpp", "qd_grad.hip",
-           "qd_output.cpp"]
+           "qd_output.cpp", "qd_plan.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("QUADESTH_ARCH", "gfx950")
 

@@ -164,6 +164,7 @@ struct AsmArgs {
   int write_A;
   const float* asmtab32;  // far-field passes: FP32 normals and weight products
   const double* elfar;    // far-field passes: k_elem_far's table
+  const unsigned* elop;   // write plan, kElemOps words per element
 };
 
 // H pass: h_elem of extin8 (assembly_mod.f90:339-384) with funkelvin's
@@ -177,6 +178,7 @@ __global__ __launch_bounds__(64) void k_hpass(AsmArgs a, const int* __restrict__
                                                const double* __restrict__ ebc_all, const double* __restrict__ evp_all,
                                                double* __restrict__ Aout) {
   constexpr int NG = (NC == 9) ? 36 : 16;
+  __shared__ double carry[kCarrySlots][9][64];  // column blocks carried between a chunk's elements
   const int chunk = chunks[coff + blockIdx.y];
   const int e0 = a.chunk_start[chunk], e1 = a.chunk_start[chunk + 1];
   const int in = blockIdx.x * 64 + threadIdx.x;
@@ -284,33 +286,12 @@ __global__ __launch_bounds__(64) void k_hpass(AsmArgs a, const int* __restrict__
       for (int q = 0; q < 6 + NC; ++q) cur[q] = nxt[q];
     }
     }
+    const int kb = inf->kode, k0 = inf->k0mask;
     if (valid && !sing) {
-      const int kb = inf->kode, k0 = inf->k0mask;
       const double* ebc = ebc_all + (size_t)e * 27;
       const double* evp = evp_all + (size_t)e * 27;
-      // a node's A columns: its nine loads issued one node ahead, in flight
-      // during the previous node's updates (the branches below are uniform; a
-      // column that is not updated is read and left alone)
-      auto load_cols = [&](int k, double (&o)[3][3]) {
-        const int ik = inf->node[k];
 #pragma unroll
-        for (int ld = 0; ld < 3; ++ld) {
-          const double* col = Aout + (long long)(3 * ik + ld) * a.lda + 3LL * in;
-#pragma unroll
-          for (int q = 0; q < 3; ++q) o[ld][q] = col[q];
-        }
-      };
-      double nxt[3][3] = {};
-      if (a.write_A) load_cols(0, nxt);
-#pragma unroll
-      for (int k = 0; k < NC; ++k) {
-        const int ik = inf->node[k];
-        double old[3][3];
-#pragma unroll
-        for (int ld = 0; ld < 3; ++ld)
-#pragma unroll
-          for (int q = 0; q < 3; ++q) old[ld][q] = nxt[ld][q];
-        if (a.write_A && k + 1 < NC) load_cols(k + 1, nxt);
+      for (int k = 0; k < NC; ++k)
 #pragma unroll
         for (int ld = 0; ld < 3; ++ld) {
           const int bit = 3 * k + ld;
@@ -320,14 +301,7 @@ __global__ __launch_bounds__(64) void k_hpass(AsmArgs a, const int* __restrict__
             fi0 -= h0 * b;
             fi1 -= h1 * b;
             fi2 -= h2 * b;
-          } else if (((k0 >> bit) & 1) == 0) {
-            if (a.write_A) {
-              double* col = Aout + (long long)(3 * ik + ld) * a.lda + 3LL * in;
-              col[0] = old[ld][0] + h0;
-              col[1] = old[ld][1] + h1;
-              col[2] = old[ld][2] + h2;
-            }
-          } else {
+          } else if ((k0 >> bit) & 1) {
             const double v = evp[bit];
             fi0 -= h0 * v;
             fi1 -= h1 * v;
@@ -336,6 +310,67 @@ __global__ __launch_bounds__(64) void k_hpass(AsmArgs a, const int* __restrict__
           d1[0 + 3 * ld] -= h0;
           d1[1 + 3 * ld] -= h1;
           d1[2 + 3 * ld] -= h2;
+        }
+    }
+    if (a.write_A && valid) {
+      // column blocks of A by the write plan: a chain's first element reads
+      // A (or starts from zero), its last stores, the ones between carry the
+      // block in LDS.  A lane whose node is in the element adds nothing but
+      // still passes its block along.  Reads of A are issued one node ahead.
+      const unsigned* eop = a.elop + (size_t)e * kElemOps;
+      auto load_cols = [&](int k, double (&o)[3][3]) {
+        const unsigned op = eop[k];
+        const int rd = (op & kOpFresh) ? (op_wcols(op) & ~op_zcols(op)) : 0;
+        const int ik = inf->node[k];
+#pragma unroll
+        for (int ld = 0; ld < 3; ++ld) {
+          const double* col = Aout + (long long)(3 * ik + ld) * a.lda + 3LL * in;
+#pragma unroll
+          for (int q = 0; q < 3; ++q) o[ld][q] = ((rd >> ld) & 1) ? col[q] : 0.0;
+        }
+      };
+      double nxt[3][3];
+      load_cols(0, nxt);
+#pragma unroll
+      for (int k = 0; k < NC; ++k) {
+        const unsigned op = eop[k];
+        const int ik = inf->node[k];
+        double v[3][3];
+#pragma unroll
+        for (int ld = 0; ld < 3; ++ld)
+#pragma unroll
+          for (int q = 0; q < 3; ++q) v[ld][q] = nxt[ld][q];
+        if (k + 1 < NC) load_cols(k + 1, nxt);
+        if (!(op & kOpFresh)) {
+          const int s = op_slot_in(op) & (kCarrySlots - 1);
+#pragma unroll
+          for (int ld = 0; ld < 3; ++ld)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) v[ld][q] = carry[s][3 * ld + q][threadIdx.x];
+        }
+        if (!sing) {
+#pragma unroll
+          for (int ld = 0; ld < 3; ++ld)
+            if (((kb >> ld) & 1) && !((k0 >> (3 * k + ld)) & 1)) {
+#pragma unroll
+              for (int q = 0; q < 3; ++q) v[ld][q] += acc[k][ld][q];
+            }
+        }
+        if (op & kOpLast) {
+          const int wc = op_wcols(op);
+#pragma unroll
+          for (int ld = 0; ld < 3; ++ld)
+            if ((wc >> ld) & 1) {
+              double* col = Aout + (long long)(3 * ik + ld) * a.lda + 3LL * in;
+#pragma unroll
+              for (int q = 0; q < 3; ++q) col[q] = v[ld][q];
+            }
+        } else {
+          const int s = op_slot_out(op) & (kCarrySlots - 1);
+#pragma unroll
+          for (int ld = 0; ld < 3; ++ld)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) carry[s][3 * ld + q][threadIdx.x] = v[ld][q];
         }
       }
     }
@@ -425,18 +460,22 @@ __global__ __launch_bounds__(64) void k_gpass(AsmArgs a, const int* __restrict__
         for (int c = 0; c < 6; ++c) acc[kk][c] += u[c] * p;
       }
     }
-    if (valid && !sing) {
+    if (valid) {
       const int kb = inf->kode;
       const double* ebc = ebc_all + (size_t)e * 27;
+      // first writes of a column (write plan bit 3k+ld) start from zero; a
+      // lane whose node is in the element adds nothing, and stores only
+      // those first writes
+      const unsigned gz = a.write_A ? a.elop[(size_t)e * kElemOps + 9] : 0u;
 #pragma unroll
       for (int k = 0; k < NC; ++k) {
         const int ik = inf->node[k];
-        // the node's A columns: all nine loads in flight before the updates
-        // (the branches below are uniform; a column not updated is left alone)
+        // the node's A columns: all loads in flight before the updates
         double old[3][3] = {};
-        if (a.write_A && (kb & 7) != 7) {
+        if (a.write_A && !sing && (kb & 7) != 7) {
 #pragma unroll
           for (int ld = 0; ld < 3; ++ld) {
+            if (((kb >> ld) & 1) || ((gz >> (3 * k + ld)) & 1)) continue;
             const double* col = Aout + (long long)(3 * ik + ld) * a.lda + 3LL * in;
 #pragma unroll
             for (int q = 0; q < 3; ++q) old[ld][q] = col[q];
@@ -452,11 +491,17 @@ __global__ __launch_bounds__(64) void k_gpass(AsmArgs a, const int* __restrict__
           if (((kb >> ld) & 1) == 0) {
             if (a.write_A) {
               double* col = Aout + (long long)(3 * ik + ld) * a.lda + 3LL * in;
-              col[0] = old[ld][0] - g0 * mu;
-              col[1] = old[ld][1] - g1 * mu;
-              col[2] = old[ld][2] - g2 * mu;
+              if (!sing) {
+                col[0] = old[ld][0] - g0 * mu;
+                col[1] = old[ld][1] - g1 * mu;
+                col[2] = old[ld][2] - g2 * mu;
+              } else if ((gz >> (3 * k + ld)) & 1) {
+                col[0] = 0.0;
+                col[1] = 0.0;
+                col[2] = 0.0;
+              }
             }
-          } else {
+          } else if (!sing) {
             const double b = ebc[3 * k + ld];
             fi0 += g0 * b;
             fi1 += g1 * b;
@@ -653,6 +698,20 @@ __global__ void k_finalize(FinArgs a) {
   for (int jj = 0; jj < 3; ++jj) a.FI[3LL * in + jj] = F[jj];
 }
 
+// The entries of A that no assembly pass writes: rows [N, lda) of every
+// column (k_zero_tail), and the columns of the write plan's unwritten list
+// (k_zero_cols, one workgroup per column).
+__global__ void k_zero_tail(long long N, long long lda, double* __restrict__ A) {
+  const long long pad = lda - N;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= pad * N) return;
+  A[(t / pad) * lda + N + t % pad] = 0.0;
+}
+__global__ void k_zero_cols(const int* __restrict__ cols, long long lda, double* __restrict__ A) {
+  double* col = A + (long long)cols[blockIdx.x] * lda;
+  for (long long r = threadIdx.x; r < lda; r += blockDim.x) col[r] = 0.0;
+}
+
 int geometry_run(qd_context* h) {
   const size_t slots = (size_t)h->ne * kGP;
   QD_ALLOC(h, h->geo, slots * 16);
@@ -681,7 +740,15 @@ int assembly_run(qd_context* h, double* fi_out_dev, bool write_A, bool far32, bo
     h->lda = lda;
     QD_ALLOC(h, Abuf, (size_t)lda * (size_t)N);
     QD_ALLOC(h, FIbuf, (size_t)N);
-    QD_HIP(h, hipMemsetAsync(Abuf.p, 0, sizeof(double) * (size_t)lda * (size_t)N, h->stream));
+    // every other entry is written by the passes below (the write plan)
+    const long long tail = (lda - N) * N;
+    if (tail > 0)
+      hipLaunchKernelGGL(k_zero_tail, dim3((unsigned)((tail + 255) / 256)), dim3(256), 0, h->stream, N, lda,
+                         Abuf.p);
+    if (!h->h_unwritten.empty())
+      hipLaunchKernelGGL(k_zero_cols, dim3((unsigned)h->h_unwritten.size()), dim3(256), 0, h->stream,
+                         h->unwritten.p, lda, Abuf.p);
+    QD_HIP(h, hipGetLastError());
   }
   QD_ALLOC(h, h->dpart, (size_t)nchunk * n * 9);
   QD_ALLOC(h, h->fpart_h, (size_t)nchunk * n * 3);
@@ -704,6 +771,7 @@ int assembly_run(qd_context* h, double* fi_out_dev, bool write_A, bool far32, bo
   a.dpart = h->dpart.p;
   a.asmtab32 = h->asmtab32.p;
   a.elfar = h->elfar.p;
+  a.elop = h->elop.p;
   a.mu = h->mat.mu;
   a.c3 = h->mat.c3;
   a.c4 = h->mat.c4;

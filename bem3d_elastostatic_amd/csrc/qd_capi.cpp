@@ -99,58 +99,6 @@ static int nsub_of(int nconec, int nodo) {
   return nodo <= 3 ? 1 : 2;
 }
 
-// Greedy colouring of element chunks so that no two chunks of one colour
-// share a node: chunks of one colour can then update the same rows of A
-// without write conflicts.
-static void build_schedule(qd_context* h) {
-  const int ne = h->ne, n = h->n;
-  int P = std::max(8, std::min(64, ne / 256));
-  h->chunk_elems = P;
-  h->h_chunk_start.clear();
-  int e = 0;
-  while (e < ne) {
-    h->h_chunk_start.push_back(e);
-    int first = e;
-    int cnt = 0;
-    while (e < ne && cnt < P && h->hnconec[e] == h->hnconec[first]) {
-      ++e;
-      ++cnt;
-    }
-  }
-  const int nchunk = (int)h->h_chunk_start.size();
-  h->h_chunk_start.push_back(ne);
-  std::vector<std::vector<int>> node_chunks(n);
-  for (int c = 0; c < nchunk; ++c)
-    for (int el = h->h_chunk_start[c]; el < h->h_chunk_start[c + 1]; ++el)
-      for (int k = 0; k < h->hnconec[el]; ++k) {
-        int nd = h->hkconec[el * 9 + k] - 1;
-        auto& v = node_chunks[nd];
-        if (v.empty() || v.back() != c) v.push_back(c);
-      }
-  std::vector<int> colour(nchunk, -1);
-  int ncol = 0;
-  std::vector<char> used;
-  for (int c = 0; c < nchunk; ++c) {
-    used.assign(ncol + 1, 0);
-    for (int el = h->h_chunk_start[c]; el < h->h_chunk_start[c + 1]; ++el)
-      for (int k = 0; k < h->hnconec[el]; ++k)
-        for (int oc : node_chunks[h->hkconec[el * 9 + k] - 1])
-          if (colour[oc] >= 0) used[colour[oc]] = 1;
-    int col = 0;
-    while (col < ncol && used[col]) ++col;
-    colour[c] = col;
-    if (col == ncol) ++ncol;
-  }
-  h->h_colour_start.assign(ncol + 1, 0);
-  h->h_colour_chunks.clear();
-  for (int col = 0; col < ncol; ++col) {
-    h->h_colour_start[col] = (int)h->h_colour_chunks.size();
-    for (int c = 0; c < nchunk; ++c)
-      if (colour[c] == col) h->h_colour_chunks.push_back(c);
-  }
-  h->h_colour_start[ncol] = (int)h->h_colour_chunks.size();
-}
-
 static int upload_i(qd_context* h, DevBuf<int>& d, const std::vector<int>& v) {
   QD_ALLOC(h, d, std::max<size_t>(v.size(), 1));
   if (!v.empty()) QD_HIP(h, hipMemcpy(d.p, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -352,56 +300,6 @@ int qd_set_material(qd_handle h, double mu, double nu) {
   return QD_OK;
 }
 
-// kode0 / vp search of assembly_mod.f90:149-161 for node nd (0-based).
-static int find_vp(qd_context* h, const int* kode, const double* bc, int nd, int ld, double* vp) {
-  for (int i = 0; i < h->nesim[nd]; ++i) {
-    int e = h->ke[h->ke_ptr[nd] + i] - 1;
-    if (kode[e * 3 + ld] == 0) {
-      for (int m = 0; m < h->hnconec[e]; ++m)
-        if (h->hkconec[e * 9 + m] - 1 == nd) {
-          *vp = bc[e * 27 + 3 * m + ld];
-          return 1;
-        }
-    }
-  }
-  *vp = 0.0;
-  return 0;
-}
-
-static int bc_tables(qd_context* h, const int* kode, const double* bc, std::vector<ElemInfo>& info,
-                     std::vector<double>& evp, std::vector<int>& nk0, std::vector<double>& nvp) {
-  const int n = h->n, ne = h->ne;
-  nk0.assign(3 * (size_t)n, 0);
-  nvp.assign(3 * (size_t)n, 0.0);
-  for (int i = 0; i < n; ++i)
-    for (int ld = 0; ld < 3; ++ld) nk0[3 * i + ld] = find_vp(h, kode, bc, i, ld, &nvp[3 * i + ld]);
-  info.assign(ne, ElemInfo{});
-  evp.assign(27 * (size_t)ne, 0.0);
-  for (int e = 0; e < ne; ++e) {
-    ElemInfo& f = info[e];
-    f.nconec = h->hnconec[e];
-    f.kode = 0;
-    f.k0mask = 0;
-    f.needG = 0;
-    for (int ld = 0; ld < 3; ++ld) {
-      if (kode[e * 3 + ld] != 0 && kode[e * 3 + ld] != 1)
-        return set_error(h, QD_ERR_ARG, "kode(%d) = %d (expected 0 or 1)", 3 * e + ld + 1,
-                         kode[e * 3 + ld]);
-      if (kode[e * 3 + ld] == 1) f.kode |= 1 << ld;
-      else f.needG = 1;
-    }
-    for (int k = 0; k < 9; ++k) f.node[k] = (k < f.nconec) ? h->hkconec[e * 9 + k] - 1 : -1;
-    for (int k = 0; k < f.nconec; ++k)
-      for (int ld = 0; ld < 3; ++ld) {
-        int nd = f.node[k];
-        if (nk0[3 * nd + ld]) f.k0mask |= 1 << (3 * k + ld);
-        evp[e * 27 + 3 * k + ld] = nvp[3 * nd + ld];
-        if (bc[e * 27 + 3 * k + ld] != 0.0 && (f.kode >> ld & 1)) f.needG = 1;
-      }
-  }
-  return QD_OK;
-}
-
 int qd_set_bc(qd_handle h, const int* kode, const double* bc) {
   if (!h || !kode || !bc) return QD_ERR_ARG;
   if (!h->have_mesh) return set_error(h, QD_ERR_STATE, "qd_set_bc before qd_set_mesh");
@@ -413,19 +311,12 @@ int qd_set_bc(qd_handle h, const int* kode, const double* bc) {
   if (rc) return rc;
   h->hkode.assign(kode, kode + 3 * (size_t)h->ne);
   h->hbc.assign(bc, bc + 27 * (size_t)h->ne);
-  // G-pass schedule: chunks holding needG elements, in colour order
-  h->h_gchunks.clear();
-  h->h_gcolour_start.assign(h->h_colour_start.size(), 0);
-  for (size_t col = 0; col + 1 < h->h_colour_start.size(); ++col) {
-    h->h_gcolour_start[col] = (int)h->h_gchunks.size();
-    for (int i = h->h_colour_start[col]; i < h->h_colour_start[col + 1]; ++i) {
-      int c = h->h_colour_chunks[i];
-      bool any = false;
-      for (int e = h->h_chunk_start[c]; e < h->h_chunk_start[c + 1]; ++e) any |= info[e].needG != 0;
-      if (any) h->h_gchunks.push_back(c);
-    }
-  }
-  h->h_gcolour_start.back() = (int)h->h_gchunks.size();
+  build_gschedule(h, info);
+  std::vector<unsigned> ops;
+  build_write_plan(h, info, ops);
+  QD_ALLOC(h, h->elop, ops.size());
+  QD_HIP(h, hipMemcpy(h->elop.p, ops.data(), ops.size() * sizeof(unsigned), hipMemcpyHostToDevice));
+  if ((rc = upload_i(h, h->unwritten, h->h_unwritten))) return rc;
   QD_ALLOC(h, h->einfo, info.size());
   QD_HIP(h, hipMemcpy(h->einfo.p, info.data(), info.size() * sizeof(ElemInfo), hipMemcpyHostToDevice));
   if ((rc = upload_d(h, h->ebc, bc, 27 * (size_t)h->ne))) return rc;
@@ -527,17 +418,7 @@ int qd_ghmat8_rhs(qd_handle h, const double* bc, double* fi) {
   QD_HIP(h, hipMemcpy(dinfo.p, info.data(), info.size() * sizeof(ElemInfo), hipMemcpyHostToDevice));
   std::swap(h->einfo.p, dinfo.p);
   std::vector<int> save_g = h->h_gchunks, save_gs = h->h_gcolour_start;
-  h->h_gchunks.clear();
-  for (size_t col = 0; col + 1 < h->h_colour_start.size(); ++col) {
-    h->h_gcolour_start[col] = (int)h->h_gchunks.size();
-    for (int i = h->h_colour_start[col]; i < h->h_colour_start[col + 1]; ++i) {
-      int c = h->h_colour_chunks[i];
-      bool any = false;
-      for (int e = h->h_chunk_start[c]; e < h->h_chunk_start[c + 1]; ++e) any |= info[e].needG != 0;
-      if (any) h->h_gchunks.push_back(c);
-    }
-  }
-  h->h_gcolour_start.back() = (int)h->h_gchunks.size();
+  build_gschedule(h, info);
   DevBuf<int> dg;
   rc = upload_i(h, dg, h->h_gchunks);
   if (!rc) {

@@ -92,6 +92,24 @@ struct SubTri {
   double v1[3], v2[3], area, pad;
 };
 
+// Write plan of the assembly passes, kElemOps words per element: words 0-8
+// are the H pass's column-block operations of the element's nodes, word 9
+// the G pass's first-write bits (bit 3k+ld: column ld of node k is written
+// for the first time, so it starts from zero instead of being read).
+// An H-pass word: kOpFresh (the block's chain starts here: read A or start
+// from zero), kOpLast (the chain ends here: store to A), else the block is
+// carried to the chain's next element in LDS slot `out`; bits 2-4 the
+// chain's written columns, bits 5-7 those it starts from zero, bits 8-11
+// the slot it is carried in from, bits 12-15 the slot it is carried out in.
+constexpr int kElemOps = 16;
+constexpr int kCarrySlots = 4;
+static_assert((kCarrySlots & (kCarrySlots - 1)) == 0, "carry slots: a power of two");
+constexpr unsigned kOpFresh = 1u, kOpLast = 2u;
+__host__ __device__ constexpr int op_wcols(unsigned op) { return (int)((op >> 2) & 7u); }
+__host__ __device__ constexpr int op_zcols(unsigned op) { return (int)((op >> 5) & 7u); }
+__host__ __device__ constexpr int op_slot_in(unsigned op) { return (int)((op >> 8) & 15u); }
+__host__ __device__ constexpr int op_slot_out(unsigned op) { return (int)((op >> 12) & 15u); }
+
 constexpr int kGP = 36;         // Gauss points per element slot (6x6; tri6 uses 16)
 constexpr int kAsmStride = 16;  // doubles per Gauss point in the assembly table
 constexpr int kAsmStride32 = 16;  // floats per Gauss point in the FP32 far-field table
@@ -131,6 +149,9 @@ struct qd_context {
   qd::DevBuf<float> asmtab32;       // (ne*36, 16): eta(3) p12(9) pad, FP32
   qd::DevBuf<double> elfar;         // (ne, 4): centroid, squared far-field distance
   qd::DevBuf<qd::ElemInfo> einfo;
+  qd::DevBuf<unsigned> elop;        // (ne, kElemOps): the assembly's write plan
+  std::vector<int> h_unwritten;     // columns of A that no pass writes (zero-filled)
+  qd::DevBuf<int> unwritten;
   qd::DevBuf<double> ebc, evp;      // (ne, 27): bc and vp per element dof
   qd::DevBuf<int> node_k0;          // (n, 3) kode0 flag per node/direction
   qd::DevBuf<double> node_vp;       // (n, 3)
@@ -229,6 +250,12 @@ int set_error(qd_context* h, int code, const char* fmt, ...);
 
 // phases implemented in the .hip files
 int geometry_run(qd_context* h);
+// host-side assembly preprocessing (qd_plan.cpp)
+void build_schedule(qd_context* h);
+int bc_tables(qd_context* h, const int* kode, const double* bc, std::vector<ElemInfo>& info,
+              std::vector<double>& evp, std::vector<int>& nk0, std::vector<double>& nvp);
+void build_gschedule(qd_context* h, const std::vector<ElemInfo>& info);
+void build_write_plan(qd_context* h, const std::vector<ElemInfo>& info, std::vector<unsigned>& ops);
 // A (or, with into_pre, A_pre) and FI (or fi_out_dev); far32: far-field pairs in FP32.
 int assembly_run(qd_context* h, double* fi_out_dev, bool write_A, bool far32 = false, bool into_pre = false);
 int somigliana_run(qd_context* h, int l, const double* pts_dev, const double* fi_dev,
