@@ -248,6 +248,20 @@ __global__ __launch_bounds__(64) void k_hpass(AsmArgs a, const int* __restrict__
 #pragma unroll
           for (int jj = 0; jj < 3; ++jj) acc[kk][ii][jj] = (double)fa[kk][ii][jj];
     } else {
+    // The traction kernel is summed as its symmetric part
+    //   S(i,j) = c5/r^2 (dr/dn) (c6 d_ij + 3 r_,i r_,j)       (6 entries)
+    // and its antisymmetric part
+    //   W(i,j) = c5/r^2 c6 (n_j r_,i - n_i r_,j)               (3 entries)
+    // per shape function; h(j, 3k+i) = S(i,j) + W(i,j) is formed once per
+    // element.
+    double aS[NC][6], aW[NC][3];  // S: 00 11 22 01 12 02; W: 01 12 02
+#pragma unroll
+    for (int kk = 0; kk < NC; ++kk) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) aS[kk][c] = 0.0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) aW[kk][c] = 0.0;
+    }
     // a Gauss point's data (position, normal, weights) are loaded one point
     // ahead, in flight during the previous point's arithmetic
     double cur[6 + NC];
@@ -260,30 +274,38 @@ __global__ __launch_bounds__(64) void k_hpass(AsmArgs a, const int* __restrict__
 #pragma unroll
       for (int q = 0; q < 6 + NC; ++q) nxt[q] = gq[q];
       const double r1 = cur[0] - xp, r2 = cur[1] - yp, r3 = cur[2] - zp;
-      const double rr = r1 * r1 + r2 * r2 + r3 * r3;
-      const double ir = inv_sqrt(rr);
-      const double eta[3] = {cur[3], cur[4], cur[5]};
-      const double rd[3] = {r1 * ir, r2 * ir, r3 * ir};
-      const double rdn = (r1 * eta[0] + r2 * eta[1] + r3 * eta[2]) * ir;
+      const double ir = inv_sqrt(r1 * r1 + r2 * r2 + r3 * r3);
+      const double d0 = r1 * ir, d1v = r2 * ir, d2 = r3 * ir;
+      const double rdn = d0 * cur[3] + d1v * cur[4] + d2 * cur[5];
       const double c5r2 = c5 * (ir * ir);
-      double t[3][3];  // t[jj][ii]
-#pragma unroll
-      for (int ii = 0; ii < 3; ++ii)
-#pragma unroll
-        for (int jj = 0; jj < 3; ++jj) {
-          const double dl = (ii == jj) ? c6 : 0.0;
-          t[jj][ii] = c5r2 * (rdn * (dl + 3.0 * rd[ii] * rd[jj]) + c6 * (eta[jj] * rd[ii] - eta[ii] * rd[jj]));
-        }
+      const double sc = c5r2 * rdn, wc = c5r2 * c6;
+      const double s3 = 3.0 * sc, sd = sc * c6;
+      const double q0 = s3 * d0, q1 = s3 * d1v, q2 = s3 * d2;
+      const double S[6] = {q0 * d0 + sd, q1 * d1v + sd, q2 * d2 + sd, q0 * d1v, q1 * d2, q0 * d2};
+      const double u0 = wc * d0, u1 = wc * d1v, u2 = wc * d2;
+      const double W[3] = {cur[4] * u0 - cur[3] * u1, cur[5] * u1 - cur[4] * u2, cur[5] * u0 - cur[3] * u2};
 #pragma unroll
       for (int kk = 0; kk < NC; ++kk) {
         const double p = cur[6 + kk];
 #pragma unroll
-        for (int ii = 0; ii < 3; ++ii)
+        for (int c = 0; c < 6; ++c) aS[kk][c] += S[c] * p;
 #pragma unroll
-          for (int jj = 0; jj < 3; ++jj) acc[kk][ii][jj] += t[jj][ii] * p;
+        for (int c = 0; c < 3; ++c) aW[kk][c] += W[c] * p;
       }
 #pragma unroll
       for (int q = 0; q < 6 + NC; ++q) cur[q] = nxt[q];
+    }
+#pragma unroll
+    for (int kk = 0; kk < NC; ++kk) {
+      acc[kk][0][0] = aS[kk][0];
+      acc[kk][1][1] = aS[kk][1];
+      acc[kk][2][2] = aS[kk][2];
+      acc[kk][0][1] = aS[kk][3] + aW[kk][0];
+      acc[kk][1][0] = aS[kk][3] - aW[kk][0];
+      acc[kk][1][2] = aS[kk][4] + aW[kk][1];
+      acc[kk][2][1] = aS[kk][4] - aW[kk][1];
+      acc[kk][0][2] = aS[kk][5] + aW[kk][2];
+      acc[kk][2][0] = aS[kk][5] - aW[kk][2];
     }
     }
     const int kb = inf->kode, k0 = inf->k0mask;
