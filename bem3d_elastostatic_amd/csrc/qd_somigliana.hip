@@ -174,54 +174,63 @@ __global__ void k_near_table(int ne, const int* __restrict__ nconec, const int* 
 }
 
 struct SomiConst {
-  double nu, c4, c6, pr4, cU, cT, cD, cS, sg;
+  double nu, c4, c6, pr4, cU, cT, cD, cS, sc6;
 };
 
 // accumulate_somigliana_range for one Gauss point (stress_mod.f90:807-885);
-// sg = +1 keeps the reference's displacement T* sign, -1 the corrected one.
+// sc6 = c6 keeps the reference's displacement T* sign, -c6 the corrected one.
+// With r_i the unit vector from the point to the Gauss point, n the normal,
+// U = u w and T = t w the weighted boundary displacement and traction, and
+// the scalars rdn = r.n, rdt = r.T, rdu = r.U, nuv = n.U, the reference's
+// terms are regrouped by the vectors they multiply:
+//   u_i   += a_t T_i + a_r r_i + a_u U_i + a_n n_i
+//   D_ij - S_ij = C_i r_j + C_j r_i + B_i U_j + B_j U_i + d_ij k_d
+// with C_i = k_t T_i + k_u U_i + k_n n_i + k_rr/2 r_i and B_i = k_nu n_i;
+// the scalar coefficients are the reference's products of fU = cU/r,
+// fT = cT/r^2, fD = cD/r^2 and fS = cS/r^3 with rdn, rdt, rdu and nuv.
 __device__ __forceinline__ void somi_gp(const double* __restrict__ t, double px, double py, double pz,
-                                        const SomiConst& k, double* u, double* s) {
+                                        const SomiConst& k, double* u, double* s, double& sdiag) {
   const double rx = t[0] - px, ry = t[1] - py, rz = t[2] - pz;
   const double inv_r = inv_sqrt(rx * rx + ry * ry + rz * rz);
-  const double inv_r2 = inv_r * inv_r, inv_r3 = inv_r2 * inv_r;
-  const double rd1 = rx * inv_r, rd2 = ry * inv_r, rd3 = rz * inv_r;
-  const double n1 = t[3], n2 = t[4], n3 = t[5];
-  const double rdn = rd1 * n1 + rd2 * n2 + rd3 * n3;
-  const double tu1 = t[6], tu2 = t[7], tu3 = t[8];
-  const double tt1 = t[9], tt2 = t[10], tt3 = t[11];
-  const double rdt = rd1 * tt1 + rd2 * tt2 + rd3 * tt3;
-  const double rdu = rd1 * tu1 + rd2 * tu2 + rd3 * tu3;
-  const double nuv = n1 * tu1 + n2 * tu2 + n3 * tu3;
-  const double fU = k.cU * inv_r, fT = k.cT * inv_r2;
-  const double c4 = k.c4, c6 = k.c6, nu = k.nu, pr4 = k.pr4, sc6 = k.sg * c6;
-  u[0] += fU * (c4 * tt1 + rd1 * rdt) - fT * (rdn * (c6 * tu1 + 3.0 * rd1 * rdu) + sc6 * (nuv * rd1 - n1 * rdu));
-  u[1] += fU * (c4 * tt2 + rd2 * rdt) - fT * (rdn * (c6 * tu2 + 3.0 * rd2 * rdu) + sc6 * (nuv * rd2 - n2 * rdu));
-  u[2] += fU * (c4 * tt3 + rd3 * rdt) - fT * (rdn * (c6 * tu3 + 3.0 * rd3 * rdu) + sc6 * (nuv * rd3 - n3 * rdu));
-  const double fD = k.cD * inv_r2, fS = k.cS * inv_r3;
-  const double Dxx = fD * (c6 * (2.0 * tt1 * rd1 - rdt) + 3.0 * rd1 * rd1 * rdt);
-  const double Dyy = fD * (c6 * (2.0 * tt2 * rd2 - rdt) + 3.0 * rd2 * rd2 * rdt);
-  const double Dzz = fD * (c6 * (2.0 * tt3 * rd3 - rdt) + 3.0 * rd3 * rd3 * rdt);
-  const double Dxy = fD * (c6 * (tt1 * rd2 + tt2 * rd1) + 3.0 * rd1 * rd2 * rdt);
-  const double Dyz = fD * (c6 * (tt2 * rd3 + tt3 * rd2) + 3.0 * rd2 * rd3 * rdt);
-  const double Dzx = fD * (c6 * (tt3 * rd1 + tt1 * rd3) + 3.0 * rd3 * rd1 * rdt);
-  const double Sxx = fS * (3.0 * rdn * (c6 * rdu + 2.0 * nu * tu1 * rd1 - 5.0 * rd1 * rd1 * rdu) +
-                           6.0 * nu * rdu * n1 * rd1 + c6 * (3.0 * nuv * rd1 * rd1 + 2.0 * n1 * tu1) - pr4 * nuv);
-  const double Syy = fS * (3.0 * rdn * (c6 * rdu + 2.0 * nu * tu2 * rd2 - 5.0 * rd2 * rd2 * rdu) +
-                           6.0 * nu * rdu * n2 * rd2 + c6 * (3.0 * nuv * rd2 * rd2 + 2.0 * n2 * tu2) - pr4 * nuv);
-  const double Szz = fS * (3.0 * rdn * (c6 * rdu + 2.0 * nu * tu3 * rd3 - 5.0 * rd3 * rd3 * rdu) +
-                           6.0 * nu * rdu * n3 * rd3 + c6 * (3.0 * nuv * rd3 * rd3 + 2.0 * n3 * tu3) - pr4 * nuv);
-  const double Sxy = fS * (3.0 * rdn * (nu * (tu1 * rd2 + tu2 * rd1) - 5.0 * rd1 * rd2 * rdu) +
-                           3.0 * nu * rdu * (n1 * rd2 + n2 * rd1) + c6 * (3.0 * nuv * rd1 * rd2 + n1 * tu2 + n2 * tu1));
-  const double Syz = fS * (3.0 * rdn * (nu * (tu2 * rd3 + tu3 * rd2) - 5.0 * rd2 * rd3 * rdu) +
-                           3.0 * nu * rdu * (n2 * rd3 + n3 * rd2) + c6 * (3.0 * nuv * rd2 * rd3 + n2 * tu3 + n3 * tu2));
-  const double Szx = fS * (3.0 * rdn * (nu * (tu3 * rd1 + tu1 * rd3) - 5.0 * rd3 * rd1 * rdu) +
-                           3.0 * nu * rdu * (n3 * rd1 + n1 * rd3) + c6 * (3.0 * nuv * rd3 * rd1 + n3 * tu1 + n1 * tu3));
-  s[0] += Dxx - Sxx;
-  s[1] += Dyy - Syy;
-  s[2] += Dzz - Szz;
-  s[3] += Dxy - Sxy;
-  s[4] += Dyz - Syz;
-  s[5] += Dzx - Szx;
+  const double inv_r2 = inv_r * inv_r;
+  const double r[3] = {rx * inv_r, ry * inv_r, rz * inv_r};
+  const double n[3] = {t[3], t[4], t[5]};
+  const double U[3] = {t[6], t[7], t[8]};
+  const double T[3] = {t[9], t[10], t[11]};
+  const double rdn = r[0] * n[0] + r[1] * n[1] + r[2] * n[2];
+  const double rdt = r[0] * T[0] + r[1] * T[1] + r[2] * T[2];
+  const double rdu = r[0] * U[0] + r[1] * U[1] + r[2] * U[2];
+  const double nuv = n[0] * U[0] + n[1] * U[1] + n[2] * U[2];
+  const double fU = k.cU * inv_r, fT = k.cT * inv_r2, fD = k.cD * inv_r2, fS = k.cS * (inv_r2 * inv_r);
+  const double rnu = rdn * rdu;
+  // displacement
+  const double a_t = fU * k.c4;
+  const double a_r = fU * rdt - fT * (3.0 * rnu + k.sc6 * nuv);
+  const double a_u = -fT * rdn * k.c6;
+  const double a_n = fT * k.sc6 * rdu;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) u[i] += a_t * T[i] + a_r * r[i] + a_u * U[i] + a_n * n[i];
+  // stress
+  const double k_t = fD * k.c6;
+  const double k_rr2 = 1.5 * fD * rdt + fS * (7.5 * rnu - 1.5 * k.c6 * nuv);
+  const double k_u = -3.0 * fS * k.nu * rdn;
+  const double k_n = -3.0 * fS * k.nu * rdu;
+  const double k_nu = -fS * k.c6;
+  sdiag += -k_t * rdt - fS * (3.0 * k.c6 * rnu - k.pr4 * nuv);
+  double C[3], B[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    C[i] = k_t * T[i] + k_u * U[i] + k_n * n[i] + k_rr2 * r[i];
+    B[i] = k_nu * n[i];
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) s[i] += 2.0 * (C[i] * r[i] + B[i] * U[i]);
+  // off-diagonal xy, yz, zx
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int i = q, j = (q + 1) % 3;
+    s[3 + q] += C[i] * r[j] + C[j] * r[i] + B[i] * U[j] + B[j] * U[i];
+  }
 }
 
 // Far pass: grid (point tiles of 64, element slices).
@@ -236,7 +245,7 @@ __global__ __launch_bounds__(64) void k_somi_far(int l, int ne, int slice, const
   const double px = pts[3 * ipr], py = pts[3 * ipr + 1], pz = pts[3 * ipr + 2];
   const int e0 = blockIdx.y * slice;
   const int e1 = min(ne, e0 + slice);
-  double u[3] = {0.0, 0.0, 0.0}, s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double u[3] = {0.0, 0.0, 0.0}, s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, sdiag = 0.0;
   for (int e = e0; e < e1; ++e) {
     const double* c = cen + 4 * e;
     const double dx = px - c[0], dy = py - c[1], dz = pz - c[2];
@@ -254,13 +263,14 @@ __global__ __launch_bounds__(64) void k_somi_far(int l, int ne, int slice, const
           const double* tq = tab + (size_t)(g + 1 < g1 ? g + 1 : g) * kSomiStride;
 #pragma unroll
           for (int q = 0; q < kSomiStride; ++q) nxt[q] = tq[q];
-          somi_gp(cur, px, py, pz, k, u, s);
+          somi_gp(cur, px, py, pz, k, u, s, sdiag);
 #pragma unroll
           for (int q = 0; q < kSomiStride; ++q) cur[q] = nxt[q];
         }
       }
     }
   }
+  for (int d = 0; d < 3; ++d) s[d] += sdiag;
   if (valid) {
     double* o = part + ((size_t)blockIdx.y * l + ip) * 9;
     for (int d = 0; d < 3; ++d) o[d] = u[d];
@@ -284,7 +294,7 @@ __global__ __launch_bounds__(64) void k_somi_near(int l, int ne, const double* _
   if (ip >= l) return;
   const int lane = threadIdx.x;
   const double px = pts[3 * ip], py = pts[3 * ip + 1], pz = pts[3 * ip + 2];
-  double u[3] = {0.0, 0.0, 0.0}, s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double u[3] = {0.0, 0.0, 0.0}, s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, sdiag = 0.0;
   unsigned long long near_gp = 0, skipped_far = 0;
   for (int base = 0; base < ne; base += 64) {
     const int e = base + lane;
@@ -312,7 +322,7 @@ __global__ __launch_bounds__(64) void k_somi_near(int l, int ne, const double* _
           const double* tq = t + (size_t)(g + 64 < kNear ? g + 64 : g) * kSomiStride;
 #pragma unroll
           for (int q = 0; q < kSomiStride; ++q) nxt[q] = tq[q];
-          somi_gp(cur, px, py, pz, k, u, s);
+          somi_gp(cur, px, py, pz, k, u, s, sdiag);
 #pragma unroll
           for (int q = 0; q < kSomiStride; ++q) cur[q] = nxt[q];
         }
@@ -321,6 +331,7 @@ __global__ __launch_bounds__(64) void k_somi_near(int l, int ne, const double* _
       skipped_far += (nconec[en] == 9) ? 36 : 16;
     }
   }
+  for (int d = 0; d < 3; ++d) s[d] += sdiag;
   double r[9];
   for (int d = 0; d < 3; ++d) r[d] = wave_sum(u[d]);
   for (int d = 0; d < 6; ++d) r[3 + d] = wave_sum(s[d]);
@@ -443,7 +454,7 @@ int somigliana_run(qd_context* h, int l, const double* pts, const double* fi, co
   k.cT = -1.0 / (8.0 * pi * (1.0 - nu));
   k.cD = 1.0 / (8.0 * pi * (1.0 - nu));
   k.cS = mu / (4.0 * pi * (1.0 - nu));
-  k.sg = variant ? -1.0 : 1.0;
+  k.sc6 = (variant ? -1.0 : 1.0) * k.c6;
   double* part = h->somi_part.p;
   double* nearp = part + (size_t)nslice * l * 9;
   // ~250 flops per Gauss-point evaluation of accumulate_somigliana_range; the
