@@ -189,6 +189,13 @@ struct qd_context {
   qd::DevBuf<int> far_start;
   qd::DevBuf<double> tmp_fi, tmp_dfi, tmp_pts, tmp_out;
   qd::DevBuf<long long> counters;
+  // near point-element pairs: point-major list (ptr, element, point), the
+  // element-major index, the near pass's work items and per-pair sums
+  qd::DevBuf<int> near_cnt, near_ptr, near_pe, near_pp, near_ecnt, near_efill, near_eptr, near_ep;
+  qd::DevBuf<int> near_cch, near_cptr;
+  qd::DevBuf<int2> near_items;
+  qd::DevBuf<double> near_out;
+  qd::DevBuf<long long> near_total;
 
   // instrumentation (qd_profile_enable / qd_profile_read)
   struct ProfRec {

@@ -15,6 +15,7 @@
 //    von Mises (stress_mod.f90:299-359).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -278,24 +279,21 @@ __global__ __launch_bounds__(64) void k_somi_far(int l, int ne, int slice, const
   }
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
-// Near pass: one wave per internal point.
-__global__ __launch_bounds__(64) void k_somi_near(int l, int ne, const double* __restrict__ pts,
-                                                   const double* __restrict__ cen,
-                                                   const int* __restrict__ nconec,
-                                                   const double* __restrict__ tab, SomiConst k,
-                                                   double* __restrict__ out,
-                                                   unsigned long long* __restrict__ counts) {
+// Near pairs.  A point and an element are near when the point is within the
+// element's near threshold of its centroid (stress_mod.f90:584-604, 654).
+// The pairs are listed point by point, elements ascending (one wave per
+// point, a ballot per 64 elements): k_near_pairs counts them (fill = 0),
+// then writes them at the point's offset (fill = 1) with the point of each
+// pair in pp.
+__global__ __launch_bounds__(64) void k_near_pairs(int l, int ne, const double* __restrict__ pts,
+                                                    const double* __restrict__ cen, int fill,
+                                                    const int* __restrict__ ptr, int* __restrict__ cnt,
+                                                    int* __restrict__ pe, int* __restrict__ pp) {
   const int ip = blockIdx.x;
   if (ip >= l) return;
   const int lane = threadIdx.x;
   const double px = pts[3 * ip], py = pts[3 * ip + 1], pz = pts[3 * ip + 2];
-  double u[3] = {0.0, 0.0, 0.0}, s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, sdiag = 0.0;
-  unsigned long long near_gp = 0, skipped_far = 0;
+  int run = fill ? ptr[ip] : 0;
   for (int base = 0; base < ne; base += 64) {
     const int e = base + lane;
     bool nearp = false;
@@ -304,41 +302,133 @@ __global__ __launch_bounds__(64) void k_somi_near(int l, int ne, const double* _
       const double dx = px - c[0], dy = py - c[1], dz = pz - c[2];
       nearp = !(dx * dx + dy * dy + dz * dz > c[3]);
     }
-    unsigned long long mask = __ballot(nearp);
-    while (mask) {
-      const int b = __builtin_ctzll(mask);
-      mask &= mask - 1;
-      const int en = base + b;
-      const double* t = tab + (size_t)en * kNear * kSomiStride;
-      if (lane < kNear) {
-        // each lane's next Gauss point loaded one ahead, in flight during the
-        // current one's arithmetic
-        double cur[kSomiStride];
-#pragma unroll
-        for (int q = 0; q < kSomiStride; ++q) cur[q] = t[(size_t)lane * kSomiStride + q];
-#pragma unroll 1
-        for (int g = lane; g < kNear; g += 64) {
-          double nxt[kSomiStride];
-          const double* tq = t + (size_t)(g + 64 < kNear ? g + 64 : g) * kSomiStride;
-#pragma unroll
-          for (int q = 0; q < kSomiStride; ++q) nxt[q] = tq[q];
-          somi_gp(cur, px, py, pz, k, u, s, sdiag);
-#pragma unroll
-          for (int q = 0; q < kSomiStride; ++q) cur[q] = nxt[q];
-        }
-      }
-      near_gp += kNear;
-      skipped_far += (nconec[en] == 9) ? 36 : 16;
+    const unsigned long long mask = __ballot(nearp);
+    if (fill && nearp) {
+      const int pos = run + __popcll(mask & ((1ull << lane) - 1ull));
+      pe[pos] = e;
+      pp[pos] = ip;
     }
+    run += __popcll(mask);
   }
-  for (int d = 0; d < 3; ++d) s[d] += sdiag;
-  double r[9];
-  for (int d = 0; d < 3; ++d) r[d] = wave_sum(u[d]);
-  for (int d = 0; d < 6; ++d) r[3 + d] = wave_sum(s[d]);
-  if (lane == 0) {
-    for (int d = 0; d < 9; ++d) out[(size_t)ip * 9 + d] = r[d];
-    atomicAdd(counts + 0, skipped_far);
-    atomicAdd(counts + 1, near_gp);
+  if (!fill && lane == 0) cnt[ip] = run;
+}
+
+// Exclusive prefix sum of in[0, n) into out[0, n], the total into out[n]
+// and *total: one workgroup, each thread summing a contiguous run.
+__global__ __launch_bounds__(1024) void k_scan_excl(int n, const int* __restrict__ in, int* __restrict__ out,
+                                                    long long* __restrict__ total) {
+  __shared__ long long part[1024];
+  const int t = threadIdx.x;
+  const int per = (n + 1023) / 1024;
+  const int b = min(n, t * per), e = min(n, b + per);
+  long long sum = 0;
+  for (int i = b; i < e; ++i) sum += in[i];
+  part[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const long long v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  long long run = part[t] - sum;
+  for (int i = b; i < e; ++i) {
+    out[i] = (int)run;
+    run += in[i];
+  }
+  if (t == 1023) {
+    out[n] = (int)part[1023];
+    if (total) *total = part[1023];
+  }
+}
+
+__global__ void k_zero_int(int n, int* __restrict__ p) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// Pairs per element (the order of the atomics does not matter: counts).
+__global__ void k_near_ecount(const int* __restrict__ ptr, int l, const int* __restrict__ pe,
+                              int* __restrict__ ecnt) {
+  const int total = ptr[l];
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x)
+    atomicAdd(ecnt + pe[q], 1);
+}
+
+// Per element: its 64-pair work items, and the evaluation counters (far
+// points skipped, near points evaluated).
+__global__ void k_near_items_count(int ne, const int* __restrict__ nconec, const int* __restrict__ ecnt,
+                                   int* __restrict__ cch, unsigned long long* __restrict__ counts) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int c = ecnt[e];
+  cch[e] = (c + 63) / 64;
+  if (c) {
+    atomicAdd(counts + 0, (unsigned long long)c * (nconec[e] == 9 ? 36 : 16));
+    atomicAdd(counts + 1, (unsigned long long)c * kNear);
+  }
+}
+
+// Element-major index of the pairs: ep[eptr[e] ..) holds the pairs of
+// element e in an arbitrary order (each pair has its own output row).
+__global__ void k_near_efill(const int* __restrict__ ptr, int l, const int* __restrict__ pe,
+                             const int* __restrict__ eptr, int* __restrict__ efill, int* __restrict__ ep) {
+  const int total = ptr[l];
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
+    const int e = pe[q];
+    ep[eptr[e] + atomicAdd(efill + e, 1)] = q;
+  }
+}
+
+__global__ void k_near_items(int ne, const int* __restrict__ cch, const int* __restrict__ cptr,
+                             int2* __restrict__ items) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  for (int c = 0; c < cch[e]; ++c) items[cptr[e] + c] = make_int2(e, 64 * c);
+}
+
+// Near pass, element-major: a work item is an element and up to 64 of its
+// near points, one per lane; the element's 40x40 table rows are
+// wave-uniform (scalar loads), so each table is read once per item.  Each
+// lane sums its pair over the table in order and writes the pair's row.
+__global__ __launch_bounds__(64) void k_somi_near(const int* __restrict__ cptr, int ne,
+                                                   const int2* __restrict__ items,
+                                                   const int* __restrict__ eptr, const int* __restrict__ ep,
+                                                   const int* __restrict__ pp, const double* __restrict__ pts,
+                                                   const double* __restrict__ tab, SomiConst k,
+                                                   double* __restrict__ out) {
+  const int nitems = cptr[ne];
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const int2 item = items[it];
+    const int e = item.x;
+    const int j = item.y + (int)threadIdx.x;
+    const bool active = eptr[e] + j < eptr[e + 1];
+    const int q = active ? ep[eptr[e] + j] : 0;
+    const int ip = active ? pp[q] : 0;
+    const double px = pts[3 * ip], py = pts[3 * ip + 1], pz = pts[3 * ip + 2];
+    double u[3] = {0.0, 0.0, 0.0}, s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, sdiag = 0.0;
+    const double* t = tab + (size_t)e * kNear * kSomiStride;
+    // each row loaded one point ahead, in flight during the previous
+    // point's arithmetic
+    double cur[kSomiStride];
+#pragma unroll
+    for (int qq = 0; qq < kSomiStride; ++qq) cur[qq] = t[qq];
+#pragma unroll 1
+    for (int g = 0; g < kNear; ++g) {
+      double nxt[kSomiStride];
+      const double* tq = t + (size_t)(g + 1 < kNear ? g + 1 : g) * kSomiStride;
+#pragma unroll
+      for (int qq = 0; qq < kSomiStride; ++qq) nxt[qq] = tq[qq];
+      somi_gp(cur, px, py, pz, k, u, s, sdiag);
+#pragma unroll
+      for (int qq = 0; qq < kSomiStride; ++qq) cur[qq] = nxt[qq];
+    }
+    if (active) {
+      double* o = out + (size_t)q * 9;
+      for (int d = 0; d < 3; ++d) o[d] = u[d];
+      for (int d = 0; d < 3; ++d) o[3 + d] = s[d] + sdiag;
+      for (int d = 3; d < 6; ++d) o[3 + d] = s[d];
+    }
   }
 }
 
@@ -374,7 +464,7 @@ __device__ void principal_vm(const double* sig, double* prin, double* vm) {
 }
 
 __global__ void k_somi_final(int l, int nslice, const double* __restrict__ part,
-                             const double* __restrict__ nearp, double* __restrict__ u_out,
+                             const int* __restrict__ nptr, const double* __restrict__ nearp, double* __restrict__ u_out,
                              double* __restrict__ s_out, double* __restrict__ p_out,
                              double* __restrict__ vm_out) {
   const int ip = blockIdx.x * blockDim.x + threadIdx.x;
@@ -385,7 +475,8 @@ __global__ void k_somi_final(int l, int nslice, const double* __restrict__ part,
     const double* o = part + ((size_t)sl * l + ip) * 9;
     for (int d = 0; d < 9; ++d) r[d] += o[d];
   }
-  for (int d = 0; d < 9; ++d) r[d] += nearp[(size_t)ip * 9 + d];
+  for (int q = nptr[ip]; q < nptr[ip + 1]; ++q)
+    for (int d = 0; d < 9; ++d) r[d] += nearp[(size_t)q * 9 + d];
   if (u_out)
     for (int d = 0; d < 3; ++d) u_out[3 * (size_t)ip + d] = r[d];
   if (s_out)
@@ -429,7 +520,7 @@ int somigliana_run(qd_context* h, int l, const double* pts, const double* fi, co
   QD_ALLOC(h, h->el_centre, 4 * (size_t)ne);
   const int slice = 256;
   const int nslice = (ne + slice - 1) / slice;
-  QD_ALLOC(h, h->somi_part, ((size_t)nslice + 1) * l * 9);
+  QD_ALLOC(h, h->somi_part, (size_t)nslice * l * 9);
   QD_ALLOC(h, h->counters, 2);
   QD_HIP(h, hipMemsetAsync(h->counters.p, 0, 2 * sizeof(long long), h->stream));
 
@@ -456,7 +547,6 @@ int somigliana_run(qd_context* h, int l, const double* pts, const double* fi, co
   k.cS = mu / (4.0 * pi * (1.0 - nu));
   k.sc6 = (variant ? -1.0 : 1.0) * k.c6;
   double* part = h->somi_part.p;
-  double* nearp = part + (size_t)nslice * l * 9;
   // ~250 flops per Gauss-point evaluation of accumulate_somigliana_range; the
   // work of both passes is set below from the evaluation counters
   {
@@ -464,13 +554,61 @@ int somigliana_run(qd_context* h, int l, const double* pts, const double* fi, co
     hipLaunchKernelGGL(k_somi_far, dim3((l + 63) / 64, nslice), dim3(64), 0, h->stream, l, ne, slice, pts,
                        h->el_centre.p, h->far_start.p, h->far_tab.p, k, part);
   }
-  {
+  // near pairs: point-major list (its size sets the allocations), then the
+  // element-major index and the work items of the near pass
+  QD_ALLOC(h, h->near_cnt, (size_t)l);
+  QD_ALLOC(h, h->near_ptr, (size_t)l + 1);
+  QD_ALLOC(h, h->near_total, 1);
+  hipLaunchKernelGGL(k_near_pairs, dim3(l), dim3(64), 0, h->stream, l, ne, pts, h->el_centre.p, 0,
+                     (const int*)nullptr, h->near_cnt.p, (int*)nullptr, (int*)nullptr);
+  hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, h->stream, l, h->near_cnt.p, h->near_ptr.p,
+                     h->near_total.p);
+  QD_HIP(h, hipGetLastError());
+  long long npairs = 0;
+  QD_HIP(h, hipMemcpyAsync(&npairs, h->near_total.p, sizeof(npairs), hipMemcpyDeviceToHost, h->stream));
+  QD_HIP(h, hipStreamSynchronize(h->stream));
+  if (npairs > 2147483647LL)
+    return set_error(h, QD_ERR_ARG, "compute_internal_points: %lld near point-element pairs", npairs);
+  const size_t np = (size_t)std::max<long long>(npairs, 1);
+  const size_t nitems_max = (size_t)(npairs / 64) + (size_t)ne;
+  QD_ALLOC(h, h->near_pe, np);
+  QD_ALLOC(h, h->near_pp, np);
+  QD_ALLOC(h, h->near_ep, np);
+  QD_ALLOC(h, h->near_out, np * 9);
+  QD_ALLOC(h, h->near_ecnt, (size_t)ne);
+  QD_ALLOC(h, h->near_efill, (size_t)ne);
+  QD_ALLOC(h, h->near_eptr, (size_t)ne + 1);
+  QD_ALLOC(h, h->near_cch, (size_t)ne);
+  QD_ALLOC(h, h->near_cptr, (size_t)ne + 1);
+  QD_ALLOC(h, h->near_items, nitems_max);
+  if (npairs > 0) {
+    hipLaunchKernelGGL(k_near_pairs, dim3(l), dim3(64), 0, h->stream, l, ne, pts, h->el_centre.p, 1,
+                       (const int*)h->near_ptr.p, (int*)nullptr, h->near_pe.p, h->near_pp.p);
+    const unsigned eb = (unsigned)((ne + 255) / 256);
+    hipLaunchKernelGGL(k_zero_int, dim3(eb), dim3(256), 0, h->stream, ne, h->near_ecnt.p);
+    hipLaunchKernelGGL(k_zero_int, dim3(eb), dim3(256), 0, h->stream, ne, h->near_efill.p);
+    const unsigned qb = (unsigned)std::min<long long>((npairs + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_near_ecount, dim3(qb), dim3(256), 0, h->stream, (const int*)h->near_ptr.p, l,
+                       (const int*)h->near_pe.p, h->near_ecnt.p);
+    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, h->stream, ne, h->near_ecnt.p, h->near_eptr.p,
+                       (long long*)nullptr);
+    hipLaunchKernelGGL(k_near_items_count, dim3(eb), dim3(256), 0, h->stream, ne, h->nconec.p, h->near_ecnt.p,
+                       h->near_cch.p, (unsigned long long*)h->counters.p);
+    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, h->stream, ne, h->near_cch.p, h->near_cptr.p,
+                       (long long*)nullptr);
+    hipLaunchKernelGGL(k_near_efill, dim3(qb), dim3(256), 0, h->stream, (const int*)h->near_ptr.p, l,
+                       (const int*)h->near_pe.p, (const int*)h->near_eptr.p, h->near_efill.p, h->near_ep.p);
+    hipLaunchKernelGGL(k_near_items, dim3(eb), dim3(256), 0, h->stream, ne, (const int*)h->near_cch.p,
+                       (const int*)h->near_cptr.p, h->near_items.p);
+    QD_HIP(h, hipGetLastError());
     ProfScope ps(h, QD_PROF_SOMIGLIANA_NEAR, 0.0);
-    hipLaunchKernelGGL(k_somi_near, dim3(l), dim3(64), 0, h->stream, l, ne, pts, h->el_centre.p,
-                       h->nconec.p, h->near_tab.p, k, nearp, (unsigned long long*)h->counters.p);
+    const unsigned grid = (unsigned)std::min<size_t>(nitems_max, 16384);
+    hipLaunchKernelGGL(k_somi_near, dim3(grid), dim3(64), 0, h->stream, (const int*)h->near_cptr.p, ne,
+                       (const int2*)h->near_items.p, (const int*)h->near_eptr.p, (const int*)h->near_ep.p,
+                       (const int*)h->near_pp.p, pts, h->near_tab.p, k, h->near_out.p);
   }
-  hipLaunchKernelGGL(k_somi_final, dim3((l + 127) / 128), dim3(128), 0, h->stream, l, nslice, part, nearp,
-                     u, sig, prin, vm);
+  hipLaunchKernelGGL(k_somi_final, dim3((l + 127) / 128), dim3(128), 0, h->stream, l, nslice, part,
+                     (const int*)h->near_ptr.p, h->near_out.p, u, sig, prin, vm);
   QD_HIP(h, hipGetLastError());
   const bool prof = ((h->prof_mask >> QD_PROF_SOMIGLIANA_FAR) & 1) || ((h->prof_mask >> QD_PROF_SOMIGLIANA_NEAR) & 1);
   if (counts || prof) {
