@@ -25,6 +25,9 @@
 namespace qd {
 
 constexpr int kSomiStride = 12;
+// far table rows add the products that do not depend on the point:
+// n.U and the symmetrised n (x) U (xx yy zz as 2 n_i U_i; xy yz zx)
+constexpr int kFarStride = 20;
 constexpr int kNear = kNearGL * kNearGL;
 
 __constant__ double c_gl40x[kNearGL];
@@ -114,7 +117,7 @@ __global__ void k_far_table(int ne, const int* __restrict__ nconec, const int* _
       tg[d] += fv * dfi[e * 27 + 3 * m + d];
     }
   }
-  double* t = tab + (size_t)(far_start[e] + k) * kSomiStride;
+  double* t = tab + (size_t)(far_start[e] + k) * kFarStride;
   t[0] = geo[4 * blk + slot];
   t[1] = geo[5 * blk + slot];
   t[2] = geo[6 * blk + slot];
@@ -125,6 +128,14 @@ __global__ void k_far_table(int ne, const int* __restrict__ nconec, const int* _
     t[6 + d] = ug[d] * wg;
     t[9 + d] = tg[d] * wg;
   }
+  const double* n = t + 3;
+  const double* U = t + 6;
+  t[12] = n[0] * U[0] + n[1] * U[1] + n[2] * U[2];
+  for (int d = 0; d < 3; ++d) {
+    t[13 + d] = 2.0 * n[d] * U[d];
+    t[16 + d] = n[d] * U[(d + 1) % 3] + n[(d + 1) % 3] * U[d];
+  }
+  t[19] = 0.0;
 }
 
 // Near table (40x40 per element), stress_mod.f90:659-726.
@@ -189,6 +200,8 @@ struct SomiConst {
 // with C_i = k_t T_i + k_u U_i + k_n n_i + k_rr/2 r_i and B_i = k_nu n_i;
 // the scalar coefficients are the reference's products of fU = cU/r,
 // fT = cT/r^2, fD = cD/r^2 and fS = cS/r^3 with rdn, rdt, rdu and nuv.
+// FAR: the row carries n.U and n (x) U (kFarStride layout).
+template <bool FAR>
 __device__ __forceinline__ void somi_gp(const double* __restrict__ t, double px, double py, double pz,
                                         const SomiConst& k, double* u, double* s, double& sdiag) {
   const double rx = t[0] - px, ry = t[1] - py, rz = t[2] - pz;
@@ -201,7 +214,7 @@ __device__ __forceinline__ void somi_gp(const double* __restrict__ t, double px,
   const double rdn = r[0] * n[0] + r[1] * n[1] + r[2] * n[2];
   const double rdt = r[0] * T[0] + r[1] * T[1] + r[2] * T[2];
   const double rdu = r[0] * U[0] + r[1] * U[1] + r[2] * U[2];
-  const double nuv = n[0] * U[0] + n[1] * U[1] + n[2] * U[2];
+  const double nuv = FAR ? t[12] : n[0] * U[0] + n[1] * U[1] + n[2] * U[2];
   const double fU = k.cU * inv_r, fT = k.cT * inv_r2, fD = k.cD * inv_r2, fS = k.cS * (inv_r2 * inv_r);
   const double rnu = rdn * rdu;
   // displacement
@@ -210,7 +223,7 @@ __device__ __forceinline__ void somi_gp(const double* __restrict__ t, double px,
   const double a_u = -fT * rdn * k.c6;
   const double a_n = fT * k.sc6 * rdu;
 #pragma unroll
-  for (int i = 0; i < 3; ++i) u[i] += a_t * T[i] + a_r * r[i] + a_u * U[i] + a_n * n[i];
+  for (int i = 0; i < 3; ++i) u[i] = fma(a_n, n[i], fma(a_u, U[i], fma(a_r, r[i], fma(a_t, T[i], u[i]))));
   // stress
   const double k_t = fD * k.c6;
   const double k_rr2 = 1.5 * fD * rdt + fS * (7.5 * rnu - 1.5 * k.c6 * nuv);
@@ -218,19 +231,29 @@ __device__ __forceinline__ void somi_gp(const double* __restrict__ t, double px,
   const double k_n = -3.0 * fS * k.nu * rdu;
   const double k_nu = -fS * k.c6;
   sdiag += -k_t * rdt - fS * (3.0 * k.c6 * rnu - k.pr4 * nuv);
-  double C[3], B[3];
+  double C[3];
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    C[i] = k_t * T[i] + k_u * U[i] + k_n * n[i] + k_rr2 * r[i];
-    B[i] = k_nu * n[i];
-  }
+  for (int i = 0; i < 3; ++i) C[i] = k_t * T[i] + k_u * U[i] + k_n * n[i] + k_rr2 * r[i];
+  if (FAR) {
 #pragma unroll
-  for (int i = 0; i < 3; ++i) s[i] += 2.0 * (C[i] * r[i] + B[i] * U[i]);
-  // off-diagonal xy, yz, zx
+    for (int i = 0; i < 3; ++i) s[i] = fma(k_nu, t[13 + i], fma(2.0 * C[i], r[i], s[i]));
+    // off-diagonal xy, yz, zx
 #pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    const int i = q, j = (q + 1) % 3;
-    s[3 + q] += C[i] * r[j] + C[j] * r[i] + B[i] * U[j] + B[j] * U[i];
+    for (int q = 0; q < 3; ++q) {
+      const int i = q, j = (q + 1) % 3;
+      s[3 + q] = fma(k_nu, t[16 + q], fma(C[j], r[i], fma(C[i], r[j], s[3 + q])));
+    }
+  } else {
+    double B[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) B[i] = k_nu * n[i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) s[i] = fma(2.0 * B[i], U[i], fma(2.0 * C[i], r[i], s[i]));
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const int i = q, j = (q + 1) % 3;
+      s[3 + q] = fma(B[j], U[i], fma(B[i], U[j], fma(C[j], r[i], fma(C[i], r[j], s[3 + q]))));
+    }
   }
 }
 
@@ -255,18 +278,18 @@ __global__ __launch_bounds__(64) void k_somi_far(int l, int ne, int slice, const
       if (g1 > g0) {
         // each Gauss point's row loaded one point ahead, in flight during the
         // previous point's arithmetic
-        double cur[kSomiStride];
+        double cur[kFarStride];
 #pragma unroll
-        for (int q = 0; q < kSomiStride; ++q) cur[q] = tab[(size_t)g0 * kSomiStride + q];
+        for (int q = 0; q < kFarStride; ++q) cur[q] = tab[(size_t)g0 * kFarStride + q];
 #pragma unroll 1
         for (int g = g0; g < g1; ++g) {
-          double nxt[kSomiStride];
-          const double* tq = tab + (size_t)(g + 1 < g1 ? g + 1 : g) * kSomiStride;
+          double nxt[kFarStride];
+          const double* tq = tab + (size_t)(g + 1 < g1 ? g + 1 : g) * kFarStride;
 #pragma unroll
-          for (int q = 0; q < kSomiStride; ++q) nxt[q] = tq[q];
-          somi_gp(cur, px, py, pz, k, u, s, sdiag);
+          for (int q = 0; q < kFarStride; ++q) nxt[q] = tq[q];
+          somi_gp<true>(cur, px, py, pz, k, u, s, sdiag);
 #pragma unroll
-          for (int q = 0; q < kSomiStride; ++q) cur[q] = nxt[q];
+          for (int q = 0; q < kFarStride; ++q) cur[q] = nxt[q];
         }
       }
     }
@@ -419,7 +442,7 @@ __global__ __launch_bounds__(64) void k_somi_near(const int* __restrict__ cptr, 
       const double* tq = t + (size_t)(g + 1 < kNear ? g + 1 : g) * kSomiStride;
 #pragma unroll
       for (int qq = 0; qq < kSomiStride; ++qq) nxt[qq] = tq[qq];
-      somi_gp(cur, px, py, pz, k, u, s, sdiag);
+      somi_gp<false>(cur, px, py, pz, k, u, s, sdiag);
 #pragma unroll
       for (int qq = 0; qq < kSomiStride; ++qq) cur[qq] = nxt[qq];
     }
@@ -515,7 +538,7 @@ int somigliana_run(qd_context* h, int l, const double* pts, const double* fi, co
   fs[ne] = tot;
   QD_ALLOC(h, h->far_start, (size_t)ne + 1);
   QD_HIP(h, hipMemcpy(h->far_start.p, fs.data(), fs.size() * sizeof(int), hipMemcpyHostToDevice));
-  QD_ALLOC(h, h->far_tab, (size_t)tot * kSomiStride);
+  QD_ALLOC(h, h->far_tab, (size_t)tot * kFarStride);
   QD_ALLOC(h, h->near_tab, (size_t)ne * kNear * kSomiStride);
   QD_ALLOC(h, h->el_centre, 4 * (size_t)ne);
   const int slice = 256;
