@@ -816,14 +816,15 @@ int assembly_run(qd_context* h, double* fi_out_dev, bool write_A, bool far32, bo
         int j = i;
         while (j < e && h->hnconec[h->h_chunk_start[list[j]]] == kind) ++j;
         dim3 grid(tiles, j - i);
-        // algorithmic work: Gauss-point evaluations x flops of one evaluation
-        // (H: Kelvin traction kernel ~110 flops + 18 per shape function of
-        // accumulation; G: displacement kernel ~40 + 12 per shape function)
+        // work: Gauss-point evaluations x FP64 flops of one evaluation as
+        // these passes compute it (H: traction kernel with 1/r ~56 flops + 18
+        // per shape function of accumulation; G: displacement kernel ~40 + 12
+        // per shape function)
         double gpe = 0.0;
         for (int q = i; q < j; ++q)
           for (int el = h->h_chunk_start[list[q]]; el < h->h_chunk_start[list[q] + 1]; ++el)
             gpe += (double)n * (kind == 9 ? 36.0 : 16.0);
-        const double fpg = pass == 0 ? 110.0 + 18.0 * kind : 40.0 + 12.0 * kind;
+        const double fpg = pass == 0 ? 56.0 + 18.0 * kind : 40.0 + 12.0 * kind;
         ProfScope ps(h, pass == 0 ? QD_PROF_ASSEMBLY_H : QD_PROF_ASSEMBLY_G, gpe * fpg);
         if (pass == 0) {
           a.fpart = h->fpart_h.p;

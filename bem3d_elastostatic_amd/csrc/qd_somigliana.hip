@@ -570,10 +570,12 @@ int somigliana_run(qd_context* h, int l, const double* pts, const double* fi, co
   k.cS = mu / (4.0 * pi * (1.0 - nu));
   k.sc6 = (variant ? -1.0 : 1.0) * k.c6;
   double* part = h->somi_part.p;
-  // ~250 flops per Gauss-point evaluation of accumulate_somigliana_range; the
-  // work of both passes is set below from the evaluation counters
+  // FP64 flops of one evaluation as somi_gp computes it (the far form reads
+  // n.U and n x U from its table); the work of both passes is set below
+  // from the evaluation counters
+  constexpr double kFarFlops = 152.0, kNearFlops = 169.0;
   {
-    ProfScope ps(h, QD_PROF_SOMIGLIANA_FAR, 250.0 * (double)tot * l);
+    ProfScope ps(h, QD_PROF_SOMIGLIANA_FAR, kFarFlops * (double)tot * l);
     hipLaunchKernelGGL(k_somi_far, dim3((l + 63) / 64, nslice), dim3(64), 0, h->stream, l, ne, slice, pts,
                        h->el_centre.p, h->far_start.p, h->far_tab.p, k, part);
   }
@@ -646,14 +648,14 @@ int somigliana_run(qd_context* h, int l, const double* pts, const double* fi, co
       counts[1] = nnear;
     }
     // the profile records of this call: the evaluations each pass made
-    // (known only now), at ~250 flops each
+    // (known only now)
     bool far_done = false, near_done = false;
     for (auto it = h->prof_pending.rbegin(); it != h->prof_pending.rend() && !(far_done && near_done); ++it) {
       if (it->kind == QD_PROF_SOMIGLIANA_FAR && !far_done) {
-        it->work = 250.0 * (double)nfar;
+        it->work = kFarFlops * (double)nfar;
         far_done = true;
       } else if (it->kind == QD_PROF_SOMIGLIANA_NEAR && !near_done) {
-        it->work = 250.0 * (double)nnear;
+        it->work = kNearFlops * (double)nnear;
         near_done = true;
       }
     }
