@@ -32,6 +32,7 @@ module quadesth_capi
   integer(c_int), parameter :: QD_OK = 0
   integer(c_int), parameter :: QD_FP64 = 0, QD_FP32_IR = 1, QD_FP16_IR = 2
   integer(c_int), parameter :: QD_SOMIGLIANA_REFERENCE = 0, QD_SOMIGLIANA_CORRECTED = 1
+  integer(c_int), parameter :: QD_ASSEMBLY_FP64 = 0, QD_ASSEMBLY_FAR_FP32 = 1, QD_ASSEMBLY_FAR_FP32_FACTOR = 2
 
   type, bind(C) :: qd_solve_report
     integer(c_int) :: precision = 0, info = 0, iterations = 0, converged = 0, gmres = 0, pivoting = 0
@@ -92,6 +93,12 @@ module quadesth_capi
       import :: c_int, c_ptr
       type(c_ptr), value :: h
     end function qd_ghmat8
+
+    integer(c_int) function qd_set_assembly_precision(h, mode) bind(C, name="qd_set_assembly_precision")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: mode
+    end function qd_set_assembly_precision
 
     integer(c_int) function qd_ghmat8_rhs(h, bc, fi) bind(C, name="qd_ghmat8_rhs")
       import :: c_int, c_ptr, c_double
@@ -192,12 +199,16 @@ module quadesth_gpu
   public :: precompute_element_geometry, ghmat8, solve_linear_system, compute_internal_points
   public :: quadesth_gpu_finalize, quadesth_gpu_handle
   public :: qd_solver_precision, qd_somigliana_variant, qd_copy_matrix_to_host, qd_last_report
+  public :: qd_assembly_precision
   public :: QD_FP64, QD_FP32_IR, QD_FP16_IR, QD_SOMIGLIANA_REFERENCE, QD_SOMIGLIANA_CORRECTED
+  public :: QD_ASSEMBLY_FP64, QD_ASSEMBLY_FAR_FP32, QD_ASSEMBLY_FAR_FP32_FACTOR
 
   ! Precision of solve_linear_system (QD_FP64 reproduces DGESV).
   integer(c_int), save :: qd_solver_precision = QD_FP64
   ! Sign convention of the Somigliana displacement (reference by default).
   integer(c_int), save :: qd_somigliana_variant = QD_SOMIGLIANA_REFERENCE
+  ! Assembly precision of ghmat8 (QD_ASSEMBLY_FP64 integrates as the reference).
+  integer(c_int), save :: qd_assembly_precision = QD_ASSEMBLY_FP64
   ! Also fill data_mod's host A in ghmat8 (3n x 3n doubles over PCIe).
   logical, save :: qd_copy_matrix_to_host = .false.
   ! Report of the last solve.
@@ -247,6 +258,7 @@ contains
   subroutine ghmat8()
     type(c_ptr) :: h
     h = quadesth_gpu_handle()
+    call check(qd_set_assembly_precision(h, qd_assembly_precision), 'qd_set_assembly_precision')
     call check(qd_ghmat8(h), 'qd_ghmat8')
     if (qd_copy_matrix_to_host) then
       call system_into(h, 3 * n, FI, A)

@@ -97,6 +97,25 @@ def test_c3_solves(c3):
     c3["X16"], c3["X64"] = X16, X64
 
 
+def test_c3_refinement_sweeps(c3):
+    """How the FP16-IR solution approaches the FP64 solution sweep by sweep:
+    the refinement stops at the first sweep whose backward error is within
+    tol, so tol = 1e-8 ... 1e-14 on the kept factors give the sweep counts and
+    the forward errors against the FP64 solution (printed).  The library's
+    default stops at 1e-14; the forward error it reaches is below 1e-10."""
+    q, FI, X64 = c3["q"], c3["FI"], c3["X64"]
+    prev_sweeps, prev_err = 0, np.inf
+    for tol in (1e-8, 1e-10, 1e-12, 1e-14):
+        X, rep = q.solve_linear_system(FI, precision="fp16", tol=tol)
+        err = rel_err(X, X64)
+        print(f"[c3] tol {tol:.0e}: {rep['iterations']} sweeps, backward {rep['residual']:.2e}, "
+              f"forward vs fp64 {err:.2e}", flush=True)
+        assert rep["converged"] == 1 and rep["residual"] <= tol
+        assert rep["iterations"] >= prev_sweeps and err <= 2.0 * prev_err + 1e-15
+        prev_sweeps, prev_err = rep["iterations"], err
+    assert prev_err < 1e-10
+
+
 def test_c3_internal_points(c3, oracle):
     p, q = c3["p"], c3["q"]
     extra = near_face_points(64)

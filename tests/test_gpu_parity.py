@@ -3,8 +3,10 @@ the reference's compiled Fortran and with the CPU oracle.
 
 Tolerances: the reference path is FP64; the bar is 1e-10 relative for the
 solution, internal u and sigma.  The assembled matrix is compared at 1e-12
-relative to its largest entry (summation order differs from the reference's;
-see DESIGN.md for the ordering)."""
+relative to its largest entry: the library sums the element contributions
+chunk by chunk in colour order, the reference element by element, and the
+kernels are regrouped algebraically, so the last bits differ.  The measured
+errors are printed."""
 import numpy as np
 import pytest
 
@@ -44,13 +46,16 @@ def test_assembly_matches_reference(name):
     q.ghmat8()
     A, FI0 = q.system()
     if "A" in g:
-        assert rel_err(A, g["A"]) < 1e-12
+        ea = rel_err(A, g["A"])
     else:
         rows = g["A_rows"]
-        assert rel_err(A[rows], g["A_sampled"]) < 1e-12
+        ea = rel_err(A[rows], g["A_sampled"])
         assert rel_err(np.abs(A).sum(axis=1), g["A_rowsum_abs"]) < 1e-12
         assert rel_err(A.sum(axis=0), g["A_colsum"]) < 1e-10
-    assert rel_err(FI0, g["FI0"]) < 1e-12
+    ef = rel_err(FI0, g["FI0"])
+    print(f"[parity] {name}: A {ea:.1e} FI {ef:.1e}", flush=True)
+    assert ea < 1e-12
+    assert ef < 1e-12
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -82,10 +87,12 @@ def test_internal_points_match(oracle, name, variant):
         ref = {k: g[k] for k in ("u", "sigma", "vm", "prin")}
     else:
         ref = oracle.internal_points(p, oracle.geometry(p), g["FI"], g["DFI"], variant=1)
-    assert rel_err(r["u"], ref["u"]) < 1e-10
-    assert rel_err(r["sigma"], ref["sigma"]) < 1e-10
-    assert rel_err(r["vm"], ref["vm"]) < 1e-10
-    assert rel_err(r["prin"], ref["prin"]) < 1e-7
+    e = {k: rel_err(r[k], ref[k]) for k in ("u", "sigma", "vm", "prin")}
+    print(f"[parity] {name} variant {variant}: " + " ".join(f"{k} {v:.1e}" for k, v in e.items()), flush=True)
+    assert e["u"] < 1e-10
+    assert e["sigma"] < 1e-10
+    assert e["vm"] < 1e-10
+    assert e["prin"] < 1e-7
     oc = oracle.internal_points(p, oracle.geometry(p), g["FI"], g["DFI"])["counts"]
     assert np.array_equal(r["counts"], oc)
 
