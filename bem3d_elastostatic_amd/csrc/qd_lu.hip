@@ -2951,7 +2951,16 @@ int lu_solve(qd_context* h, int precision, int nrhs, double* b, long long ldb, d
     bool done = false, stalled = false;
     const int maxit = max_iter > 0 ? max_iter : 30;
     const double tl = tol > 0.0 ? tol : 1e-14;
+    // QUADESTH_IR_SWEEPS=k hands the solve to GMRES after k plain sweeps
+    // that have not met the tolerance (the fallback's own trigger is a sweep
+    // that fails to halve the residual)
+    int ir_sweeps = maxit;
+    if (const char* e = std::getenv("QUADESTH_IR_SWEEPS")) ir_sweeps = std::max(1, std::min(maxit, std::atoi(e)));
     while (it < maxit) {
+      if (it == ir_sweeps) {
+        stalled = true;
+        break;
+      }
       rc = apply_inverse(h, trans, nrhs, Rr, D, T64);
       if (rc) return rc;
       hipLaunchKernelGGL(k_axpy_inplace, dim3((unsigned)((nrhs * N + 255) / 256)), dim3(256), 0, h->stream,

@@ -382,15 +382,22 @@ def bench_gradient(args):
         dirs.append((d, dnu, an))
         if i % 8 == 0:
             log(f"fd direction {i}: adjoint {an:.10e} fd {fd:.10e} rel {errs[-1]:.2e}")
-    # The central difference carries J's rounding noise / h.  For the worst
-    # direction, larger steps show whether the gap is that noise (it shrinks
-    # with 1/h) or a gradient error (it does not).
+    # The central difference carries J's rounding noise / h and a truncation
+    # term ~h^2.  For the worst direction, larger steps and Richardson's
+    # combination show which of the two the gap is; a gradient error would
+    # survive both.
     iw = int(np.argmax(errs))
     d, dnu, an = dirs[iw]
-    worst = {"direction": iw, "adjoint": an, "rel_err_h": {}}
+    worst = {"direction": iw, "adjoint": an, "rel_err_h": {}, "richardson_rel_err_h": {}}
+    cd = {}
+    for hh in (1e-6, 1e-5, 2e-5, 1e-4, 2e-4):
+        cd[hh] = central(d, dnu, hh)
     for hh in (1e-6, 1e-5, 1e-4):
-        fd = central(d, dnu, hh)
-        worst["rel_err_h"][f"{hh:g}"] = abs(an - fd) / max(abs(an), abs(fd), 1e-300)
+        worst["rel_err_h"][f"{hh:g}"] = abs(an - cd[hh]) / max(abs(an), abs(cd[hh]), 1e-300)
+    # Richardson: (4 D(h) - D(2h)) / 3 removes the h^2 term of the central difference
+    for hh in (1e-5, 1e-4):
+        r = (4.0 * cd[hh] - cd[2 * hh]) / 3.0
+        worst["richardson_rel_err_h"][f"{hh:g}"] = abs(an - r) / max(abs(an), abs(r), 1e-300)
     log(f"worst direction {iw}: {worst}")
     out = {
         "metric": "adjoint gradient of sum(vm^2) w.r.t. 18438 coordinates + nu: backward/forward time",

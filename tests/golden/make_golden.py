@@ -212,13 +212,52 @@ def run_grad_case(name, ref):
     print(name, "n", p.n, "J", J0, "fd", fd)
 
 
+def run_c2(ref):
+    """Config 2: the 18,438-DOF cube with the eight right-hand sides of
+    mesh.config2_rhs (one kode pattern, so one matrix).  For each, the
+    reference's FI before the solve and its solution, the solves sharing one
+    LU (scipy) of the reference's A, which is checked to be the same matrix
+    every time.  No internal points (the solve is what config 2 checks)."""
+    import dataclasses
+
+    import scipy.linalg as sla
+
+    p = mesh.config1(npoints=0)
+    bcs = mesh.config2_rhs(p)
+    cache = {}
+
+    def solve(A, b):
+        if "lu" not in cache:
+            cache["A"] = np.array(A)
+            cache["lu"] = sla.lu_factor(cache["A"])
+        else:
+            assert np.array_equal(A, cache["A"])
+        return sla.lu_solve(cache["lu"], b)
+
+    FI0, X = [], []
+    for r in range(bcs.shape[0]):
+        q = dataclasses.replace(p, bc=bcs[r])
+        with tempfile.TemporaryDirectory(dir="/tmp") as td:
+            path = os.path.join(td, "deck.dat")
+            deck.write_deck(q, path)
+            out = ref.run(path, solve=solve)
+        FI0.append(out["FI0"])
+        X.append(out["X"])
+        print("c2 right-hand side", r, "done", flush=True)
+    np.savez_compressed(os.path.join(OUT, "c2.npz"), bc=bcs, FI0=np.stack(FI0, axis=1), X=np.stack(X, axis=1),
+                        mu=np.float64(out["mu"]), nu=np.float64(out["nu"]))
+    print("c2 written")
+
+
 def main(argv):
     build_reference()
     ref = Reference()
-    names = argv or (list(CASES) + list(GRAD_CASES) + ["outputs"])
+    names = argv or (list(CASES) + list(GRAD_CASES) + ["outputs", "c2"])
     for name in names:
         if name == "outputs":
             run_outputs(ref)
+        elif name == "c2":
+            run_c2(ref)
         elif name in GRAD_CASES:
             run_grad_case(name, ref)
         else:

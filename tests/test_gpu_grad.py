@@ -148,3 +148,72 @@ def test_config4_gradient():
         print(f"[grad] config4 direction {i}: adjoint {an:.12e} fd {fd:.12e}", flush=True)
         assert _rel(an, fd) < 1e-6, (i, an, fd)
     s.close()
+
+
+def _near_switches(p, xyz, d, h):
+    """Point-element pairs whose near/far rule (stress_mod.f90:584-604: the
+    point within 3.5 radii of the element's centroid) differs between the
+    meshes moved by -h d and +h d."""
+    nc = np.asarray(p.nconec)
+    kc = np.asarray(p.kconec).reshape(p.ne, -1)
+    P = np.asarray(p.points)
+
+    def near(X):
+        c = np.zeros((p.ne, 3))
+        t = np.zeros(p.ne)
+        for e in range(p.ne):
+            nd = X[kc[e, :nc[e]] - 1]
+            c[e] = nd.mean(axis=0)
+            t[e] = 3.5 ** 2 * ((nd - c[e]) ** 2).sum(axis=1).max()
+        return ((P[:, None, :] - c[None]) ** 2).sum(axis=-1) <= t[None]
+
+    return int((near(xyz + h * d) != near(xyz - h * d)).sum())
+
+
+def test_config4_worst_direction_richardson():
+    """The direction of the 64 seeded ones in ``bench.py --config c4`` whose
+    central difference at h = 1e-6 disagrees most with the adjoint (direction
+    13 of default_rng(64): 3.2e-6 relative; |dJ.d| = 15.4, against about
+    2,600 for a typical direction, |grad J| |d| / sqrt(dim)).
+    * Its central difference moves away from the adjoint as h grows (1.5e-4
+      at 1e-5, 1.5e-2 at 1e-4): the h^2 truncation term, large for this
+      direction.  Richardson's combination (4 D(h) - D(2h)) / 3 removes it.
+    * What remains is about 2e-5 at h = 1e-5 and at 1e-4 alike, with
+      opposite signs (+2.1e-5, -1.9e-5), 1e-8 of the typical derivative: J
+      is smooth only between changes of the near/far rule of a point-element
+      pair, and the number of pairs that change between -h d and +h d grows
+      like h (29 at 1e-5, 234 at 1e-4, counted here), so the sum of their
+      jumps over 2h is a bias of the same size at every h, of either sign,
+      which the adjoint (the derivative at fixed rules) does not have.  A
+      gradient error would keep its sign.  The bound is 1e-7 of the typical
+      derivative."""
+    from bem3d_elastostatic_amd import mesh
+    from bem3d_elastostatic_amd.autograd import GradSession, gradient_vm2, value_vm2
+    p = mesh.config4()
+    print("[grad] config4 worst direction", flush=True)
+    s = GradSession(p)
+    xyz = p.coords()
+    J, gc, gnu = gradient_vm2(s, xyz, p.nu)
+    rng = np.random.default_rng(64)
+    for _ in range(14):
+        d = rng.standard_normal((p.n, 3))
+        dnu = 0.1 * rng.standard_normal()
+    an = float(np.sum(gc * d) + gnu * dnu)
+    typical = np.sqrt(np.sum(gc * gc) + gnu * gnu) * np.sqrt(np.sum(d * d) + dnu * dnu) / np.sqrt(gc.size + 1)
+
+    def central(h):
+        return (value_vm2(s, xyz + h * d, p.nu + h * dnu) - value_vm2(s, xyz - h * d, p.nu - h * dnu)) / (2 * h)
+
+    rich, switches = {}, {}
+    for h in (1e-5, 1e-4):
+        d1, d2 = central(h), central(2 * h)
+        rich[h] = (4.0 * d1 - d2) / 3.0
+        switches[h] = _near_switches(p, xyz, d, h)
+        print(f"[grad] direction 13, h {h:g}: adjoint {an:.12e} central {d1:.12e} richardson {rich[h]:.12e} "
+              f"rel {_rel(an, rich[h]):.2e}, of the typical derivative {abs(an - rich[h]) / typical:.2e}; "
+              f"{switches[h]} pairs change rule", flush=True)
+        assert _rel(an, d1) > 1e-5          # the plain difference quotient is off at these steps
+        assert abs(an - rich[h]) < 1e-7 * typical
+    assert abs(an - 15.395) < 0.01
+    assert 0 < switches[1e-5] < switches[1e-4]
+    s.close()
