@@ -1314,6 +1314,279 @@ __global__ __launch_bounds__(256, QD_G16_OCC) void k_gemm16(long long M, long lo
   }
 }
 
+// ---------------------------------------------------------------------------
+// Wide-tile FP16 update for the large launches: the same operation as
+// k_gemm16 (C -= colscale[n/64] * (A16 B16^T), FP32 C row-major), with one
+// 256 x 256 tile per workgroup of 4 waves, one wave per SIMD, each wave
+// 128 x 128 as 4 x 4 blocks of MFMA f32 32x32x16 f16 (256 accumulators in
+// AGPRs; 32x32x16 keeps one wave per SIMD at the full MFMA rate, 16x16x32
+// does not).  K steps of 32 through four LDS stages (32 KB each: A and B as
+// 64-byte rows, 16-byte chunks XOR-swizzled by row as in k_gemm16) filled by
+// global_load_lds three steps ahead; each step reads the next step's
+// fragments while its 32 MFMAs run.  The epilogue stages C through the free
+// stages with global_load_lds as well (64-row quarters, two in flight, rows
+// with their 16-byte units XOR-swizzled so that the two half-waves of a read
+// hit different banks): the loads in flight hold no registers, which gives
+// each CU ~2.4x the C bandwidth of register loads (the CU's epilogue is
+// bound by its bytes in flight, not by HBM, until every CU is in it).
+namespace w16 {
+constexpr int BM = 256, BN = 256, BK = 32, S = 4;
+constexpr int kStage = (BM + BN) * BK * 2;  // 32 KB
+constexpr int kLoads = (BM + BN) / 64;      // global_load_lds per thread per stage
+typedef float f16v __attribute__((ext_vector_type(16)));
+
+struct Smem {
+  unsigned char st[S][kStage];
+};
+static_assert(sizeof(Smem) == 2 * 65536, "the epilogue's two C quarters fill the stages");
+
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+  static_assert(N >= 0 && N < 64, "vmcnt");
+  __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
+}
+__device__ __forceinline__ void wait_lgkm0() { __builtin_amdgcn_s_waitcnt(0xc07f); }
+__device__ __forceinline__ void wait_lgkm15() { __builtin_amdgcn_s_waitcnt(0xcf7f); }
+
+struct Frag {
+  h8 a[2][4], b[2][4];  // [16-wide k half][32-row block]
+};
+
+// swizzled fragment offsets repeat every 32 rows (g16_swz(r + 32) == g16_swz(r))
+__device__ __forceinline__ void read_frags(const unsigned char* st, int offa0, int offa1, int offb0, int offb1,
+                                           Frag& f) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    f.a[0][i] = *reinterpret_cast<const h8*>(st + offa0 + i * 32 * 64);
+    f.a[1][i] = *reinterpret_cast<const h8*>(st + offa1 + i * 32 * 64);
+    f.b[0][i] = *reinterpret_cast<const h8*>(st + offb0 + i * 32 * 64);
+    f.b[1][i] = *reinterpret_cast<const h8*>(st + offb1 + i * 32 * 64);
+  }
+}
+
+__device__ __forceinline__ void mfmas(const Frag& f, f16v (&acc)[4][4]) {
+#pragma unroll
+  for (int s = 0; s < 2; ++s)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.a[s][i], f.b[s][j], acc[i][j], 0, 0, 0);
+}
+
+// this thread's operand rows (16 B of a 64-byte row per instruction, 8
+// instructions per stage): A rows (q*4 + wave)*16 + lane/4 of the tile, then B
+struct Loader {
+  const _Float16* pa[4];
+  const _Float16* pb[4];
+  __device__ __forceinline__ void issue(unsigned char* st, int kb, int wave) const {
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      __builtin_amdgcn_global_load_lds(pa[q] + kb * BK, (lds_ptr_t)(st + (q * 4 + wave) * 1024), 16, 0, 0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      __builtin_amdgcn_global_load_lds(pb[q] + kb * BK, (lds_ptr_t)(st + BM * 64 + (q * 4 + wave) * 1024), 16, 0, 0);
+  }
+};
+
+// One K step on stage slot SL.  kLoad: 1 = the stage S-1 ahead exists, 2 =
+// only if it does (the last steps).
+template <int SL, int kLoad>
+__device__ __forceinline__ void kstep(int kt, int nk, Smem& sm, const Loader& ld, int wave, int offa0, int offa1,
+                                      int offb0, int offb1, Frag& cur, Frag& nxt, f16v (&acc)[4][4]) {
+  // this wave's loads of stage kt+1 have landed once at most the later
+  // stages' (kt+2 .. kt+S-2) are outstanding
+  if (kLoad == 1 || kt + S - 2 <= nk - 1) wait_vm<kLoads * (S - 3)>();
+  else wait_vm<0>();
+  // The slot refilled below was read two steps ago (its fragments fed the
+  // previous step's MFMAs); only this step's fragment reads may be in flight.
+  wait_lgkm15();
+  asm volatile("s_barrier" ::: "memory");
+  if (kLoad == 1 || kt + S - 1 < nk) ld.issue(sm.st[(SL + S - 1) % S], kt + S - 1, wave);
+  // the next step's fragments (past the last step: read from a stale stage, never used)
+  read_frags(sm.st[(SL + 1) % S], offa0, offa1, offb0, offb1, nxt);
+  mfmas(cur, acc);
+  // 16 fragment reads among the 32 MFMAs
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+    __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+  }
+}
+
+template <int kLoad, int... Us>
+__device__ __forceinline__ void steps(std::integer_sequence<int, Us...>, int kt, int nk, Smem& sm, const Loader& ld,
+                                      int wave, int offa0, int offa1, int offb0, int offb1, Frag& f0, Frag& f1,
+                                      f16v (&acc)[4][4]) {
+  auto one = [&](auto uc) {
+    constexpr int u = decltype(uc)::value;
+    if (kLoad == 1 || kt + u < nk) {
+      if constexpr (u % 2 == 0) kstep<u % S, kLoad>(kt + u, nk, sm, ld, wave, offa0, offa1, offb0, offb1, f0, f1, acc);
+      else kstep<u % S, kLoad>(kt + u, nk, sm, ld, wave, offa0, offa1, offb0, offb1, f1, f0, acc);
+    }
+  };
+  (one(std::integral_constant<int, Us>{}), ...);
+}
+
+__device__ __forceinline__ void tile(long long m0, long long n0, long long M, long long N, int Kp,
+                                     const _Float16* __restrict__ A16, const _Float16* __restrict__ B16,
+                                     const float* __restrict__ colscale, float* __restrict__ C, long long ldc,
+                                     bool rows16, Smem& sm) {
+  const int tid = threadIdx.x, lane = tid % 64;
+  const int wave = __builtin_amdgcn_readfirstlane(tid / 64);  // wave-uniform: LDS addresses in SGPRs
+  const int wm = (wave / 2) * 128, wn = (wave % 2) * 128;
+  Loader ld;
+  {
+    // rows past M (N) read row M-1 (N-1): they feed only outputs not stored
+    const int rl = lane >> 2, cs = lane & 3;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int row = (q * 4 + wave) * 16 + rl;
+      const int kc = cs ^ g16_swz(row);
+      ld.pa[q] = A16 + min(m0 + row, M - 1) * Kp + 8 * kc;
+      ld.pb[q] = B16 + min(n0 + row, N - 1) * Kp + 8 * kc;
+    }
+  }
+  // MFMA 32x32x16 operands: lane l supplies row l%32 of a 32-row block, k
+  // chunk 2s + l/32 of the step's four 8-wide chunks
+  const int l32 = lane & 31, lh = lane >> 5;
+  const int ra = wm + l32, rb = wn + l32;
+  const int offa0 = ra * 64 + 16 * ((0 + lh) ^ g16_swz(ra)), offa1 = ra * 64 + 16 * ((2 + lh) ^ g16_swz(ra));
+  const int offb0 = BM * 64 + rb * 64 + 16 * ((0 + lh) ^ g16_swz(rb));
+  const int offb1 = BM * 64 + rb * 64 + 16 * ((2 + lh) ^ g16_swz(rb));
+  f16v acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+  const int nk = Kp / BK;
+  // every wave is done with the stages of the previous tile
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < S - 1; ++s)
+    if (s < nk) ld.issue(sm.st[s], s, wave);
+  if (nk > S - 2) wait_vm<kLoads * (S - 2)>();
+  else wait_vm<0>();
+  asm volatile("s_barrier" ::: "memory");
+  Frag f0, f1;
+  read_frags(sm.st[0], offa0, offa1, offb0, offb1, f0);
+  static_assert(S == 4, "the hot loop's unroll (4 steps) keeps the slots and the fragment parity in step");
+  int kt = 0;
+  for (; kt + 4 + S - 1 <= nk; kt += 4)
+    steps<1>(std::make_integer_sequence<int, 4>{}, kt, nk, sm, ld, wave, offa0, offa1, offb0, offb1, f0, f1, acc);
+  // the rest (kt is a multiple of 4: slot 0, fragments f0): at most 4 + S - 2 steps
+  steps<2>(std::make_integer_sequence<int, 4 + S - 2>{}, kt, nk, sm, ld, wave, offa0, offa1, offb0, offb1, f0, f1,
+           acc);
+  // Epilogue.  acc[i][j][r] is C row wm + 32 i + 8 (r/4) + 4 lh + r%4, column
+  // wn + 32 j + l32 of the tile; a column block of 32 shares its scale.
+  if (m0 + BM <= M && n0 + BN <= N && rows16) {
+    unsigned char* lds = &sm.st[0][0];
+    // quarter q: tile rows 32q .. 32q+31 of each wave row (wm = 0, 128), i.e.
+    // acc block row q of every wave; quarter row lr = (wm/128)*32 + row in block
+    auto issue_q = [&](int q) {
+      unsigned char* region = lds + (q & 1) * 65536;
+#pragma unroll
+      for (int t = 0; t < 16; ++t) {
+        const int lr = wave * 16 + t;
+        const long long grow = m0 + (lr / 32) * 128 + 32 * q + (lr % 32);
+        const int u = lane ^ (8 * ((lr >> 2) & 1));
+        // cache policy 2: non-temporal (C is not read again in the launch)
+        __builtin_amdgcn_global_load_lds(C + grow * ldc + n0 + 4 * u, (lds_ptr_t)(region + lr * 1024), 16, 0, 2);
+      }
+    };
+    float sc[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) sc[j] = colscale[(n0 + wn + 32 * j) >> 6];
+    float* cbase = C + (m0 + wm + 4 * lh) * ldc + n0 + wn + l32;
+    __syncthreads();  // every wave is done with the operand stages
+    issue_q(0);
+    issue_q(1);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      // Quarter q has landed.  At q = 0 the next quarter's 16 loads may be in
+      // flight; later also this wave's stores of the previous quarter (up to
+      // 80, in order, of which 63 can be waited for).
+      if (q == 0) wait_vm<16>();
+      else wait_vm<63>();
+      asm volatile("s_barrier" ::: "memory");
+      const unsigned char* region = lds + (q & 1) * 65536;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int lr = (wm / 128) * 32 + 8 * (r / 4) + 4 * lh + (r % 4);
+        float* p = cbase + (32 * q + 8 * (r / 4) + (r % 4)) * ldc;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int col = wn + 32 * j + l32;
+          const float cv = *reinterpret_cast<const float*>(region + lr * 1024 +
+                                                          16 * ((col >> 2) ^ (8 * ((lr >> 2) & 1))) + 4 * (col & 3));
+          __builtin_nontemporal_store(cv - sc[j] * acc[q][j][r], p + 32 * j);
+        }
+      }
+      if (q + 2 < 4) {
+        // every wave has read quarter q: its region takes quarter q + 2
+        wait_lgkm0();
+        asm volatile("s_barrier" ::: "memory");
+        issue_q(q + 2);
+      }
+    }
+    return;
+  }
+  // edge tiles (and C rows that are not 16-byte aligned): predicated
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    float cv[4][16];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const long long col = n0 + wn + 32 * j + l32;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const long long row = m0 + wm + 32 * i + 8 * (r / 4) + 4 * lh + (r % 4);
+        cv[j][r] = (row < M && col < N) ? C[row * ldc + col] : 0.0f;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const long long col = n0 + wn + 32 * j + l32;
+      const long long cb = n0 + wn + 32 * j;
+      const float s = (cb < N) ? colscale[cb >> 6] : 0.0f;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const long long row = m0 + wm + 32 * i + 8 * (r / 4) + 4 * lh + (r % 4);
+        if (row < M && col < N) C[row * ldc + col] = cv[j][r] - s * acc[i][j][r];
+      }
+    }
+  }
+}
+}  // namespace w16
+
+// One 256 x 256 tile per workgroup; the tiles go to the XCDs in 8 contiguous
+// bands and, within a band, 4 row bands sweep the column tiles together (as
+// in k_gemm16).
+template <int kRole>
+__global__ __launch_bounds__(256, 1) void k_gemm16w(long long M, long long N, int Kp, const _Float16* __restrict__ A16,
+                                                     const _Float16* __restrict__ B16,
+                                                     const float* __restrict__ colscale, float* __restrict__ C,
+                                                     long long ldc, int rows16) {
+  __shared__ __attribute__((aligned(16))) w16::Smem sm;
+  const unsigned tiles_n = (unsigned)((N + w16::BN - 1) / w16::BN);
+  const unsigned tiles_m = (unsigned)((M + w16::BM - 1) / w16::BM);
+  const unsigned total = tiles_m * tiles_n;
+  const unsigned per_xcd = (total + 7) / 8;
+  const unsigned xcd = blockIdx.x % 8, lw = blockIdx.x / 8, wx = (gridDim.x + 7 - xcd) / 8;
+  for (unsigned i = lw; i < per_xcd; i += wx) {
+    const unsigned vid = xcd * per_xcd + i;
+    if (vid >= total) break;
+    const unsigned grp = vid / (kG16Group * tiles_n);
+    const unsigned first_m = grp * kG16Group;
+    const unsigned gsz = min(tiles_m - first_m, (unsigned)kG16Group);
+    const unsigned in_grp = vid - grp * kG16Group * tiles_n;
+    const unsigned tm = first_m + in_grp % gsz, tn = in_grp / gsz;
+    w16::tile((long long)tm * w16::BM, (long long)tn * w16::BN, M, N, Kp, A16, B16, colscale, C, ldc, rows16 != 0, sm);
+  }
+}
+
 }  // namespace qd
 
 #include "qd_tslu.h"
@@ -1340,9 +1613,34 @@ struct LuCtx {
 
 // The FP16 update C -= (A16 B16^T) * colscale on stream st; max_wgs > 0
 // caps the workgroups (each then loops over several tiles).
+// Launches with at least this many 256 x 256 tiles take the wide-tile kernel
+// (QUADESTH_GEMM16W_MIN; 0 turns it off).  Measured on the 76,806-DOF
+// factorisation's update shapes: 7% faster than k_gemm16 on the square
+// updates, 2% slower on the 2048-column ones (9 waves of tiles), slower on
+// small launches.
+static long long wide_gemm_min_tiles() {
+  static const long long v = [] {
+    const char* e = std::getenv("QUADESTH_GEMM16W_MIN");
+    return e ? std::atoll(e) : 1024LL;
+  }();
+  return v;
+}
+
 static int launch_f16_update(qd_context* h, bool trailing, long long M, long long N, int Kp, const _Float16* A16,
                              const _Float16* B16, const float* colscale, float* C, long long ldc, int rows16,
                              int max_wgs, hipStream_t st) {
+  const long long wmin = wide_gemm_min_tiles();
+  const long long wtiles = ((N + w16::BN - 1) / w16::BN) * ((M + w16::BM - 1) / w16::BM);
+  if (wmin > 0 && wtiles >= wmin && max_wgs <= 0) {
+    if (trailing)
+      hipLaunchKernelGGL(k_gemm16w<1>, dim3((unsigned)wtiles), dim3(256), 0, st, M, N, Kp, A16, B16, colscale, C, ldc,
+                         rows16);
+    else
+      hipLaunchKernelGGL(k_gemm16w<0>, dim3((unsigned)wtiles), dim3(256), 0, st, M, N, Kp, A16, B16, colscale, C, ldc,
+                         rows16);
+    QD_HIP(h, hipGetLastError());
+    return QD_OK;
+  }
   const long long tiles = ((N + H_BN - 1) / H_BN) * ((M + H_BM - 1) / H_BM);
   const dim3 g((unsigned)(max_wgs > 0 ? std::min<long long>(tiles, max_wgs) : tiles));
   if (trailing)
