@@ -1320,25 +1320,33 @@ __global__ __launch_bounds__(256, QD_G16_OCC) void k_gemm16(long long M, long lo
 // 256 x 256 tile per workgroup of 4 waves, one wave per SIMD, each wave
 // 128 x 128 as 4 x 4 blocks of MFMA f32 32x32x16 f16 (256 accumulators in
 // AGPRs; 32x32x16 keeps one wave per SIMD at the full MFMA rate, 16x16x32
-// does not).  K steps of 32 through four LDS stages (32 KB each: A and B as
-// 64-byte rows, 16-byte chunks XOR-swizzled by row as in k_gemm16) filled by
-// global_load_lds three steps ahead; each step reads the next step's
-// fragments while its 32 MFMAs run.  The epilogue stages C through the free
-// stages with global_load_lds as well (64-row quarters, two in flight, rows
-// with their 16-byte units XOR-swizzled so that the two half-waves of a read
-// hit different banks): the loads in flight hold no registers, which gives
-// each CU ~2.4x the C bandwidth of register loads (the CU's epilogue is
-// bound by its bytes in flight, not by HBM, until every CU is in it).
+// does not).  K steps of 64 through two LDS stages of 64 KB, A and B as
+// 128-byte rows, filled by global_load_lds: each instruction moves 8 whole
+// rows (64-byte row pieces, as 32-wide stages give, move at half the rate
+// into LDS).  The 16-byte chunks of a row are XOR-swizzled by row so that the
+// fragment reads of 8 consecutive rows hit distinct banks.  Fragments are
+// read one 32-wide half step ahead of their MFMAs.  The epilogue stages C
+// through the free stages with global_load_lds as well (64-row quarters, two
+// in flight, rows with their 16-byte units XOR-swizzled so that the two
+// half-waves of a read hit different banks): the loads in flight hold no
+// registers, which gives each CU ~2.4x the C bandwidth of register loads (a
+// CU's epilogue is bound by its bytes in flight, not by HBM, until every CU
+// is in it).  Kp must be a multiple of 64.
 namespace w16 {
-constexpr int BM = 256, BN = 256, BK = 32, S = 4;
-constexpr int kStage = (BM + BN) * BK * 2;  // 32 KB
-constexpr int kLoads = (BM + BN) / 64;      // global_load_lds per thread per stage
+constexpr int BM = 256, BN = 256, BK = 64;
+constexpr int kStage = (BM + BN) * BK * 2;  // 64 KB
+constexpr int kLoads = (BM + BN) / 32;      // global_load_lds per thread per stage (8 rows each)
 typedef float f16v __attribute__((ext_vector_type(16)));
 
 struct Smem {
-  unsigned char st[S][kStage];
+  unsigned char st[2][kStage];
 };
 static_assert(sizeof(Smem) == 2 * 65536, "the epilogue's two C quarters fill the stages");
+
+// chunk c (0..7) of row r sits in slot c ^ swz(r): distinct for 8 consecutive
+// rows and for each parity class of 16; period 32 rows, so the four 32-row
+// blocks of a wave share their fragment offsets
+__device__ __forceinline__ int swz(int row) { return (row & 7) ^ ((row >> 3) & 3); }
 
 template <int N>
 __device__ __forceinline__ void wait_vm() {
@@ -1346,21 +1354,21 @@ __device__ __forceinline__ void wait_vm() {
   __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
 }
 __device__ __forceinline__ void wait_lgkm0() { __builtin_amdgcn_s_waitcnt(0xc07f); }
-__device__ __forceinline__ void wait_lgkm15() { __builtin_amdgcn_s_waitcnt(0xcf7f); }
 
 struct Frag {
-  h8 a[2][4], b[2][4];  // [16-wide k half][32-row block]
+  h8 a[2][4], b[2][4];  // [16-wide k substep of the half step][32-row block]
 };
 
-// swizzled fragment offsets repeat every 32 rows (g16_swz(r + 32) == g16_swz(r))
-__device__ __forceinline__ void read_frags(const unsigned char* st, int offa0, int offa1, int offb0, int offb1,
+// fragments of half H (k 32H .. 32H+31) of a stage: substeps 2H, 2H+1
+template <int H>
+__device__ __forceinline__ void read_frags(const unsigned char* st, const int (&offa)[4], const int (&offb)[4],
                                            Frag& f) {
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    f.a[0][i] = *reinterpret_cast<const h8*>(st + offa0 + i * 32 * 64);
-    f.a[1][i] = *reinterpret_cast<const h8*>(st + offa1 + i * 32 * 64);
-    f.b[0][i] = *reinterpret_cast<const h8*>(st + offb0 + i * 32 * 64);
-    f.b[1][i] = *reinterpret_cast<const h8*>(st + offb1 + i * 32 * 64);
+    f.a[0][i] = *reinterpret_cast<const h8*>(st + offa[2 * H] + i * 32 * 128);
+    f.a[1][i] = *reinterpret_cast<const h8*>(st + offa[2 * H + 1] + i * 32 * 128);
+    f.b[0][i] = *reinterpret_cast<const h8*>(st + offb[2 * H] + i * 32 * 128);
+    f.b[1][i] = *reinterpret_cast<const h8*>(st + offb[2 * H + 1] + i * 32 * 128);
   }
 }
 
@@ -1374,39 +1382,8 @@ __device__ __forceinline__ void mfmas(const Frag& f, f16v (&acc)[4][4]) {
         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.a[s][i], f.b[s][j], acc[i][j], 0, 0, 0);
 }
 
-// this thread's operand rows (16 B of a 64-byte row per instruction, 8
-// instructions per stage): A rows (q*4 + wave)*16 + lane/4 of the tile, then B
-struct Loader {
-  const _Float16* pa[4];
-  const _Float16* pb[4];
-  __device__ __forceinline__ void issue(unsigned char* st, int kb, int wave) const {
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-      __builtin_amdgcn_global_load_lds(pa[q] + kb * BK, (lds_ptr_t)(st + (q * 4 + wave) * 1024), 16, 0, 0);
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-      __builtin_amdgcn_global_load_lds(pb[q] + kb * BK, (lds_ptr_t)(st + BM * 64 + (q * 4 + wave) * 1024), 16, 0, 0);
-  }
-};
-
-// One K step on stage slot SL.  kLoad: 1 = the stage S-1 ahead exists, 2 =
-// only if it does (the last steps).
-template <int SL, int kLoad>
-__device__ __forceinline__ void kstep(int kt, int nk, Smem& sm, const Loader& ld, int wave, int offa0, int offa1,
-                                      int offb0, int offb1, Frag& cur, Frag& nxt, f16v (&acc)[4][4]) {
-  // this wave's loads of stage kt+1 have landed once at most the later
-  // stages' (kt+2 .. kt+S-2) are outstanding
-  if (kLoad == 1 || kt + S - 2 <= nk - 1) wait_vm<kLoads * (S - 3)>();
-  else wait_vm<0>();
-  // The slot refilled below was read two steps ago (its fragments fed the
-  // previous step's MFMAs); only this step's fragment reads may be in flight.
-  wait_lgkm15();
-  asm volatile("s_barrier" ::: "memory");
-  if (kLoad == 1 || kt + S - 1 < nk) ld.issue(sm.st[(SL + S - 1) % S], kt + S - 1, wave);
-  // the next step's fragments (past the last step: read from a stale stage, never used)
-  read_frags(sm.st[(SL + 1) % S], offa0, offa1, offb0, offb1, nxt);
-  mfmas(cur, acc);
-  // 16 fragment reads among the 32 MFMAs
+// 16 fragment reads among the 32 MFMAs of a half step
+__device__ __forceinline__ void interleave16() {
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
     __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
@@ -1414,15 +1391,55 @@ __device__ __forceinline__ void kstep(int kt, int nk, Smem& sm, const Loader& ld
   }
 }
 
+// Lane l of instruction q covers row (q*4 + wave)*8 + l/8 of A (and of B),
+// 16-byte slot l%8; its offsets are 32-bit element offsets from A16 / B16
+// (rows past M or N read row M-1 or N-1: they feed only outputs not stored).
+struct Loader {
+  const _Float16* A16;
+  const _Float16* B16;
+  unsigned oa[8], ob[8];
+  __device__ __forceinline__ void issue(unsigned char* st, int kb, int wave) const {
+    const _Float16* a = A16 + kb * BK;
+    const _Float16* b = B16 + kb * BK;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      __builtin_amdgcn_global_load_lds(a + oa[q], (lds_ptr_t)(st + (q * 4 + wave) * 1024), 16, 0, 0);
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      __builtin_amdgcn_global_load_lds(b + ob[q], (lds_ptr_t)(st + ((q + 8) * 4 + wave) * 1024), 16, 0, 0);
+  }
+};
+
+// Half step (stage k, half H) on the stage in slot SL.  At H = 1 stage k+1
+// must have landed (its first half is read from now on) and stage k's slot,
+// whose fragments are all in registers, takes stage k+2 (kLoad 1: it exists,
+// 2: only if it does).
+template <int SL, int H, int kLoad>
+__device__ __forceinline__ void hstep(int k, int nk, Smem& sm, const Loader& ld, int wave, const int (&offa)[4],
+                                      const int (&offb)[4], Frag& cur, Frag& nxt, f16v (&acc)[4][4]) {
+  if constexpr (H == 1) {
+    wait_vm<0>();  // stage k+1's loads were the only ones in flight
+    wait_lgkm0();
+    asm volatile("s_barrier" ::: "memory");
+    if (kLoad == 1 || k + 2 < nk) ld.issue(sm.st[SL], k + 2, wave);
+    read_frags<0>(sm.st[SL ^ 1], offa, offb, nxt);  // past the last stage: stale, never used
+  } else {
+    read_frags<1>(sm.st[SL], offa, offb, nxt);
+  }
+  mfmas(cur, acc);
+  interleave16();
+}
+
 template <int kLoad, int... Us>
-__device__ __forceinline__ void steps(std::integer_sequence<int, Us...>, int kt, int nk, Smem& sm, const Loader& ld,
-                                      int wave, int offa0, int offa1, int offb0, int offb1, Frag& f0, Frag& f1,
-                                      f16v (&acc)[4][4]) {
+__device__ __forceinline__ void hsteps(std::integer_sequence<int, Us...>, int hs0, int nk, Smem& sm, const Loader& ld,
+                                       int wave, const int (&offa)[4], const int (&offb)[4], Frag& f0, Frag& f1,
+                                       f16v (&acc)[4][4]) {
   auto one = [&](auto uc) {
     constexpr int u = decltype(uc)::value;
-    if (kLoad == 1 || kt + u < nk) {
-      if constexpr (u % 2 == 0) kstep<u % S, kLoad>(kt + u, nk, sm, ld, wave, offa0, offa1, offb0, offb1, f0, f1, acc);
-      else kstep<u % S, kLoad>(kt + u, nk, sm, ld, wave, offa0, offa1, offb0, offb1, f1, f0, acc);
+    const int hs = hs0 + u;
+    if (kLoad == 1 || hs < 2 * nk) {
+      if constexpr (u % 2 == 0) hstep<(u / 2) % 2, u % 2, kLoad>(hs / 2, nk, sm, ld, wave, offa, offb, f0, f1, acc);
+      else hstep<(u / 2) % 2, u % 2, kLoad>(hs / 2, nk, sm, ld, wave, offa, offb, f1, f0, acc);
     }
   };
   (one(std::integral_constant<int, Us>{}), ...);
@@ -1436,24 +1453,28 @@ __device__ __forceinline__ void tile(long long m0, long long n0, long long M, lo
   const int wave = __builtin_amdgcn_readfirstlane(tid / 64);  // wave-uniform: LDS addresses in SGPRs
   const int wm = (wave / 2) * 128, wn = (wave % 2) * 128;
   Loader ld;
+  ld.A16 = A16;
+  ld.B16 = B16;
   {
-    // rows past M (N) read row M-1 (N-1): they feed only outputs not stored
-    const int rl = lane >> 2, cs = lane & 3;
+    const int rl = lane >> 3, cs = lane & 7;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int row = (q * 4 + wave) * 16 + rl;
-      const int kc = cs ^ g16_swz(row);
-      ld.pa[q] = A16 + min(m0 + row, M - 1) * Kp + 8 * kc;
-      ld.pb[q] = B16 + min(n0 + row, N - 1) * Kp + 8 * kc;
+    for (int q = 0; q < 8; ++q) {
+      const int row = (q * 4 + wave) * 8 + rl;
+      const int kc = cs ^ swz(row);
+      ld.oa[q] = (unsigned)(min(m0 + row, M - 1) * Kp + 8 * kc);
+      ld.ob[q] = (unsigned)(min(n0 + row, N - 1) * Kp + 8 * kc);
     }
   }
   // MFMA 32x32x16 operands: lane l supplies row l%32 of a 32-row block, k
-  // chunk 2s + l/32 of the step's four 8-wide chunks
+  // chunk 2s + l/32 of the stage's eight 8-wide chunks at substep s
   const int l32 = lane & 31, lh = lane >> 5;
   const int ra = wm + l32, rb = wn + l32;
-  const int offa0 = ra * 64 + 16 * ((0 + lh) ^ g16_swz(ra)), offa1 = ra * 64 + 16 * ((2 + lh) ^ g16_swz(ra));
-  const int offb0 = BM * 64 + rb * 64 + 16 * ((0 + lh) ^ g16_swz(rb));
-  const int offb1 = BM * 64 + rb * 64 + 16 * ((2 + lh) ^ g16_swz(rb));
+  int offa[4], offb[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    offa[s] = ra * 128 + 16 * ((2 * s + lh) ^ swz(ra));
+    offb[s] = BM * 128 + rb * 128 + 16 * ((2 * s + lh) ^ swz(rb));
+  }
   f16v acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -1462,23 +1483,24 @@ __device__ __forceinline__ void tile(long long m0, long long n0, long long M, lo
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
   const int nk = Kp / BK;
-  // every wave is done with the stages of the previous tile
-  __syncthreads();
-#pragma unroll
-  for (int s = 0; s < S - 1; ++s)
-    if (s < nk) ld.issue(sm.st[s], s, wave);
-  if (nk > S - 2) wait_vm<kLoads * (S - 2)>();
-  else wait_vm<0>();
+  __syncthreads();  // every wave is done with the previous tile's stages
+  ld.issue(sm.st[0], 0, wave);
+  if (nk > 1) {
+    ld.issue(sm.st[1], 1, wave);
+    wait_vm<kLoads>();
+  } else {
+    wait_vm<0>();
+  }
   asm volatile("s_barrier" ::: "memory");
   Frag f0, f1;
-  read_frags(sm.st[0], offa0, offa1, offb0, offb1, f0);
-  static_assert(S == 4, "the hot loop's unroll (4 steps) keeps the slots and the fragment parity in step");
-  int kt = 0;
-  for (; kt + 4 + S - 1 <= nk; kt += 4)
-    steps<1>(std::make_integer_sequence<int, 4>{}, kt, nk, sm, ld, wave, offa0, offa1, offb0, offb1, f0, f1, acc);
-  // the rest (kt is a multiple of 4: slot 0, fragments f0): at most 4 + S - 2 steps
-  steps<2>(std::make_integer_sequence<int, 4 + S - 2>{}, kt, nk, sm, ld, wave, offa0, offa1, offb0, offb1, f0, f1,
-           acc);
+  read_frags<0>(sm.st[0], offa, offb, f0);
+  // stage k sits in slot k & 1; the hot loop's 4 half steps cover stages k,
+  // k+1 (slots 0, 1) and load stages k+2, k+3
+  int hs = 0;
+  for (; hs / 2 + 3 < nk; hs += 4)
+    hsteps<1>(std::make_integer_sequence<int, 4>{}, hs, nk, sm, ld, wave, offa, offb, f0, f1, acc);
+  // the rest: at most 3 stages (hs is a multiple of 4: slot 0, fragments f0)
+  hsteps<2>(std::make_integer_sequence<int, 6>{}, hs, nk, sm, ld, wave, offa, offb, f0, f1, acc);
   // Epilogue.  acc[i][j][r] is C row wm + 32 i + 8 (r/4) + 4 lh + r%4, column
   // wn + 32 j + l32 of the tile; a column block of 32 shares its scale.
   if (m0 + BM <= M && n0 + BN <= N && rows16) {
@@ -1613,6 +1635,10 @@ struct LuCtx {
 
 // The FP16 update C -= (A16 B16^T) * colscale on stream st; max_wgs > 0
 // caps the workgroups (each then loops over several tiles).
+// K of the FP16 operand copies, zero-padded to a multiple of 64 (the wide-tile
+// kernel's K step; k_gemm16 takes any multiple of 32).
+static inline int f16_kpad(int K) { return (K + 63) / 64 * 64; }
+
 // Launches with at least this many 256 x 256 tiles take the wide-tile kernel
 // (QUADESTH_GEMM16W_MIN; 0 turns it off).  Measured on the 76,806-DOF
 // factorisation's update shapes: 7% faster than k_gemm16 on the square
@@ -1631,7 +1657,9 @@ static int launch_f16_update(qd_context* h, bool trailing, long long M, long lon
                              int max_wgs, hipStream_t st) {
   const long long wmin = wide_gemm_min_tiles();
   const long long wtiles = ((N + w16::BN - 1) / w16::BN) * ((M + w16::BM - 1) / w16::BM);
-  if (wmin > 0 && wtiles >= wmin && max_wgs <= 0) {
+  // the wide kernel takes K steps of 64 and 32-bit element offsets into A16 / B16
+  if (wmin > 0 && wtiles >= wmin && max_wgs <= 0 && Kp % w16::BK == 0 &&
+      std::max(M, N) * (long long)Kp < (1LL << 32)) {
     if (trailing)
       hipLaunchKernelGGL(k_gemm16w<1>, dim3((unsigned)wtiles), dim3(256), 0, st, M, N, Kp, A16, B16, colscale, C, ldc,
                          rows16);
@@ -1696,7 +1724,7 @@ static int gemm_sub(LuCtx<T>& c, long long M, long long Nn, int K, const T* A, c
     // the FP32 MFMA kernel on W directly is faster.
     if (c.f16 && allow_f16 && (trailing || (double)M * (double)Nn * (double)K >= f16_min_work())) {
       const long long nb = (Nn + 63) / 64;
-      const int Kp = (K + 31) / 32 * 32;
+      const int Kp = f16_kpad(K);
       QD_ALLOC(c.h, c.sc->colscale, (size_t)nb);
       QD_ALLOC(c.h, c.sc->h16a, (size_t)M * Kp);
       QD_ALLOC(c.h, c.sc->h16b, (size_t)Nn * Kp);
@@ -1897,7 +1925,7 @@ template <typename T>
 static int update_split(LuCtx<T>& c, long long j, int jb, long long c0, long long c1, long long split,
                         hipEvent_t ev) {
   const long long r0 = j + jb, M = c.N - r0, Nn = c1 - c0;
-  const int K = jb, Kp = (K + 31) / 32 * 32;
+  const int K = jb, Kp = f16_kpad(K);
   const float* B = reinterpret_cast<const float*>(c.W + j * c.ld + c0);
   const long long nb = (Nn + 63) / 64;
   QD_ALLOC(c.h, c.sc->colscale, (size_t)nb);
@@ -1999,7 +2027,7 @@ static int factor(LuCtx<T>& c, const PanelWidths& pw) {
   const bool look = !(la && la[0] == '0');
   if (c.f16) {
     // sized up front, so that nothing is reallocated while two streams run
-    const size_t Kp = (size_t)((NB + 31) / 32 * 32), Kh = (size_t)((NB / 2 + 31) / 32 * 32);
+    const size_t Kp = (size_t)f16_kpad(NB), Kh = (size_t)f16_kpad(NB / 2);
     QD_ALLOC(c.h, s->sc_main.h16a, (size_t)N * Kp);
     QD_ALLOC(c.h, s->sc_main.h16b, (size_t)N * Kp);
     QD_ALLOC(c.h, s->sc_main.colscale, (size_t)(N / 64 + 2));
