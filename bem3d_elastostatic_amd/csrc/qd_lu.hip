@@ -1841,11 +1841,19 @@ static int tslu_leaf(LuCtx<float>& c, long long c0, int w, long long p0, long lo
   if (c.diag) {
     // diagonal pivots (leaves of up to kDW columns): no interchanges
     if (w > kDW) return set_error(c.h, QD_ERR_STATE, "diagonal-pivot leaf of %d columns", w);
-    hipLaunchKernelGGL(k_leaf_diag, dim3(1), dim3(64), 0, c.st, (const float*)c.W, c.ld, c.N, c0, w, s->ts_lu.p,
-                       s->ts_linv.p + (c0 / kTsW) * kTsW * kTsW, s->ipiv_dev.p, s->growth.p);
     const long long below = std::max<long long>(m - w, 1);
-    hipLaunchKernelGGL(k_finish_diag, dim3((unsigned)((below + 255) / 256)), dim3(256), 0, c.st, c.W, c.ld, c.N, c0,
-                       w, (const float*)s->ts_lu.p, s->growth.p);
+    float* linv = s->ts_linv.p + (c0 / kTsW) * kTsW * kTsW;
+    if (w == kDW && m >= kDW) {
+      hipLaunchKernelGGL(k_leaf_diag64, dim3(1), dim3(64), 0, c.st, (const float*)c.W, c.ld, c0, s->ts_lu.p, linv,
+                         s->ipiv_dev.p, s->growth.p);
+      hipLaunchKernelGGL(k_finish_diag64, dim3((unsigned)((below + 255) / 256)), dim3(256), 0, c.st, c.W, c.ld, c.N,
+                         c0, (const float*)s->ts_lu.p, s->growth.p);
+    } else {
+      hipLaunchKernelGGL(k_leaf_diag, dim3(1), dim3(64), 0, c.st, (const float*)c.W, c.ld, c.N, c0, w, s->ts_lu.p, linv,
+                         s->ipiv_dev.p, s->growth.p);
+      hipLaunchKernelGGL(k_finish_diag, dim3((unsigned)((below + 255) / 256)), dim3(256), 0, c.st, c.W, c.ld, c.N,
+                         c0, w, (const float*)s->ts_lu.p, s->growth.p);
+    }
     QD_HIP(c.h, hipGetLastError());
     return QD_OK;
   }

@@ -382,6 +382,17 @@ __device__ __forceinline__ void ts_growth_max(float v, unsigned* growth) {
   if ((threadIdx.x & 63) == 0) atomicMax(growth, b);
 }
 
+// The same for a whole 256-thread workgroup with one atomic: every thread of
+// the workgroup must call it (it holds a barrier).  Atomics of every wave on
+// the one word serialise at its L2 channel (a 70,000-row finish spent ~8 us
+// on them).
+__device__ __forceinline__ void ts_growth_max_wg(float v, unsigned* growth, unsigned (&wmax)[4]) {
+  const unsigned b = wave_max_u32(__float_as_uint(fabsf(v)));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = b;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicMax(growth, max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3])));
+}
+
 // L21 = A21 U11^-1 (one row per thread, U11 read from LDS by broadcast),
 // against the root's winners' LU, which workgroup 0 also writes into the top
 // rows; then, in workgroups row_blocks.., the permutation on the panel's
@@ -703,7 +714,115 @@ __global__ __launch_bounds__(256) void k_finish_diag(float* __restrict__ W, long
         if (j < w) dst[j] = x[j];
     }
   }
-  ts_growth_max(g, growth);
+  __shared__ unsigned wmax[4];
+  ts_growth_max_wg(g, growth, wmax);
+}
+
+// The kernels above for a leaf of exactly kDW columns (every leaf but the
+// last of the matrix), where no column or row test is left at run time.
+// Without the per-column width masks the 63 pivot-row broadcasts of a step
+// go to distinct scalar registers and pipeline (the generic leaf funnels them
+// through one, 25.5 -> 15.4 us per leaf alone), and the finish reads U11's
+// rows as wave-uniform values (scalar loads) instead of through LDS.  Results
+// are bit-identical to the generic kernels'.
+template <int K>
+__device__ __forceinline__ void dw_step64(float (&a)[kDW], float& gmax) {
+  const int lane = threadIdx.x;
+  const float piv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a[K]), K));
+  if (piv == 0.0f) gmax = abs_max_keep_nan(gmax, __int_as_float(0x7f800000));
+  const float rp = (piv != 0.0f) ? __builtin_amdgcn_rcpf(piv) : 0.0f;
+  const bool act = lane > K;
+  const float l = a[K] * rp;
+  if (act) {
+    a[K] = l;
+    gmax = abs_max_keep_nan(gmax, l);
+  }
+  float u[kDW];
+#pragma unroll
+  for (int j = K + 1; j < kDW; ++j) u[j] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a[j]), K));
+#pragma unroll
+  for (int j = K + 1; j < kDW; ++j)
+    if (act) a[j] -= l * u[j];
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+template <int... Ks>
+__device__ __forceinline__ void dw_all64(std::integer_sequence<int, Ks...>, float (&a)[kDW], float& gmax) {
+  (dw_step64<Ks>(a, gmax), ...);
+}
+
+__global__ __launch_bounds__(64) void k_leaf_diag64(const float* __restrict__ W, long long ld, long long c0,
+                                                    float* __restrict__ lu_out, float* __restrict__ linv_out,
+                                                    int* __restrict__ ipiv, unsigned* __restrict__ growth) {
+  __shared__ float sL[kDW][kDW + 1];
+  __builtin_amdgcn_s_setprio(3);
+  const int lane = threadIdx.x;
+  float a[kDW];
+  dw_load_row(W, ld, c0 + lane, c0, kDW, a);
+  float gmax = 0.0f;
+  dw_all64(std::make_integer_sequence<int, kDW>{}, a, gmax);
+#pragma unroll
+  for (int j4 = 0; j4 < kDW / 4; ++j4)
+    *reinterpret_cast<float4*>(lu_out + lane * kDW + 4 * j4) =
+        make_float4(a[4 * j4], a[4 * j4 + 1], a[4 * j4 + 2], a[4 * j4 + 3]);
+  ts_growth_max(gmax, growth);
+  ipiv[c0 + lane] = (int)c0 + lane;
+#pragma unroll
+  for (int j = 0; j < kDW; ++j) sL[lane][j] = (j < lane) ? a[j] : 0.0f;
+  __builtin_amdgcn_s_waitcnt(0xc07f);
+  __builtin_amdgcn_wave_barrier();
+  const int blk = lane >> 5, cc = lane & 31, o = 32 * blk;
+  float y[32];
+#pragma unroll
+  for (int r = 0; r < 32; ++r) {
+    float acc = (r == cc) ? 1.0f : 0.0f;
+#pragma unroll
+    for (int k = 0; k < r; ++k) acc -= sL[o + r][o + k] * y[k];
+    y[r] = acc;
+  }
+#pragma unroll
+  for (int r = 0; r < 32; ++r) linv_out[blk * 1024 + r * 32 + cc] = y[r];
+}
+
+template <int K>
+__device__ __forceinline__ void dws_step64(float (&x)[kDW], const float* __restrict__ lu) {
+  const float d = lu[K * kDW + K];
+  x[K] *= (d != 0.0f) ? 1.0f / d : 0.0f;
+#pragma unroll
+  for (int j = K + 1; j < kDW; ++j) x[j] -= x[K] * lu[K * kDW + j];
+}
+
+template <int... Ks>
+__device__ __forceinline__ void dws_all64(std::integer_sequence<int, Ks...>, float (&x)[kDW],
+                                          const float* __restrict__ lu) {
+  (dws_step64<Ks>(x, lu), ...);
+}
+
+__global__ __launch_bounds__(256) void k_finish_diag64(float* __restrict__ W, long long ld, long long N, long long c0,
+                                                       const float* __restrict__ lu_in, unsigned* __restrict__ growth) {
+  __shared__ unsigned wmax[4];
+  __builtin_amdgcn_s_setprio(3);
+  const int tid = threadIdx.x;
+  const long long r = c0 + kDW + (long long)blockIdx.x * 256 + tid;
+  float x[kDW];
+  if (r < N) dw_load_row(W, ld, r, c0, kDW, x);
+  if (blockIdx.x == 0)
+#pragma unroll
+    for (int q = 0; q < kDW * kDW / 256; ++q) {
+      const int i = tid + 256 * q;
+      W[(c0 + i / kDW) * ld + c0 + i % kDW] = lu_in[i];
+    }
+  float g = 0.0f;
+  if (r < N) {
+    dws_all64(std::make_integer_sequence<int, kDW>{}, x, lu_in);
+#pragma unroll
+    for (int j = 0; j < kDW; ++j) g = abs_max_keep_nan(g, x[j]);
+    float* dst = W + r * ld + c0;
+#pragma unroll
+    for (int j4 = 0; j4 < kDW / 4; ++j4)
+      *reinterpret_cast<float4*>(dst + 4 * j4) = make_float4(x[4 * j4], x[4 * j4 + 1], x[4 * j4 + 2], x[4 * j4 + 3]);
+  }
+  ts_growth_max_wg(g, growth, wmax);
 }
 
 }  // namespace qd
