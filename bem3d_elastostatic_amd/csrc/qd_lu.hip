@@ -2974,13 +2974,14 @@ static int ensure_factors(qd_context* h, int precision, qd_solve_report* rep) {
   int rc;
   // Panel width: the FP16 trailing update streams its FP32 C once per panel,
   // so it takes a wider panel than the MFMA f64/f32 updates need.  With the
-  // look-ahead, measured on MI355X at 76,806 DOF: 2048 columns while more
-  // than 30,000 remain (the update hides the panel there), 1024 after; a
-  // factor 8% shorter than 1024 throughout, 1% shorter than 2048 throughout.
+  // look-ahead, measured on MI355X at 76,806 DOF: 4096 columns while more
+  // than 30,000 remain (the update hides the panel there), 1024 after.  With
+  // the wide-tile update, 4096 gave a factor 3-5 ms shorter than 2048 on two
+  // boxes (3072: 8 ms longer); 2048 had been 8% shorter than 1024 throughout.
   PanelWidths pw;
   pw.nb = precision == QD_FP16_IR ? 1024 : 512;
   if (precision == QD_FP16_IR) {
-    pw.wide = 2048;
+    pw.wide = 4096;
     pw.wide_rem = 30000;
   }
   if (const char* e = std::getenv("QUADESTH_NB")) pw.nb = std::max(64, std::atoi(e) / 32 * 32);

@@ -199,7 +199,7 @@ HBM_PEAK_GBS = 8000.0
 
 def trailing_shapes(N, precision, diagonal=True):
     """(M, Nn, K) of every outer trailing-update launch of one factorisation:
-    the library's panel widths (FP16: 2048 while more than 30,000 columns
+    the library's panel widths (FP16: 4096 while more than 30,000 columns
     remain, then 1024; else 512; QUADESTH_NB, QUADESTH_NB_WIDE and
     QUADESTH_NB_WIDE_REM override) and its look-ahead split (the next panel's columns, then the
     rest).  With diagonal pivots the FP16 update of the rest is two launches,
@@ -208,7 +208,7 @@ def trailing_shapes(N, precision, diagonal=True):
     nb = max(64, nb // 32 * 32)
     wide = rem = 0
     if precision == "fp16":
-        wide = max(0, int(os.environ.get("QUADESTH_NB_WIDE", "2048")) // 32 * 32)
+        wide = max(0, int(os.environ.get("QUADESTH_NB_WIDE", "4096")) // 32 * 32)
         rem = max(0, int(os.environ.get("QUADESTH_NB_WIDE_REM", "30000")))
     tail = tail_rem = 0
     if precision == "fp16":

@@ -5,8 +5,8 @@ stream starts the next step's solve for the panel after next.  Panels wider
 than the strip solve takes (2048) give that main-stream solve inner FP16
 updates, so the two solves need separate scratch: shared scratch corrupts the
 factors (the refinement then needs tens of sweeps instead of a handful).  The
-schedules below cover 1024 throughout, 2048 throughout, 2048 then 1024, and 768
-throughout."""
+schedules below cover 1024 throughout, 2048 throughout, 2048 then 1024, 4096
+then 1024 (the default schedule's widths) and 768 throughout."""
 import os
 
 import pytest
@@ -19,6 +19,7 @@ SCHEDULES = {
     "narrow": {"QUADESTH_NB": "1024", "QUADESTH_NB_WIDE": "0"},
     "wide": {"QUADESTH_NB": "2048", "QUADESTH_NB_WIDE": "0"},
     "wide_then_narrow": {"QUADESTH_NB": "1024", "QUADESTH_NB_WIDE": "2048", "QUADESTH_NB_WIDE_REM": "8192"},
+    "wider_then_narrow": {"QUADESTH_NB": "1024", "QUADESTH_NB_WIDE": "4096", "QUADESTH_NB_WIDE_REM": "8192"},
     # a first panel that is not whole 512-column slices: A's conversion into
     # the factors' matrix rounds its first launch up to whole slices
     "odd_width": {"QUADESTH_NB": "768", "QUADESTH_NB_WIDE": "0"},
