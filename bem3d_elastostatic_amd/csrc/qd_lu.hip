@@ -3220,6 +3220,13 @@ static int backward_error(qd_context* h, int nrhs, const double* Rr, const doubl
   return QD_OK;
 }
 
+// The refinement's default bound on the normwise backward error
+// ||b - Ax||inf / (||A||inf ||x||inf + ||b||inf): LAPACK's mixed-precision
+// DSGESV (dsgesv.f) stops once ||r||inf < ||x||inf ||A||inf eps sqrt(n), eps =
+// 2^-53; with ||b|| in its denominator this bound is at least as strict.
+// Below 8,100 unknowns it is 1e-14.
+static double default_ir_tol(long long N) { return std::max(1e-14, std::ldexp(1.0, -53) * std::sqrt((double)N)); }
+
 int lu_solve(qd_context* h, int precision, int nrhs, double* b, long long ldb, double tol, int max_iter,
              bool trans, qd_solve_report* rep) {
   if (nrhs > 8) {
@@ -3285,7 +3292,7 @@ int lu_solve(qd_context* h, int precision, int nrhs, double* b, long long ldb, d
     int it = 0;
     bool done = false, stalled = false;
     const int maxit = max_iter > 0 ? max_iter : 30;
-    const double tl = tol > 0.0 ? tol : 1e-14;
+    const double tl = tol > 0.0 ? tol : default_ir_tol(N);
     // QUADESTH_IR_SWEEPS=k hands the solve to GMRES after k plain sweeps
     // that have not met the tolerance (the fallback's own trigger is a sweep
     // that fails to halve the residual)
@@ -3352,7 +3359,7 @@ int lu_solve(qd_context* h, int precision, int nrhs, double* b, long long ldb, d
   }
   if (precision != QD_FP64 && !R->converged)
     return set_error(h, QD_ERR_NOCONV, "refinement stopped at relative residual %.3e (tol %.1e) after %d sweeps",
-                     R->residual, tol > 0 ? tol : 1e-14, R->iterations);
+                     R->residual, tol > 0 ? tol : default_ir_tol(N), R->iterations);
   return QD_OK;
 }
 

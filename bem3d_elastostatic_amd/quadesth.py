@@ -143,10 +143,12 @@ class Quadesth:
         check(self.h, self.lib.qd_system_device(self.h, C.byref(pa), C.byref(lda), C.byref(pf)))
         return C.cast(pa, C.c_void_p).value, lda.value, C.cast(pf, C.c_void_p).value
 
-    def solve_linear_system(self, rhs=None, precision: str = "fp64", tol: float = 1e-14,
+    def solve_linear_system(self, rhs=None, precision: str = "fp64", tol: float = 0.0,
                             max_iter: int = 30):
         """solver_mod.f90:17-36. ``rhs`` None solves for the assembled FI;
-        otherwise (N,) or (N, k).  Returns (X, report dict)."""
+        otherwise (N,) or (N, k).  Returns (X, report dict).  ``tol`` bounds
+        the mixed precisions' refinement (normwise backward error); 0 selects
+        the library's default, max(1e-14, 2^-53 sqrt(N)) (LAPACK DSGESV's)."""
         rep = SolveReport()
         prec = PRECISIONS[precision]
         if rhs is None:
@@ -235,7 +237,7 @@ class Quadesth:
         paths.append(tpath.decode())
         return paths
 
-    def run(self, precision: str = "fp64", variant: int = 0, points=None, tol: float = 1e-14):
+    def run(self, precision: str = "fp64", variant: int = 0, points=None, tol: float = 0.0):
         """The whole hot path in main.f90's order; returns every result."""
         self.precompute_element_geometry()
         self.ghmat8()

@@ -3,7 +3,8 @@
 One step = the whole hot path over the synthetic thick-cantilever box
 (10 x 2 x 1, h = 0.1: 25,602 nodes, 2,400 quad9 + 8,000 tri6, 76,806 DOF,
 65,536 internal points): ghmat8 assembly, the mixed-precision solve (FP16
-MFMA trailing updates, FP64 refinement to 1e-14), reorganisation, and the
+MFMA trailing updates, FP64 refinement to LAPACK DSGESV's backward-error bound
+2^-53 sqrt(N)), reorganisation, and the
 Somigliana internal-point integrals.  Mesh, codes and points are resident in
 HBM before the timed region.  After the timed steps the same GPU's FP64 LU
 path is timed once on the same system, for the speed-up and for the error of
@@ -77,7 +78,8 @@ class Pipeline:
         if rc != 0:
             raise RuntimeError(f"hipMemcpy (device to device) failed: {rc}")
         rep = _lib.SolveReport()
-        rc = q.lib.qd_solve_device(q.h, _lib.PRECISIONS[prec], 1, C.c_void_p(self.b.data_ptr()), N, 1e-14, 30,
+        # tol 0: the library's default bound, max(1e-14, 2^-53 sqrt(N)) (LAPACK DSGESV's)
+        rc = q.lib.qd_solve_device(q.h, _lib.PRECISIONS[prec], 1, C.c_void_p(self.b.data_ptr()), N, 0.0, 30,
                                    C.byref(rep))
         _lib.check(q.h, rc, "solve")
         t2 = time.perf_counter()

@@ -136,7 +136,10 @@ int qd_system_device(qd_handle h, double** A_dev, long long* lda, double** FI_de
  * b: host (N x nrhs, column-major, leading dim ldb) right-hand sides on
  * input (NULL: solve for the assembled FI and return nothing), solutions on
  * output.  The factorisation is kept for later qd_solve calls with the same
- * precision until qd_ghmat8 runs again. */
+ * precision until qd_ghmat8 runs again.  tol bounds the mixed precisions'
+ * refinement: it stops once ||b - Ax||inf / (||A||inf ||x||inf + ||b||inf)
+ * <= tol; tol <= 0 selects max(1e-14, 2^-53 sqrt(N)), the bound LAPACK's
+ * DSGESV stops at.  max_iter <= 0 selects 30 sweeps. */
 int qd_solve(qd_handle h, int precision, int nrhs, double* b, int ldb, double tol, int max_iter,
              qd_solve_report* rep);
 /* Same with device right-hand sides (overwritten by the solutions). */
