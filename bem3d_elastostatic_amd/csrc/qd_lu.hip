@@ -64,6 +64,10 @@ struct qd_lu_state {
   F16Scratch sc_main, sc_panel;
   F16Scratch sc_solve;  // the inner updates of the main stream's U12 solve, so sc_main keeps L21
   F16Scratch sc_u12;    // those of the aux stream's U12 solve, which can overlap the main one
+  F16Scratch sc_l21;    // the product L21 = A21 U11^-1 of a diagonal-block panel (aux stream)
+  qd::DevBuf<float> binv;   // a diagonal-block panel's U11^-1 (npad x npad)
+  qd::DevBuf<float> binv_t; // its recursive doubling's intermediate products
+  qd::DevBuf<float> bdiag;  // u_kk, then -1 / u_kk (2 x npad)
   hipStream_t aux = nullptr;     // panel stream of the look-ahead (non-blocking, default priority)
   hipEvent_t ev_cols = nullptr, ev_panel = nullptr;
   hipEvent_t ev_top = nullptr, ev_u12 = nullptr;  // split steps: next U12 rows updated / solved
@@ -1616,6 +1620,448 @@ __global__ __launch_bounds__(256, 1) void k_gemm16w(long long M, long long N, in
 namespace qd {
 
 // ---------------------------------------------------------------------------
+// Diagonal-block panels.  With diagonal pivots a panel [j, j+jb) needs no
+// search over the rows below it: only its jb x jb diagonal block is factored
+// as a chain of small launches (D = L11 U11), and the multipliers below follow
+// from one product, L21 = A21 U11^-1.  U11^-1 is formed explicitly by
+// recursive doubling on 64-column blocks (the inverse of [U1 U12; 0 U2] is
+// [Y1, -Y1 U12 Y2; 0, Y2]), in FP32, padded to npad = 64 * 2^k with the
+// identity.  The product runs on the FP16 update kernels with both operands
+// scaled to O(1) entries through the diagonal of U11: A16 = -A21 diag(1/u_kk)
+// and B = diag(u_kk) U11^-1 (the inverse of U11 with its columns divided by
+// their diagonal entries: unit upper triangular), so that C -= colscale *
+// (A16 B16^T) on a cleared C leaves A21 U11^-1.
+
+// npad of a diagonal block of jb columns.
+static inline int binv_pad(int jb) {
+  int p = 64;
+  while (p < jb) p *= 2;
+  return p;
+}
+
+// ---- the diagonal block in 64-column steps ----------------------------------
+// The jb x jb diagonal block (jb a multiple of 64, origin (c0, c0) in W) in
+// 64 x 64 tiles, two launches per step k: k_blk_solve(k) turns tile column k
+// below the diagonal into multipliers (L_ik = A_ik U_kk^-1, row by row) and
+// tile row k right of it into rows of U (U_kj = L_kk^-1 A_kj, column by
+// column), both by substitution; k_blk_update(k) subtracts L_ik U_kj from
+// every trailing tile, and its workgroup 0, which holds tile (k+1, k+1),
+// then factors that tile.  k_blk_update(-1) only factors tile 0.  A launch is
+// a few microseconds of work; the chain of 2 jb / 64 - 1 launches replaces
+// the recursive panel's four to five per 64 columns.
+
+// Factor of the 64 x 64 tile held in sT (rows, already updated) at block tile
+// t, by a 256-thread workgroup: wave 0 eliminates with register rows and
+// readlane pivot-row broadcasts (the diagonal-pivot leaf's code), stores the
+// LU rows into W, ipiv = identity, and folds its largest multiplier (a zero
+// pivot counts as +inf) into *growth; then wave 1 forms the inverses of L's
+// two 32 x 32 diagonal blocks (for the strip solves of U12 outside the
+// block), wave 2 U_tt^-1 into Y's diagonal block t (row pitch npad) and
+// wave 3 dg[k] = u_kk, dg[npad + k] = -1 / u_kk for the tile's rows k.
+__device__ __forceinline__ void blk_factor_tile(float (*sT)[68], float* __restrict__ W, long long ld, long long c0,
+                                                int t, float* __restrict__ Y, int npad, float* __restrict__ dg,
+                                                float* __restrict__ linv, int* __restrict__ ipiv,
+                                                unsigned* __restrict__ growth) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const long long g0 = c0 + 64LL * t;
+  if (wave == 0) {
+    float a[kDW];
+#pragma unroll
+    for (int j = 0; j < kDW; ++j) a[j] = sT[lane][j];
+    float gmax = 0.0f;
+    dw_all64(std::make_integer_sequence<int, kDW>{}, a, gmax);
+    float* dst = W + (g0 + lane) * ld + g0;
+#pragma unroll
+    for (int j4 = 0; j4 < kDW / 4; ++j4)
+      *reinterpret_cast<float4*>(dst + 4 * j4) = make_float4(a[4 * j4], a[4 * j4 + 1], a[4 * j4 + 2], a[4 * j4 + 3]);
+#pragma unroll
+    for (int j = 0; j < kDW; ++j) sT[lane][j] = a[j];
+    ts_growth_max(gmax, growth);
+    ipiv[g0 + lane] = (int)(g0 + lane);
+  }
+  __syncthreads();
+  if (wave == 1) {
+    // lane c: column c % 32 of the inverse of L's diagonal block c / 32
+    const int blk = lane >> 5, cc = lane & 31, o = 32 * blk;
+    float y[32];
+#pragma unroll
+    for (int r = 0; r < 32; ++r) {
+      float acc = (r == cc) ? 1.0f : 0.0f;
+#pragma unroll
+      for (int m = 0; m < r; ++m) acc -= sT[o + r][o + m] * y[m];
+      y[r] = acc;
+    }
+    float* out = linv + (g0 / kTsW) * kTsW * kTsW + blk * 1024;
+#pragma unroll
+    for (int r = 0; r < 32; ++r) out[r * 32 + cc] = y[r];
+  } else if (wave == 2 && Y) {
+    // lane c: column c of U_tt^-1 by back substitution
+    float y[kDW];
+#pragma unroll
+    for (int r = kDW - 1; r >= 0; --r) {
+      float acc = (r == lane) ? 1.0f : 0.0f;
+#pragma unroll
+      for (int m = r + 1; m < kDW; ++m) acc -= sT[r][m] * y[m];
+      y[r] = acc / sT[r][r];
+    }
+    float* out = Y + (64LL * t) * npad + 64LL * t + lane;
+#pragma unroll
+    for (int r = 0; r < kDW; ++r) out[(long long)r * npad] = y[r];
+  } else if (wave == 3 && dg) {
+    const float d = sT[lane][lane];
+    dg[64 * t + lane] = d;
+    dg[npad + 64 * t + lane] = -1.0f / d;
+  }
+}
+
+// Step k's multipliers and U rows inside the block: workgroups [0, nb) take
+// the rows of tile column k below the diagonal (one row per thread: x =
+// a U_kk^-1, U from W's factored tile by LDS broadcast), the others the
+// columns of tile row k right of it (x = L_kk^-1 a).  nb = the workgroups of
+// either half.  The largest |l| goes to *growth.
+__global__ __launch_bounds__(256) void k_blk_solve(float* __restrict__ W, long long ld, long long c0, int jb, int k,
+                                                   int nb, unsigned* __restrict__ growth) {
+  __shared__ __attribute__((aligned(16))) float sU[kDW][kDW];
+  __shared__ float sR[kDW];
+  __shared__ unsigned wmax[4];
+  const int tid = threadIdx.x;
+  const bool rows = blockIdx.x < (unsigned)nb;
+  const int q = rows ? blockIdx.x : blockIdx.x - nb;
+  const long long d0 = c0 + 64LL * k;            // the diagonal tile's first row and column
+  const int rel = 64 * (k + 1) + q * 256 + tid;  // row (or column) in the block
+  const bool act = rel < jb;
+  float x[kDW];
+  if (act) {
+    if (rows) {
+      dw_load_row(W, ld, c0 + rel, d0, kDW, x);
+    } else {
+#pragma unroll
+      for (int i = 0; i < kDW; ++i) x[i] = W[(d0 + i) * ld + c0 + rel];
+    }
+  }
+  // rows: sU[i][j] = U_kk (upper part); columns: sU[m][i] = L_kk[i][m] (the
+  // unit lower part, transposed so that column m's entries are contiguous)
+#pragma unroll
+  for (int p = 0; p < kDW * kDW / 256; ++p) {
+    const int i = (tid + 256 * p) / kDW, j = (tid + 256 * p) % kDW;
+    const float v = W[(d0 + i) * ld + d0 + j];
+    if (rows) sU[i][j] = (j >= i) ? v : 0.0f;
+    else sU[j][i] = (j < i) ? v : 0.0f;
+  }
+  __syncthreads();
+  if (rows && tid < kDW) {
+    const float d = sU[tid][tid];
+    sR[tid] = (d != 0.0f) ? 1.0f / d : 0.0f;
+  }
+  __syncthreads();
+  float g = 0.0f;
+  if (act) {
+    if (rows) {
+      dwf_all(std::make_integer_sequence<int, kDW>{}, x, kDW, sU, sR);
+#pragma unroll
+      for (int j = 0; j < kDW; ++j) g = abs_max_keep_nan(g, x[j]);
+      float* dst = W + (c0 + rel) * ld + d0;
+#pragma unroll
+      for (int j4 = 0; j4 < kDW / 4; ++j4)
+        *reinterpret_cast<float4*>(dst + 4 * j4) = make_float4(x[4 * j4], x[4 * j4 + 1], x[4 * j4 + 2], x[4 * j4 + 3]);
+    } else {
+#pragma unroll
+      for (int m = 0; m < kDW; ++m) {
+        const float4* l4 = reinterpret_cast<const float4*>(sU[m]);
+#pragma unroll
+        for (int i4 = (m + 1) / 4; i4 < kDW / 4; ++i4) {
+          const float4 v = l4[i4];
+          if (4 * i4 > m) x[4 * i4] -= x[m] * v.x;
+          if (4 * i4 + 1 > m) x[4 * i4 + 1] -= x[m] * v.y;
+          if (4 * i4 + 2 > m) x[4 * i4 + 2] -= x[m] * v.z;
+          if (4 * i4 + 3 > m) x[4 * i4 + 3] -= x[m] * v.w;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < kDW; ++i) W[(d0 + i) * ld + c0 + rel] = x[i];
+    }
+  }
+  ts_growth_max_wg(g, growth, wmax);
+}
+
+// Step k's trailing update inside the block: tile (k+1+i, k+1+j) -= L_{k+1+i,k}
+// U_{k,k+1+j} for the nt x nt trailing tiles (workgroup b: i = b / nt, j = b %
+// nt), FP32 MFMA 16x16x4, each wave a 32 x 32 quarter; workgroup 0 (tile
+// (k+1, k+1)) keeps its result in LDS and factors it.  k = -1: one
+// workgroup, tile 0 factored as loaded.
+__global__ __launch_bounds__(256) void k_blk_update(float* __restrict__ W, long long ld, long long c0, int k, int nt,
+                                                    float* __restrict__ Y, int npad, float* __restrict__ dg,
+                                                    float* __restrict__ linv, int* __restrict__ ipiv,
+                                                    unsigned* __restrict__ growth) {
+  __shared__ __attribute__((aligned(16))) float sA[kDW][68];  // L tile transposed: sA[kk][m]; then the tile
+  __shared__ __attribute__((aligned(16))) float sB[kDW][68];  // U tile: sB[kk][n]
+  const int tid = threadIdx.x;
+  const int ti = k + 1 + (int)blockIdx.x / nt, tj = k + 1 + (int)blockIdx.x % nt;
+  if (k < 0) {
+#pragma unroll
+    for (int p = 0; p < kDW * kDW / 1024; ++p) {
+      const int idx = tid + 256 * p, r = idx / 16, c4 = (idx % 16) * 4;
+      *reinterpret_cast<float4*>(&sA[r][c4]) = *reinterpret_cast<const float4*>(W + (c0 + r) * ld + c0 + c4);
+    }
+    __syncthreads();
+    blk_factor_tile(sA, W, ld, c0, 0, Y, npad, dg, linv, ipiv, growth);
+    return;
+  }
+  const long long rA = c0 + 64LL * ti, cA = c0 + 64LL * k;  // L_{ti,k}
+  const long long rB = c0 + 64LL * k, cB = c0 + 64LL * tj;  // U_{k,tj}
+  float4 va[4], vb[4];
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const int idx = tid + 256 * p, r = idx / 16, c4 = (idx % 16) * 4;
+    va[p] = *reinterpret_cast<const float4*>(W + (rA + r) * ld + cA + c4);
+    vb[p] = *reinterpret_cast<const float4*>(W + (rB + r) * ld + cB + c4);
+  }
+  const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const long long rC = c0 + 64LL * ti, cC = c0 + 64LL * tj;
+  float cv[2][2][4];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) cv[i][j][r] = W[(rC + wm + 16 * i + 4 * lk + r) * ld + cC + wn + 16 * j + lr];
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const int idx = tid + 256 * p, r = idx / 16, c4 = (idx % 16) * 4;
+    sA[c4][r] = va[p].x;
+    sA[c4 + 1][r] = va[p].y;
+    sA[c4 + 2][r] = va[p].z;
+    sA[c4 + 3][r] = va[p].w;
+    *reinterpret_cast<float4*>(&sB[r][c4]) = vb[p];
+  }
+  __syncthreads();
+  f4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = (f4){0, 0, 0, 0};
+#pragma unroll
+  for (int kk = 0; kk < kDW; kk += 4) {
+    float a[2], b[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) a[i] = sA[kk + lk][wm + 16 * i + lr];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) b[j] = sB[kk + lk][wn + 16 * j + lr];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
+  }
+  if (blockIdx.x != 0) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          W[(rC + wm + 16 * i + 4 * lk + r) * ld + cC + wn + 16 * j + lr] = cv[i][j][r] - acc[i][j][r];
+    return;
+  }
+  __syncthreads();  // every wave is done reading sA
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) sA[wm + 16 * i + 4 * lk + r][wn + 16 * j + lr] = cv[i][j][r] - acc[i][j][r];
+  __syncthreads();
+  blk_factor_tile(sA, W, ld, c0, k + 1, Y, npad, dg, linv, ipiv, growth);
+}
+
+// Identity in Y's diagonal blocks [t0, npad / 64) (the padding of U11^-1).
+__global__ __launch_bounds__(256) void k_binv_identity(int t0, int npad, float* __restrict__ Y) {
+  const long long b0 = 64LL * (t0 + blockIdx.x);
+  for (int i = threadIdx.x; i < kDW * kDW; i += 256) Y[(b0 + i / kDW) * npad + b0 + i % kDW] = (i / kDW == i % kDW);
+}
+
+// Batched FP32 product C_p = alpha * A_p B_p (C not read), each operand of
+// batch p = blockIdx.z at base + p * stride; all of M, N, K multiples of TM.
+// A's entries at row ar0 + p * astep + r >= alim or column ac0 + p * astep +
+// k >= alim read as zero (the identity padding of U's off-diagonal blocks).
+// kTriA / kTriB: that operand is upper triangular, its entries below the
+// diagonal read as zero (they are not stored).  TM x TM tile per 256-thread
+// workgroup, 2 x 2 waves of (TM/2)^2, MFMA f32 16x16x4, K staged through LDS
+// in steps of 16.
+template <int TM, bool kTriA, bool kTriB>
+__global__ __launch_bounds__(256) void k_gemm_bat(int K, const float* __restrict__ A, long long lda, long long sa,
+                                                  int ar0, int ac0, int astep, int alim,
+                                                  const float* __restrict__ B, long long ldb, long long sb,
+                                                  float* __restrict__ C, long long ldc, long long sc, float alpha) {
+  constexpr int BK = 16, MI = TM / 32, PAD = 4;
+  __shared__ float sA[BK][TM + PAD];
+  __shared__ float sB[BK][TM + PAD];
+  const int p = blockIdx.z;
+  A += (long long)p * sa;
+  B += (long long)p * sb;
+  C += (long long)p * sc;
+  const int rlim = alim - (ar0 + p * astep), klim = alim - (ac0 + p * astep);
+  const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TM;
+  const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+  const int wm = (wave / 2) * (TM / 2), wn = (wave % 2) * (TM / 2);
+  const int lr = lane % 16, lk = lane / 16;
+  f4 acc[MI][MI];
+#pragma unroll
+  for (int i = 0; i < MI; ++i)
+#pragma unroll
+    for (int j = 0; j < MI; ++j) acc[i][j] = (f4){0, 0, 0, 0};
+  constexpr int NA = TM * BK / 256;  // elements per thread and operand
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    float va[NA], vb[NA];
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+      const int idx = tid + 256 * i;
+      const int r = m0 + idx / BK, k = k0 + idx % BK;
+      va[i] = (r < rlim && k < klim && (!kTriA || k >= r)) ? A[(long long)r * lda + k] : 0.0f;
+    }
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+      const int idx = tid + 256 * i;
+      const int k = k0 + idx / TM, n = n0 + idx % TM;
+      vb[i] = (!kTriB || n >= k) ? B[(long long)k * ldb + n] : 0.0f;
+    }
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+      const int idx = tid + 256 * i;
+      sA[idx % BK][idx / BK] = va[i];
+      sB[idx / TM][idx % TM] = vb[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      float a[MI], b[MI];
+#pragma unroll
+      for (int i = 0; i < MI; ++i) a[i] = sA[kk + lk][wm + 16 * i + lr];
+#pragma unroll
+      for (int j = 0; j < MI; ++j) b[j] = sB[kk + lk][wn + 16 * j + lr];
+#pragma unroll
+      for (int i = 0; i < MI; ++i)
+#pragma unroll
+        for (int j = 0; j < MI; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < MI; ++i)
+#pragma unroll
+    for (int j = 0; j < MI; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        C[(long long)(m0 + wm + 16 * i + 4 * lk + r) * ldc + n0 + wn + 16 * j + lr] = alpha * acc[i][j][r];
+}
+
+// Y[k][n] = dg[k] Y[k][n] for k < n < jb, 0 for n <= k (row pitch npad, a
+// multiple of 64): U11^-1 becomes the inverse of U11 with its columns divided
+// by their diagonal entries (unit upper triangular), less the identity.
+__global__ __launch_bounds__(256) void k_binv_rowscale(int jb, int npad, float* __restrict__ Y,
+                                                       const float* __restrict__ dg) {
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  const int per = npad / 4;
+  if (idx >= (long long)jb * per) return;
+  const int k = (int)(idx / per), n = (int)(idx % per) * 4;
+  if (n >= jb) return;
+  float4* p = reinterpret_cast<float4*>(Y + (long long)k * npad + n);
+  const float d = dg[k];
+  float4 v = *p;
+  v.x = (n > k) ? v.x * d : 0.0f;
+  v.y = (n + 1 > k) ? v.y * d : 0.0f;
+  v.z = (n + 2 > k) ? v.z * d : 0.0f;
+  v.w = (n + 3 > k) ? v.w * d : 0.0f;
+  *p = v;
+}
+
+// A16[r][k] = -A[r][k] / u_kk (nrd[k] = -1 / u_kk) for k < K, zero up to Kp,
+// and A[r][k] itself replaced by A[r][k] / u_kk (FP32): with N the strictly
+// upper part of the unit upper triangular inverse of U11 diag(1/u_kk),
+// L21 = A21 U11^-1 = Ahat + Ahat N for Ahat = A21 diag(1/u_kk), and only the
+// correction Ahat N goes through FP16 (C -= colscale * (A16 B16^T)).  One
+// thread per 8 halves; vec: A and lda 16-byte aligned and K a multiple of 8.
+__global__ __launch_bounds__(256) void k_cvt_a16_l21(long long M, int K, int Kp, float* __restrict__ A, long long lda,
+                                                     const float* __restrict__ nrd, _Float16* __restrict__ A16,
+                                                     int vec) {
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  const int per = Kp / 8;
+  if (idx >= M * per) return;
+  const long long r = idx / per;
+  const int k0 = (int)(idx % per) * 8;
+  h8 v;
+  if (vec && k0 + 8 <= K) {
+    float* src = A + r * lda + k0;
+    const float4 p0 = *reinterpret_cast<const float4*>(src);
+    const float4 p1 = *reinterpret_cast<const float4*>(src + 4);
+    const float4 s0 = *reinterpret_cast<const float4*>(nrd + k0);
+    const float4 s1 = *reinterpret_cast<const float4*>(nrd + k0 + 4);
+    v[0] = (_Float16)(p0.x * s0.x);
+    v[1] = (_Float16)(p0.y * s0.y);
+    v[2] = (_Float16)(p0.z * s0.z);
+    v[3] = (_Float16)(p0.w * s0.w);
+    v[4] = (_Float16)(p1.x * s1.x);
+    v[5] = (_Float16)(p1.y * s1.y);
+    v[6] = (_Float16)(p1.z * s1.z);
+    v[7] = (_Float16)(p1.w * s1.w);
+    *reinterpret_cast<float4*>(src) = make_float4(-p0.x * s0.x, -p0.y * s0.y, -p0.z * s0.z, -p0.w * s0.w);
+    *reinterpret_cast<float4*>(src + 4) = make_float4(-p1.x * s1.x, -p1.y * s1.y, -p1.z * s1.z, -p1.w * s1.w);
+  } else {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int k = k0 + q;
+      float x = 0.0f;
+      if (k < K) {
+        x = A[r * lda + k] * nrd[k];
+        A[r * lda + k] = -x;
+      }
+      v[q] = (_Float16)x;
+    }
+  }
+  *(h8*)(A16 + r * Kp + k0) = v;
+}
+
+// k_cvt_a16 that also folds the largest |a| (NaN kept) into *growth: the
+// trailing update's conversion of L21, which reads every multiplier below a
+// diagonal-pivot panel once.
+__global__ __launch_bounds__(256) void k_cvt_a16_growth(long long M, int K, int Kp, const float* __restrict__ A,
+                                                        long long lda, _Float16* __restrict__ A16, int vec,
+                                                        unsigned* __restrict__ growth) {
+  __shared__ unsigned wmax[4];
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  const int per = Kp / 8;
+  float g = 0.0f;
+  if (idx < M * per) {
+    const long long r = idx / per;
+    const int k0 = (int)(idx % per) * 8;
+    float x[8];
+    if (vec && k0 + 8 <= K) {
+      const float4 p0 = *reinterpret_cast<const float4*>(A + r * lda + k0);
+      const float4 p1 = *reinterpret_cast<const float4*>(A + r * lda + k0 + 4);
+      x[0] = p0.x;
+      x[1] = p0.y;
+      x[2] = p0.z;
+      x[3] = p0.w;
+      x[4] = p1.x;
+      x[5] = p1.y;
+      x[6] = p1.z;
+      x[7] = p1.w;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) x[q] = (k0 + q < K) ? A[r * lda + k0 + q] : 0.0f;
+    }
+    h8 v;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      v[q] = (_Float16)x[q];
+      g = abs_max_keep_nan(g, x[q]);
+    }
+    *(h8*)(A16 + r * Kp + k0) = v;
+  }
+  ts_growth_max_wg(g, growth, wmax);
+}
+
+// ---------------------------------------------------------------------------
 // host-side LU driver
 template <typename T>
 struct LuCtx {
@@ -1713,9 +2159,10 @@ constexpr long long kStripNarrowCols = 8192;
 // previous call on this context's stream).
 // max_wgs > 0 caps the workgroups of the FP16 kernel (each then loops over
 // several tiles).
+// growth: the largest |a| of A goes there as well (FP16 path only).
 template <typename T>
 static int gemm_sub(LuCtx<T>& c, long long M, long long Nn, int K, const T* A, const T* B, T* C, bool allow_f16,
-                    bool trailing = false, bool reuse_a16 = false, int max_wgs = 0) {
+                    bool trailing = false, bool reuse_a16 = false, int max_wgs = 0, unsigned* growth = nullptr) {
   if (M <= 0 || Nn <= 0 || K <= 0) return QD_OK;
   int rc_ = QD_OK;
   dim3 grid((unsigned)((Nn + GBN - 1) / GBN), (unsigned)((M + GBM - 1) / GBM));
@@ -1732,7 +2179,10 @@ static int gemm_sub(LuCtx<T>& c, long long M, long long Nn, int K, const T* A, c
       const int va = ((reinterpret_cast<uintptr_t>(A) & 15) == 0 && c.ld % 4 == 0 && K % 8 == 0) ? 1 : 0;
       hipLaunchKernelGGL(k_scale_cvt_b16t, dim3((unsigned)nb), dim3(256), 0, c.st, K, Nn, Kp, B, c.ld,
                          c.sc->colscale.p, c.sc->h16b.p, vb);
-      if (!reuse_a16)
+      if (!reuse_a16 && growth)
+        hipLaunchKernelGGL(k_cvt_a16_growth, dim3((unsigned)((M * (Kp / 8) + 255) / 256)), dim3(256), 0, c.st, M,
+                           K, Kp, A, c.ld, c.sc->h16a.p, va, growth);
+      else if (!reuse_a16)
         hipLaunchKernelGGL(k_cvt_a16, dim3((unsigned)((M * (Kp / 8) + 255) / 256)), dim3(256), 0, c.st, M, K,
                            Kp, A, c.ld, c.sc->h16a.p, va);
       ProfScope ps(c.h, trailing ? QD_PROF_LU_TRAILING_GEMM : -1, 2.0 * (double)M * (double)Nn * K, c.st);
@@ -1895,11 +2345,98 @@ static int panel(LuCtx<T>& c, long long c0, int w, long long p0, long long p1) {
   return panel(c, c0 + w1, w - w1, p0, p1);
 }
 
+// QUADESTH_BLOCK_PANEL=0 keeps diagonal-pivot panels as tall recursive
+// factorisations of all their rows.
+static bool block_panels() {
+  static const bool on = [] {
+    const char* e = std::getenv("QUADESTH_BLOCK_PANEL");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
+// Diagonal-pivot panel [j, j+jb) with rows below it (jb a multiple of 64):
+// the jb x jb diagonal block factored in 64-column steps of two launches,
+// U11^-1 (whose diagonal blocks the steps leave in Y) completed by recursive
+// doubling, then L21 = A21 U11^-1 in one FP16 product into W.  The growth of
+// L21 is taken by the trailing update's conversion of it (update_cols).
+static int panel_block(LuCtx<float>& c, long long j, int jb) {
+  qd_lu_state* s = c.s;
+  const int npad = binv_pad(jb), T = jb / 64;
+  const long long r0 = j + jb, M = c.N - r0;
+  const int Kp = f16_kpad(jb);
+  const long long nb = (jb + 63) / 64;
+  qd_lu_state::F16Scratch& sc = s->sc_l21;
+  if (jb % 64 != 0 || T < 2)
+    return set_error(c.h, QD_ERR_STATE, "diagonal-block panel of %d columns (a multiple of 64 from 128 up)", jb);
+  if (s->binv.count < (size_t)npad * npad || s->binv_t.count < (size_t)npad * npad / 4 ||
+      s->bdiag.count < (size_t)2 * npad || sc.h16a.count < (size_t)(M * Kp) || sc.h16b.count < (size_t)jb * Kp ||
+      sc.colscale.count < (size_t)nb)
+    return set_error(c.h, QD_ERR_STATE, "diagonal-block panel buffers sized for fewer than %d columns", jb);
+  float* Y = s->binv.p;
+  float* Tm = s->binv_t.p;
+  float* dg = s->bdiag.p;
+  float* linv = s->ts_linv.p;
+  // the diagonal block: tile 0, then per step k the solves of tile row and
+  // column k and the trailing update with the factorisation of tile k+1
+  hipLaunchKernelGGL(k_blk_update, dim3(1), dim3(256), 0, c.st, c.W, c.ld, j, -1, 1, Y, npad, dg, linv,
+                     s->ipiv_dev.p, s->growth.p);
+  for (int k = 0; k + 1 < T; ++k) {
+    const int rest = jb - 64 * (k + 1), nt = T - 1 - k;
+    const int hb = (rest + 255) / 256;
+    hipLaunchKernelGGL(k_blk_solve, dim3((unsigned)(2 * hb)), dim3(256), 0, c.st, c.W, c.ld, j, jb, k, hb,
+                       s->growth.p);
+    hipLaunchKernelGGL(k_blk_update, dim3((unsigned)(nt * nt)), dim3(256), 0, c.st, c.W, c.ld, j, k, nt, Y, npad, dg,
+                       linv, s->ipiv_dev.p, s->growth.p);
+  }
+  QD_HIP(c.h, hipGetLastError());
+  if (npad > jb)
+    hipLaunchKernelGGL(k_binv_identity, dim3((unsigned)(npad / 64 - T)), dim3(256), 0, c.st, T, npad, Y);
+  const float* U = c.W + j * c.ld + j;
+  constexpr int kNoLim = 1 << 30;
+  for (int b = 64; b < npad; b *= 2) {
+    // pair p covers rows and columns [2pb, 2pb + 2b): T_p = U12_p Y2_p, then
+    // Y12_p = -Y1_p T_p (Y1, Y2 upper triangular)
+    const unsigned P = (unsigned)(npad / (2 * b));
+    const long long sy = 2LL * b * (npad + 1), su = 2LL * b * (c.ld + 1), st = (long long)b * b;
+    const float* Y2 = Y + (long long)b * npad + b;
+    if (b == 64) {
+      hipLaunchKernelGGL((k_gemm_bat<64, false, true>), dim3(1, 1, P), dim3(256), 0, c.st, b, U + b, c.ld, su, 0, b,
+                         2 * b, jb, Y2, (long long)npad, sy, Tm, (long long)b, st, 1.0f);
+      hipLaunchKernelGGL((k_gemm_bat<64, true, false>), dim3(1, 1, P), dim3(256), 0, c.st, b, (const float*)Y,
+                         (long long)npad, sy, 0, 0, 0, kNoLim, (const float*)Tm, (long long)b, st, Y + b,
+                         (long long)npad, sy, -1.0f);
+    } else {
+      const dim3 g((unsigned)(b / 128), (unsigned)(b / 128), P);
+      hipLaunchKernelGGL((k_gemm_bat<128, false, true>), g, dim3(256), 0, c.st, b, U + b, c.ld, su, 0, b, 2 * b, jb,
+                         Y2, (long long)npad, sy, Tm, (long long)b, st, 1.0f);
+      hipLaunchKernelGGL((k_gemm_bat<128, true, false>), g, dim3(256), 0, c.st, b, (const float*)Y, (long long)npad,
+                         sy, 0, 0, 0, kNoLim, (const float*)Tm, (long long)b, st, Y + b, (long long)npad, sy, -1.0f);
+    }
+  }
+  hipLaunchKernelGGL(k_binv_rowscale, dim3((unsigned)(((long long)jb * (npad / 4) + 255) / 256)), dim3(256), 0, c.st,
+                     jb, npad, Y, (const float*)dg);
+  hipLaunchKernelGGL(k_scale_cvt_b16t, dim3((unsigned)nb), dim3(256), 0, c.st, jb, (long long)jb, Kp, (const float*)Y,
+                     (long long)npad, sc.colscale.p, sc.h16b.p, 1);
+  float* A21 = c.W + r0 * c.ld + j;
+  const int va = (c.ld % 4 == 0 && j % 4 == 0 && jb % 8 == 0) ? 1 : 0;
+  hipLaunchKernelGGL(k_cvt_a16_l21, dim3((unsigned)((M * (Kp / 8) + 255) / 256)), dim3(256), 0, c.st, M, jb, Kp, A21,
+                     c.ld, (const float*)(dg + npad), sc.h16a.p, va);
+  QD_HIP(c.h, hipGetLastError());
+  const int rows16 = (c.ld % 4 == 0 && (reinterpret_cast<uintptr_t>(A21) & 15) == 0) ? 1 : 0;
+  return launch_f16_update(c.h, false, M, jb, Kp, sc.h16a.p, sc.h16b.p, sc.colscale.p, A21, c.ld, rows16, 0, c.st);
+}
+
 // Panel of columns [j, j+jb) on the context's stream, timed as one lu_panel.
 template <typename T>
 static int panel_step(LuCtx<T>& c, long long j, int jb) {
   const double m = (double)(c.N - j);
   ProfScope ps(c.h, QD_PROF_LU_PANEL, m * jb * (double)jb - (double)jb * jb * jb / 3.0, c.st);
+  if constexpr (std::is_same<T, float>::value) {
+    if (c.calu && c.diag && c.f16 && block_panels() && jb > kDW && jb % kDW == 0 && j % 4 == 0 && j + jb < c.N &&
+        c.ld % 4 == 0)
+      return panel_block(c, j, jb);
+  }
   return panel(c, j, jb, j, j + jb);
 }
 
@@ -1908,8 +2445,10 @@ template <typename T>
 static int update_cols(LuCtx<T>& c, long long j, int jb, long long c0, long long c1, bool reuse_a16,
                        int max_wgs = 0) {
   const long long r0 = j + jb;
+  // with diagonal pivots the first conversion of L21 also takes its growth
+  unsigned* g = (c.calu && c.diag && !reuse_a16) ? c.s->growth.p : nullptr;
   return gemm_sub(c, c.N - r0, c1 - c0, jb, c.W + r0 * c.ld + j, c.W + j * c.ld + c0, c.W + r0 * c.ld + c0, true,
-                  true, reuse_a16, max_wgs);
+                  true, reuse_a16, max_wgs, g);
 }
 
 // Largest multiplier |l| with which a diagonal-pivot factorisation is kept.
@@ -2045,6 +2584,15 @@ static int factor(LuCtx<T>& c, const PanelWidths& pw) {
     QD_ALLOC(c.h, s->sc_solve.h16a, Kh * Kh);
     QD_ALLOC(c.h, s->sc_solve.h16b, (size_t)N * Kh);
     QD_ALLOC(c.h, s->sc_solve.colscale, (size_t)(N / 64 + 2));
+    if (c.calu && c.diag && block_panels()) {
+      const size_t np = (size_t)binv_pad(NB);
+      QD_ALLOC(c.h, s->binv, np * np);
+      QD_ALLOC(c.h, s->binv_t, np * np / 4);
+      QD_ALLOC(c.h, s->bdiag, 2 * np);
+      QD_ALLOC(c.h, s->sc_l21.h16a, (size_t)N * Kp);
+      QD_ALLOC(c.h, s->sc_l21.h16b, (size_t)NB * Kp);
+      QD_ALLOC(c.h, s->sc_l21.colscale, (size_t)(NB / 64 + 2));
+    }
     if (look) {
       QD_ALLOC(c.h, s->sc_u12.h16a, Kh * Kh);
       QD_ALLOC(c.h, s->sc_u12.h16b, (size_t)N * Kh);
