@@ -68,6 +68,12 @@ struct qd_lu_state {
   qd::DevBuf<float> binv;   // a diagonal-block panel's U11^-1 (npad x npad)
   qd::DevBuf<float> binv_t; // its recursive doubling's intermediate products
   qd::DevBuf<float> bdiag;  // u_kk, then -1 / u_kk (2 x npad)
+  qd::DevBuf<float> bstep;  // step buffers of the diagonal block's 64-column steps
+  // FP16 copies of the L21 that the last two diagonal-block panels wrote (the
+  // trailing updates' left operand), and the first column of their panels
+  qd::DevBuf<_Float16> l21h[2];
+  long long l21h_col[2] = {-1, -1};
+  int l21h_next = 0;
   hipStream_t aux = nullptr;     // panel stream of the look-ahead (non-blocking, default priority)
   hipEvent_t ev_cols = nullptr, ev_panel = nullptr;
   hipEvent_t ev_top = nullptr, ev_u12 = nullptr;  // split steps: next U12 rows updated / solved
@@ -1133,10 +1139,40 @@ __device__ __forceinline__ void g16_issue(unsigned char* __restrict__ st, long l
   }
 }
 
+// FP16 copy of an update's result (row-major, row pitch ld, rows counted from
+// the update's first row) and the word that takes the largest |result|.
+struct Out16 {
+  _Float16* h;
+  long long ld;
+  unsigned* growth;
+};
+// Running maximum of |v| that keeps NaN (unsigned order of the magnitudes'
+// bit patterns: NaN above +inf).
+__device__ __forceinline__ float out16_absmax(float g, float v) {
+  return __uint_as_float(max(__float_as_uint(g), __float_as_uint(fabsf(v))));
+}
+// The wave's largest magnitude folded into *growth (whole waves call it).
+__device__ __forceinline__ void out16_growth(float g, unsigned* growth) {
+  unsigned b = __float_as_uint(g);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) b = max(b, (unsigned)__shfl_xor((int)b, off));
+  if ((threadIdx.x & 63) == 0) atomicMax(growth, b);
+}
+__device__ __forceinline__ void out16_put(const Out16& o, long long r, long long c, float v, float& g) {
+  o.h[r * o.ld + c] = (_Float16)v;
+  g = out16_absmax(g, v);
+}
+__device__ __forceinline__ void out16_put4(const Out16& o, long long r, long long c, float4 v, float& g) {
+  typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+  *reinterpret_cast<h4*>(o.h + r * o.ld + c) = (h4){(_Float16)v.x, (_Float16)v.y, (_Float16)v.z, (_Float16)v.w};
+  g = out16_absmax(out16_absmax(out16_absmax(out16_absmax(g, v.x), v.y), v.z), v.w);
+}
+
+template <bool kOut16>
 __device__ __forceinline__ void gemm16_tile(long long m0, long long n0, long long M, long long N, int Kp,
                                             const _Float16* __restrict__ A16, const _Float16* __restrict__ B16,
                                             const float* __restrict__ colscale, float* __restrict__ C, long long ldc,
-                                            bool rows16, G16Smem& sm) {
+                                            bool rows16, G16Smem& sm, const Out16& o16, float& gmax) {
   const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
   const int wm = (wave / 2) * (H_BM / 2), wn = (wave % 2) * 64;
   const int lr = lane % 16, lk = lane / 16;
@@ -1240,6 +1276,10 @@ __device__ __forceinline__ void gemm16_tile(long long m0, long long n0, long lon
           o.w = cv[t].w - s * av[t].w;
           __builtin_nontemporal_store((f4){o.x, o.y, o.z, o.w},
                                       reinterpret_cast<f4*>(cbase + (32 * q + 4 * (4 * h + t)) * ld));
+          if constexpr (kOut16) {
+            const long long orow = m0 + wm + (lane >> 4) + 32 * q + 4 * (4 * h + t);
+            out16_put4(o16, orow, n0 + wn + 4 * c4, o, gmax);
+          }
         }
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1263,7 +1303,11 @@ __device__ __forceinline__ void gemm16_tile(long long m0, long long n0, long lon
 #pragma unroll
       for (int i = 0; i < kG16Mi; ++i)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) C[(m0 + wm + 16 * i + 4 * lk + r) * ldc + gc] = cv[i][r] - s * acc[i][j][r];
+        for (int r = 0; r < 4; ++r) {
+          const float o = cv[i][r] - s * acc[i][j][r];
+          C[(m0 + wm + 16 * i + 4 * lk + r) * ldc + gc] = o;
+          if constexpr (kOut16) out16_put(o16, m0 + wm + 16 * i + 4 * lk + r, gc, o, gmax);
+        }
     }
     return;
   }
@@ -1286,18 +1330,24 @@ __device__ __forceinline__ void gemm16_tile(long long m0, long long n0, long lon
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const long long gr = m0 + wm + 16 * i + 4 * lk + r;
-        if (gr < M && gc < N) C[gr * ldc + gc] = cv[i][r] - s * acc[i][j][r];
+        if (gr < M && gc < N) {
+          const float o = cv[i][r] - s * acc[i][j][r];
+          C[gr * ldc + gc] = o;
+          if constexpr (kOut16) out16_put(o16, gr, gc, o, gmax);
+        }
       }
   }
 }
 
-// rows16: C and ldc keep every row start 16-byte aligned.
+// rows16: C and ldc keep every row start 16-byte aligned.  kRole 2 also
+// writes the result as FP16 into o16 and its largest |c| into o16.growth.
 template <int kRole>
 __global__ __launch_bounds__(256, QD_G16_OCC) void k_gemm16(long long M, long long N, int Kp,
                                                             const _Float16* __restrict__ A16,
                                                 const _Float16* __restrict__ B16, const float* __restrict__ colscale,
-                                                float* __restrict__ C, long long ldc, int rows16) {
+                                                float* __restrict__ C, long long ldc, int rows16, Out16 o16) {
   __shared__ __attribute__((aligned(16))) G16Smem sm;
+  float gmax = 0.0f;
   const unsigned tiles_n = (unsigned)((N + H_BN - 1) / H_BN);
   const unsigned tiles_m = (unsigned)((M + H_BM - 1) / H_BM);
   const unsigned total = tiles_m * tiles_n;
@@ -1314,8 +1364,10 @@ __global__ __launch_bounds__(256, QD_G16_OCC) void k_gemm16(long long M, long lo
     const unsigned gsz = min(tiles_m - first_m, (unsigned)kG16Group);
     const unsigned in_grp = vid - grp * kG16Group * tiles_n;
     const unsigned tm = first_m + in_grp % gsz, tn = in_grp / gsz;
-    gemm16_tile((long long)tm * H_BM, (long long)tn * H_BN, M, N, Kp, A16, B16, colscale, C, ldc, rows16 != 0, sm);
+    gemm16_tile<kRole == 2>((long long)tm * H_BM, (long long)tn * H_BN, M, N, Kp, A16, B16, colscale, C, ldc,
+                            rows16 != 0, sm, o16, gmax);
   }
+  if constexpr (kRole == 2) out16_growth(gmax, o16.growth);
 }
 
 // ---------------------------------------------------------------------------
@@ -1449,10 +1501,11 @@ __device__ __forceinline__ void hsteps(std::integer_sequence<int, Us...>, int hs
   (one(std::integral_constant<int, Us>{}), ...);
 }
 
+template <bool kOut16>
 __device__ __forceinline__ void tile(long long m0, long long n0, long long M, long long N, int Kp,
                                      const _Float16* __restrict__ A16, const _Float16* __restrict__ B16,
                                      const float* __restrict__ colscale, float* __restrict__ C, long long ldc,
-                                     bool rows16, Smem& sm) {
+                                     bool rows16, Smem& sm, const Out16& o16, float& gmax) {
   const int tid = threadIdx.x, lane = tid % 64;
   const int wave = __builtin_amdgcn_readfirstlane(tid / 64);  // wave-uniform: LDS addresses in SGPRs
   const int wm = (wave / 2) * 128, wn = (wave % 2) * 128;
@@ -1547,7 +1600,10 @@ __device__ __forceinline__ void tile(long long m0, long long n0, long long M, lo
           const int col = wn + 32 * j + l32;
           const float cv = *reinterpret_cast<const float*>(region + lr * 1024 +
                                                           16 * ((col >> 2) ^ (8 * ((lr >> 2) & 1))) + 4 * (col & 3));
-          __builtin_nontemporal_store(cv - sc[j] * acc[q][j][r], p + 32 * j);
+          const float o = cv - sc[j] * acc[q][j][r];
+          __builtin_nontemporal_store(o, p + 32 * j);
+          if constexpr (kOut16)
+            out16_put(o16, m0 + wm + 4 * lh + 32 * q + 8 * (r / 4) + (r % 4), n0 + col, o, gmax);
         }
       }
       if (q + 2 < 4) {
@@ -1580,7 +1636,11 @@ __device__ __forceinline__ void tile(long long m0, long long n0, long long M, lo
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const long long row = m0 + wm + 32 * i + 8 * (r / 4) + 4 * lh + (r % 4);
-        if (row < M && col < N) C[row * ldc + col] = cv[j][r] - s * acc[i][j][r];
+        if (row < M && col < N) {
+          const float o = cv[j][r] - s * acc[i][j][r];
+          C[row * ldc + col] = o;
+          if constexpr (kOut16) out16_put(o16, row, col, o, gmax);
+        }
       }
     }
   }
@@ -1594,8 +1654,9 @@ template <int kRole>
 __global__ __launch_bounds__(256, 1) void k_gemm16w(long long M, long long N, int Kp, const _Float16* __restrict__ A16,
                                                      const _Float16* __restrict__ B16,
                                                      const float* __restrict__ colscale, float* __restrict__ C,
-                                                     long long ldc, int rows16) {
+                                                     long long ldc, int rows16, Out16 o16) {
   __shared__ __attribute__((aligned(16))) w16::Smem sm;
+  float gmax = 0.0f;
   const unsigned tiles_n = (unsigned)((N + w16::BN - 1) / w16::BN);
   const unsigned tiles_m = (unsigned)((M + w16::BM - 1) / w16::BM);
   const unsigned total = tiles_m * tiles_n;
@@ -1609,8 +1670,10 @@ __global__ __launch_bounds__(256, 1) void k_gemm16w(long long M, long long N, in
     const unsigned gsz = min(tiles_m - first_m, (unsigned)kG16Group);
     const unsigned in_grp = vid - grp * kG16Group * tiles_n;
     const unsigned tm = first_m + in_grp % gsz, tn = in_grp / gsz;
-    w16::tile((long long)tm * w16::BM, (long long)tn * w16::BN, M, N, Kp, A16, B16, colscale, C, ldc, rows16 != 0, sm);
+    w16::tile<kRole == 2>((long long)tm * w16::BM, (long long)tn * w16::BN, M, N, Kp, A16, B16, colscale, C, ldc,
+                          rows16 != 0, sm, o16, gmax);
   }
+  if constexpr (kRole == 2) out16_growth(gmax, o16.growth);
 }
 
 }  // namespace qd
@@ -1641,27 +1704,58 @@ static inline int binv_pad(int jb) {
 
 // ---- the diagonal block in 64-column steps ----------------------------------
 // The jb x jb diagonal block (jb a multiple of 64, origin (c0, c0) in W) in
-// 64 x 64 tiles, two launches per step k: k_blk_solve(k) turns tile column k
-// below the diagonal into multipliers (L_ik = A_ik U_kk^-1, row by row) and
-// tile row k right of it into rows of U (U_kj = L_kk^-1 A_kj, column by
-// column), both by substitution; k_blk_update(k) subtracts L_ik U_kj from
-// every trailing tile, and its workgroup 0, which holds tile (k+1, k+1),
-// then factors that tile.  k_blk_update(-1) only factors tile 0.  A launch is
-// a few microseconds of work; the chain of 2 jb / 64 - 1 launches replaces
-// the recursive panel's four to five per 64 columns.
+// 64 x 64 tiles, one launch per step k (k_blk_step): the workgroup of
+// trailing tile (i, j), i, j > k, forms L_ik = A_ik U_kk^-1 and U_kj = L_kk^-1
+// A_kj itself, from the inverses of tile k's factors and from copies of tile
+// column and row k that the previous step left in colbuf / rowbuf, then
+// subtracts L_ik U_kj from its tile.  Workgroups of tile column k+1 also
+// store L_ik into W and their result into the next step's colbuf, those of
+// tile row k+1 U_kj and the next rowbuf; workgroup 0, tile (k+1, k+1),
+// factors its result.  Step buffers alternate by the parity of k, so no
+// workgroup reads what another writes in the same launch.  A launch is a few
+// microseconds of work: the chain of jb / 64 launches replaces the recursive
+// panel's four to five per 64 columns.
 
-// Factor of the 64 x 64 tile held in sT (rows, already updated) at block tile
-// t, by a 256-thread workgroup: wave 0 eliminates with register rows and
-// readlane pivot-row broadcasts (the diagonal-pivot leaf's code), stores the
-// LU rows into W, ipiv = identity, and folds its largest multiplier (a zero
-// pivot counts as +inf) into *growth; then wave 1 forms the inverses of L's
-// two 32 x 32 diagonal blocks (for the strip solves of U12 outside the
-// block), wave 2 U_tt^-1 into Y's diagonal block t (row pitch npad) and
-// wave 3 dg[k] = u_kk, dg[npad + k] = -1 / u_kk for the tile's rows k.
+// Row R of the substitutions for column `lane` of a 64 x 64 triangular
+// factor's inverse (sT rows: L below the diagonal, unit; U on and above it),
+// unrolled through integer sequences so that y stays in registers.
+template <int R>
+__device__ __forceinline__ void blk_fwd_row(const float (*sT)[68], int lane, float (&y)[kDW]) {
+  float acc = (R == lane) ? 1.0f : 0.0f;
+#pragma unroll
+  for (int m = 0; m < R; ++m) acc -= sT[R][m] * y[m];
+  y[R] = acc;
+}
+template <int... Rs>
+__device__ __forceinline__ void blk_fwd(std::integer_sequence<int, Rs...>, const float (*sT)[68], int lane,
+                                        float (&y)[kDW]) {
+  (blk_fwd_row<Rs>(sT, lane, y), ...);
+}
+template <int R>
+__device__ __forceinline__ void blk_bwd_row(const float (*sT)[68], int lane, float (&y)[kDW]) {
+  float acc = (R == lane) ? 1.0f : 0.0f;
+#pragma unroll
+  for (int m = R + 1; m < kDW; ++m) acc -= sT[R][m] * y[m];
+  y[R] = acc / sT[R][R];
+}
+template <int... Rs>
+__device__ __forceinline__ void blk_bwd(std::integer_sequence<int, Rs...>, const float (*sT)[68], int lane,
+                                        float (&y)[kDW]) {
+  (blk_bwd_row<kDW - 1 - Rs>(sT, lane, y), ...);
+}
+
+// Factor of the 64 x 64 tile held in sT (rows) at block tile t, by a
+// 256-thread workgroup: wave 0 eliminates with register rows and readlane
+// pivot-row broadcasts (the diagonal-pivot leaf's code), stores the LU rows
+// into W, ipiv = identity, and folds its largest multiplier (a zero pivot
+// counts as +inf) into *growth; then wave 1 forms L_tt^-1 (into inv, and its
+// two 32 x 32 diagonal blocks into linv for the strip solves of U12 outside
+// the block), wave 2 U_tt^-1 (into inv + 4096 and Y's diagonal block t, row
+// pitch npad), wave 3 dg[k] = u_kk, dg[npad + k] = -1 / u_kk.
 __device__ __forceinline__ void blk_factor_tile(float (*sT)[68], float* __restrict__ W, long long ld, long long c0,
-                                                int t, float* __restrict__ Y, int npad, float* __restrict__ dg,
-                                                float* __restrict__ linv, int* __restrict__ ipiv,
-                                                unsigned* __restrict__ growth) {
+                                                int t, float* __restrict__ inv, float* __restrict__ Y, int npad,
+                                                float* __restrict__ dg, float* __restrict__ linv,
+                                                int* __restrict__ ipiv, unsigned* __restrict__ growth) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const long long g0 = c0 + 64LL * t;
   if (wave == 0) {
@@ -1681,141 +1775,144 @@ __device__ __forceinline__ void blk_factor_tile(float (*sT)[68], float* __restri
   }
   __syncthreads();
   if (wave == 1) {
-    // lane c: column c % 32 of the inverse of L's diagonal block c / 32
-    const int blk = lane >> 5, cc = lane & 31, o = 32 * blk;
-    float y[32];
+    // lane c: column c of L_tt^-1 by forward substitution (unit diagonal)
+    float y[kDW];
+    blk_fwd(std::make_integer_sequence<int, kDW>{}, sT, lane, y);
 #pragma unroll
-    for (int r = 0; r < 32; ++r) {
-      float acc = (r == cc) ? 1.0f : 0.0f;
+    for (int r = 0; r < kDW; ++r) inv[r * kDW + lane] = y[r];
+    // the 32 x 32 diagonal blocks of L^-1 are the inverses of L's
+    // (two guarded loops, so that y is indexed by constants only)
+    const int cc = lane & 31;
+    float* out = linv + (g0 / kTsW) * kTsW * kTsW;
+    if (lane < 32) {
 #pragma unroll
-      for (int m = 0; m < r; ++m) acc -= sT[o + r][o + m] * y[m];
-      y[r] = acc;
+      for (int r = 0; r < 32; ++r) out[r * 32 + cc] = y[r];
+    } else {
+#pragma unroll
+      for (int r = 0; r < 32; ++r) out[1024 + r * 32 + cc] = y[32 + r];
     }
-    float* out = linv + (g0 / kTsW) * kTsW * kTsW + blk * 1024;
-#pragma unroll
-    for (int r = 0; r < 32; ++r) out[r * 32 + cc] = y[r];
-  } else if (wave == 2 && Y) {
+  } else if (wave == 2) {
     // lane c: column c of U_tt^-1 by back substitution
     float y[kDW];
+    blk_bwd(std::make_integer_sequence<int, kDW>{}, sT, lane, y);
 #pragma unroll
-    for (int r = kDW - 1; r >= 0; --r) {
-      float acc = (r == lane) ? 1.0f : 0.0f;
-#pragma unroll
-      for (int m = r + 1; m < kDW; ++m) acc -= sT[r][m] * y[m];
-      y[r] = acc / sT[r][r];
-    }
+    for (int r = 0; r < kDW; ++r) inv[kDW * kDW + r * kDW + lane] = y[r];
     float* out = Y + (64LL * t) * npad + 64LL * t + lane;
 #pragma unroll
     for (int r = 0; r < kDW; ++r) out[(long long)r * npad] = y[r];
-  } else if (wave == 3 && dg) {
+  } else if (wave == 3) {
     const float d = sT[lane][lane];
     dg[64 * t + lane] = d;
     dg[npad + 64 * t + lane] = -1.0f / d;
   }
 }
 
-// Step k's multipliers and U rows inside the block: workgroups [0, nb) take
-// the rows of tile column k below the diagonal (one row per thread: x =
-// a U_kk^-1, U from W's factored tile by LDS broadcast), the others the
-// columns of tile row k right of it (x = L_kk^-1 a).  nb = the workgroups of
-// either half.  The largest |l| goes to *growth.
-__global__ __launch_bounds__(256) void k_blk_solve(float* __restrict__ W, long long ld, long long c0, int jb, int k,
-                                                   int nb, unsigned* __restrict__ growth) {
-  __shared__ __attribute__((aligned(16))) float sU[kDW][kDW];
-  __shared__ float sR[kDW];
-  __shared__ unsigned wmax[4];
-  const int tid = threadIdx.x;
-  const bool rows = blockIdx.x < (unsigned)nb;
-  const int q = rows ? blockIdx.x : blockIdx.x - nb;
-  const long long d0 = c0 + 64LL * k;            // the diagonal tile's first row and column
-  const int rel = 64 * (k + 1) + q * 256 + tid;  // row (or column) in the block
-  const bool act = rel < jb;
-  float x[kDW];
-  if (act) {
-    if (rows) {
-      dw_load_row(W, ld, c0 + rel, d0, kDW, x);
-    } else {
-#pragma unroll
-      for (int i = 0; i < kDW; ++i) x[i] = W[(d0 + i) * ld + c0 + rel];
-    }
-  }
-  // rows: sU[i][j] = U_kk (upper part); columns: sU[m][i] = L_kk[i][m] (the
-  // unit lower part, transposed so that column m's entries are contiguous)
-#pragma unroll
-  for (int p = 0; p < kDW * kDW / 256; ++p) {
-    const int i = (tid + 256 * p) / kDW, j = (tid + 256 * p) % kDW;
-    const float v = W[(d0 + i) * ld + d0 + j];
-    if (rows) sU[i][j] = (j >= i) ? v : 0.0f;
-    else sU[j][i] = (j < i) ? v : 0.0f;
-  }
-  __syncthreads();
-  if (rows && tid < kDW) {
-    const float d = sU[tid][tid];
-    sR[tid] = (d != 0.0f) ? 1.0f / d : 0.0f;
-  }
-  __syncthreads();
-  float g = 0.0f;
-  if (act) {
-    if (rows) {
-      dwf_all(std::make_integer_sequence<int, kDW>{}, x, kDW, sU, sR);
-#pragma unroll
-      for (int j = 0; j < kDW; ++j) g = abs_max_keep_nan(g, x[j]);
-      float* dst = W + (c0 + rel) * ld + d0;
-#pragma unroll
-      for (int j4 = 0; j4 < kDW / 4; ++j4)
-        *reinterpret_cast<float4*>(dst + 4 * j4) = make_float4(x[4 * j4], x[4 * j4 + 1], x[4 * j4 + 2], x[4 * j4 + 3]);
-    } else {
-#pragma unroll
-      for (int m = 0; m < kDW; ++m) {
-        const float4* l4 = reinterpret_cast<const float4*>(sU[m]);
-#pragma unroll
-        for (int i4 = (m + 1) / 4; i4 < kDW / 4; ++i4) {
-          const float4 v = l4[i4];
-          if (4 * i4 > m) x[4 * i4] -= x[m] * v.x;
-          if (4 * i4 + 1 > m) x[4 * i4 + 1] -= x[m] * v.y;
-          if (4 * i4 + 2 > m) x[4 * i4 + 2] -= x[m] * v.z;
-          if (4 * i4 + 3 > m) x[4 * i4 + 3] -= x[m] * v.w;
-        }
-      }
-#pragma unroll
-      for (int i = 0; i < kDW; ++i) W[(d0 + i) * ld + c0 + rel] = x[i];
-    }
-  }
-  ts_growth_max_wg(g, growth, wmax);
-}
-
-// Step k's trailing update inside the block: tile (k+1+i, k+1+j) -= L_{k+1+i,k}
-// U_{k,k+1+j} for the nt x nt trailing tiles (workgroup b: i = b / nt, j = b %
-// nt), FP32 MFMA 16x16x4, each wave a 32 x 32 quarter; workgroup 0 (tile
-// (k+1, k+1)) keeps its result in LDS and factors it.  k = -1: one
-// workgroup, tile 0 factored as loaded.
-__global__ __launch_bounds__(256) void k_blk_update(float* __restrict__ W, long long ld, long long c0, int k, int nt,
-                                                    float* __restrict__ Y, int npad, float* __restrict__ dg,
-                                                    float* __restrict__ linv, int* __restrict__ ipiv,
-                                                    unsigned* __restrict__ growth) {
-  __shared__ __attribute__((aligned(16))) float sA[kDW][68];  // L tile transposed: sA[kk][m]; then the tile
-  __shared__ __attribute__((aligned(16))) float sB[kDW][68];  // U tile: sB[kk][n]
-  const int tid = threadIdx.x;
-  const int ti = k + 1 + (int)blockIdx.x / nt, tj = k + 1 + (int)blockIdx.x % nt;
-  if (k < 0) {
-#pragma unroll
-    for (int p = 0; p < kDW * kDW / 1024; ++p) {
-      const int idx = tid + 256 * p, r = idx / 16, c4 = (idx % 16) * 4;
-      *reinterpret_cast<float4*>(&sA[r][c4]) = *reinterpret_cast<const float4*>(W + (c0 + r) * ld + c0 + c4);
-    }
-    __syncthreads();
-    blk_factor_tile(sA, W, ld, c0, 0, Y, npad, dg, linv, ipiv, growth);
-    return;
-  }
-  const long long rA = c0 + 64LL * ti, cA = c0 + 64LL * k;  // L_{ti,k}
-  const long long rB = c0 + 64LL * k, cB = c0 + 64LL * tj;  // U_{k,tj}
-  float4 va[4], vb[4];
+// 64 x 64 tile (row pitch ld) into LDS rows, and transposed (sX[c][r]).
+__device__ __forceinline__ void blk_load(const float* __restrict__ src, long long ld, float4 (&v)[4]) {
 #pragma unroll
   for (int p = 0; p < 4; ++p) {
-    const int idx = tid + 256 * p, r = idx / 16, c4 = (idx % 16) * 4;
-    va[p] = *reinterpret_cast<const float4*>(W + (rA + r) * ld + cA + c4);
-    vb[p] = *reinterpret_cast<const float4*>(W + (rB + r) * ld + cB + c4);
+    const int idx = threadIdx.x + 256 * p, r = idx / 16, c4 = (idx % 16) * 4;
+    v[p] = *reinterpret_cast<const float4*>(src + r * ld + c4);
   }
+}
+__device__ __forceinline__ void blk_put(float (*sX)[68], const float4 (&v)[4]) {
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const int idx = threadIdx.x + 256 * p, r = idx / 16, c4 = (idx % 16) * 4;
+    *reinterpret_cast<float4*>(&sX[r][c4]) = v[p];
+  }
+}
+__device__ __forceinline__ void blk_put_t(float (*sX)[68], const float4 (&v)[4]) {
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const int idx = threadIdx.x + 256 * p, r = idx / 16, c4 = (idx % 16) * 4;
+    sX[c4][r] = v[p].x;
+    sX[c4 + 1][r] = v[p].y;
+    sX[c4 + 2][r] = v[p].z;
+    sX[c4 + 3][r] = v[p].w;
+  }
+}
+
+// acc (this wave's 32 x 32 quarter at (wm, wn)) = sAt^T sB over k < 64, with
+// sAt[k][m] and sB[k][n]; MFMA f32 16x16x4.
+__device__ __forceinline__ void blk_mm(const float (*sAt)[68], const float (*sB)[68], int wm, int wn, int lr, int lk,
+                                       f4 (&acc)[2][2]) {
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = (f4){0, 0, 0, 0};
+#pragma unroll
+  for (int kk = 0; kk < kDW; kk += 4) {
+    float a[2], b[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) a[i] = sAt[kk + lk][wm + 16 * i + lr];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) b[j] = sB[kk + lk][wn + 16 * j + lr];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
+  }
+}
+
+// Step buffers of a diagonal-block factorisation (floats): [2][T][4096]
+// copies of tile column k, the same for tile row k, [2][2][4096] inverses of
+// tile k's factors (L, then U).
+struct BlkBufs {
+  float* col;
+  float* row;
+  float* inv;
+  int T;
+};
+
+// Step k of the diagonal block (see above); k = -1: workgroup 0 factors tile
+// 0, workgroups 1 .. T-1 copy tile column 0 and T .. 2T-2 tile row 0 into the
+// step buffers.  nt = T - 1 - k trailing tiles per side, workgroup b takes
+// tile (k+1 + b / nt, k+1 + b % nt).
+__global__ __launch_bounds__(256, 1) void k_blk_step(float* __restrict__ W, long long ld, long long c0, int k, int nt,
+                                                  BlkBufs bb, float* __restrict__ Y, int npad, float* __restrict__ dg,
+                                                  float* __restrict__ linv, int* __restrict__ ipiv,
+                                                  unsigned* __restrict__ growth) {
+  __shared__ __attribute__((aligned(16))) float s0[kDW][68];
+  __shared__ __attribute__((aligned(16))) float s1[kDW][68];
+  __shared__ __attribute__((aligned(16))) float s2[kDW][68];
+  __shared__ __attribute__((aligned(16))) float s3[kDW][68];
+  const int tid = threadIdx.x;
+  constexpr int TS = kDW * kDW;
+  if (k < 0) {
+    const int T = bb.T;
+    if (blockIdx.x == 0) {
+      float4 v[4];
+      blk_load(W + c0 * ld + c0, ld, v);
+      blk_put(s0, v);
+      __syncthreads();
+      blk_factor_tile(s0, W, ld, c0, 0, bb.inv, Y, npad, dg, linv, ipiv, growth);
+      return;
+    }
+    // tile (i, 0) or (0, j): row segments of 16 floats, one float4 per thread
+    // and pass
+    const bool colt = (int)blockIdx.x < T;
+    const int t = colt ? (int)blockIdx.x : (int)blockIdx.x - T + 1;
+    const float* src = colt ? W + (c0 + 64LL * t) * ld + c0 : W + c0 * ld + c0 + 64LL * t;
+    float4* dst = reinterpret_cast<float4*>((colt ? bb.col : bb.row) + (size_t)t * TS);
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const int idx = tid + 256 * p, r = idx / 16, c4 = (idx % 16) * 4;
+      dst[idx] = *reinterpret_cast<const float4*>(src + r * ld + c4);
+    }
+    return;
+  }
+  const int par = k & 1, T = bb.T;
+  const int ti = k + 1 + (int)blockIdx.x / nt, tj = k + 1 + (int)blockIdx.x % nt;
+  const float* cur_inv = bb.inv + (size_t)par * 2 * TS;
+  const float* cur_col = bb.col + ((size_t)par * T + ti) * TS;
+  const float* cur_row = bb.row + ((size_t)par * T + tj) * TS;
+  // every load in flight before the LDS stores
+  float4 va[4], vu[4], vl[4], vb[4];
+  blk_load(cur_col, kDW, va);        // A_ik
+  blk_load(cur_inv + TS, kDW, vu);   // U_kk^-1
+  blk_load(cur_inv, kDW, vl);        // L_kk^-1
+  blk_load(cur_row, kDW, vb);        // A_kj
   const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   const long long rC = c0 + 64LL * ti, cC = c0 + 64LL * tj;
@@ -1826,52 +1923,84 @@ __global__ __launch_bounds__(256) void k_blk_update(float* __restrict__ W, long 
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) cv[i][j][r] = W[(rC + wm + 16 * i + 4 * lk + r) * ld + cC + wn + 16 * j + lr];
-#pragma unroll
-  for (int p = 0; p < 4; ++p) {
-    const int idx = tid + 256 * p, r = idx / 16, c4 = (idx % 16) * 4;
-    sA[c4][r] = va[p].x;
-    sA[c4 + 1][r] = va[p].y;
-    sA[c4 + 2][r] = va[p].z;
-    sA[c4 + 3][r] = va[p].w;
-    *reinterpret_cast<float4*>(&sB[r][c4]) = vb[p];
-  }
+  blk_put_t(s0, va);
+  blk_put(s1, vu);
+  blk_put_t(s2, vl);
+  blk_put(s3, vb);
   __syncthreads();
-  f4 acc[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = (f4){0, 0, 0, 0};
-#pragma unroll
-  for (int kk = 0; kk < kDW; kk += 4) {
-    float a[2], b[2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i) a[i] = sA[kk + lk][wm + 16 * i + lr];
-#pragma unroll
-    for (int j = 0; j < 2; ++j) b[j] = sB[kk + lk][wn + 16 * j + lr];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
-  }
-  if (blockIdx.x != 0) {
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j)
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          W[(rC + wm + 16 * i + 4 * lk + r) * ld + cC + wn + 16 * j + lr] = cv[i][j][r] - acc[i][j][r];
-    return;
-  }
-  __syncthreads();  // every wave is done reading sA
+  f4 accL[2][2], accU[2][2];
+  blk_mm(s0, s1, wm, wn, lr, lk, accL);  // L_ik = A_ik U_kk^-1
+  blk_mm(s2, s3, wm, wn, lr, lk, accU);  // U_kj = L_kk^-1 A_kj
+  __syncthreads();
+  // L_ik as the next product's transposed left operand, U_kj as its right one
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) sA[wm + 16 * i + 4 * lk + r][wn + 16 * j + lr] = cv[i][j][r] - acc[i][j][r];
+      for (int r = 0; r < 4; ++r) {
+        s0[wn + 16 * j + lr][wm + 16 * i + 4 * lk + r] = accL[i][j][r];
+        s1[wm + 16 * i + 4 * lk + r][wn + 16 * j + lr] = accU[i][j][r];
+      }
+  if (tj == k + 1) {
+    // tile (ti, k) of W takes L_ik
+    float g = 0.0f;
+    const long long rL = c0 + 64LL * ti, cL = c0 + 64LL * k;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          W[(rL + wm + 16 * i + 4 * lk + r) * ld + cL + wn + 16 * j + lr] = accL[i][j][r];
+          g = abs_max_keep_nan(g, accL[i][j][r]);
+        }
+    ts_growth_max(g, growth);
+  }
+  if (ti == k + 1) {
+    // tile (k, tj) of W takes U_kj
+    const long long rU = c0 + 64LL * k, cU = c0 + 64LL * tj;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) W[(rU + wm + 16 * i + 4 * lk + r) * ld + cU + wn + 16 * j + lr] = accU[i][j][r];
+  }
   __syncthreads();
-  blk_factor_tile(sA, W, ld, c0, k + 1, Y, npad, dg, linv, ipiv, growth);
+  f4 acc[2][2];
+  blk_mm(s0, s1, wm, wn, lr, lk, acc);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) cv[i][j][r] -= acc[i][j][r];
+  if (blockIdx.x == 0) {
+    // tile (k+1, k+1): factor the result
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) s2[wm + 16 * i + 4 * lk + r][wn + 16 * j + lr] = cv[i][j][r];
+    __syncthreads();
+    blk_factor_tile(s2, W, ld, c0, k + 1, bb.inv + (size_t)(par ^ 1) * 2 * TS, Y, npad, dg, linv, ipiv, growth);
+    return;
+  }
+  float* ncol = (tj == k + 1) ? bb.col + ((size_t)(par ^ 1) * T + ti) * TS : nullptr;
+  float* nrow = (ti == k + 1) ? bb.row + ((size_t)(par ^ 1) * T + tj) * TS : nullptr;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int rr = wm + 16 * i + 4 * lk + r, cc = wn + 16 * j + lr;
+        W[(rC + rr) * ld + cC + cc] = cv[i][j][r];
+        if (ncol) ncol[rr * kDW + cc] = cv[i][j][r];
+        if (nrow) nrow[rr * kDW + cc] = cv[i][j][r];
+      }
 }
 
 // Identity in Y's diagonal blocks [t0, npad / 64) (the padding of U11^-1).
@@ -2077,6 +2206,9 @@ struct LuCtx {
   bool diag = false;  // with calu: the leaves take their diagonal pivots instead (checked by growth)
   const double* src = nullptr;  // column-major FP64 A that the factorisation first converts into W
   long long src_ld = 0;
+  // the current step's L21 in FP16 ([rows][Kp], from its panel's product):
+  // trailing updates read it instead of converting L21 (nullptr: convert)
+  const _Float16* a16 = nullptr;
 };
 
 // The FP16 update C -= (A16 B16^T) * colscale on stream st; max_wgs > 0
@@ -2098,29 +2230,36 @@ static long long wide_gemm_min_tiles() {
   return v;
 }
 
+// o16.h non-null: the result also goes to o16 as FP16, its largest |c| to
+// o16.growth (kernel role 2).
 static int launch_f16_update(qd_context* h, bool trailing, long long M, long long N, int Kp, const _Float16* A16,
                              const _Float16* B16, const float* colscale, float* C, long long ldc, int rows16,
-                             int max_wgs, hipStream_t st) {
+                             int max_wgs, hipStream_t st, Out16 o16 = Out16{nullptr, 0, nullptr}) {
   const long long wmin = wide_gemm_min_tiles();
   const long long wtiles = ((N + w16::BN - 1) / w16::BN) * ((M + w16::BM - 1) / w16::BM);
   // the wide kernel takes K steps of 64 and 32-bit element offsets into A16 / B16
   if (wmin > 0 && wtiles >= wmin && max_wgs <= 0 && Kp % w16::BK == 0 &&
       std::max(M, N) * (long long)Kp < (1LL << 32)) {
-    if (trailing)
+    if (o16.h)
+      hipLaunchKernelGGL(k_gemm16w<2>, dim3((unsigned)wtiles), dim3(256), 0, st, M, N, Kp, A16, B16, colscale, C, ldc,
+                         rows16, o16);
+    else if (trailing)
       hipLaunchKernelGGL(k_gemm16w<1>, dim3((unsigned)wtiles), dim3(256), 0, st, M, N, Kp, A16, B16, colscale, C, ldc,
-                         rows16);
+                         rows16, o16);
     else
       hipLaunchKernelGGL(k_gemm16w<0>, dim3((unsigned)wtiles), dim3(256), 0, st, M, N, Kp, A16, B16, colscale, C, ldc,
-                         rows16);
+                         rows16, o16);
     QD_HIP(h, hipGetLastError());
     return QD_OK;
   }
   const long long tiles = ((N + H_BN - 1) / H_BN) * ((M + H_BM - 1) / H_BM);
   const dim3 g((unsigned)(max_wgs > 0 ? std::min<long long>(tiles, max_wgs) : tiles));
-  if (trailing)
-    hipLaunchKernelGGL(k_gemm16<1>, g, dim3(256), 0, st, M, N, Kp, A16, B16, colscale, C, ldc, rows16);
+  if (o16.h)
+    hipLaunchKernelGGL(k_gemm16<2>, g, dim3(256), 0, st, M, N, Kp, A16, B16, colscale, C, ldc, rows16, o16);
+  else if (trailing)
+    hipLaunchKernelGGL(k_gemm16<1>, g, dim3(256), 0, st, M, N, Kp, A16, B16, colscale, C, ldc, rows16, o16);
   else
-    hipLaunchKernelGGL(k_gemm16<0>, g, dim3(256), 0, st, M, N, Kp, A16, B16, colscale, C, ldc, rows16);
+    hipLaunchKernelGGL(k_gemm16<0>, g, dim3(256), 0, st, M, N, Kp, A16, B16, colscale, C, ldc, rows16, o16);
   QD_HIP(h, hipGetLastError());
   return QD_OK;
 }
@@ -2179,6 +2318,9 @@ static int gemm_sub(LuCtx<T>& c, long long M, long long Nn, int K, const T* A, c
       const int va = ((reinterpret_cast<uintptr_t>(A) & 15) == 0 && c.ld % 4 == 0 && K % 8 == 0) ? 1 : 0;
       hipLaunchKernelGGL(k_scale_cvt_b16t, dim3((unsigned)nb), dim3(256), 0, c.st, K, Nn, Kp, B, c.ld,
                          c.sc->colscale.p, c.sc->h16b.p, vb);
+      // a trailing update with its panel's FP16 L21 at hand converts nothing
+      const _Float16* a16 = (trailing && c.a16) ? c.a16 : c.sc->h16a.p;
+      if (trailing && c.a16) reuse_a16 = true;
       if (!reuse_a16 && growth)
         hipLaunchKernelGGL(k_cvt_a16_growth, dim3((unsigned)((M * (Kp / 8) + 255) / 256)), dim3(256), 0, c.st, M,
                            K, Kp, A, c.ld, c.sc->h16a.p, va, growth);
@@ -2187,8 +2329,8 @@ static int gemm_sub(LuCtx<T>& c, long long M, long long Nn, int K, const T* A, c
                            Kp, A, c.ld, c.sc->h16a.p, va);
       ProfScope ps(c.h, trailing ? QD_PROF_LU_TRAILING_GEMM : -1, 2.0 * (double)M * (double)Nn * K, c.st);
       const int rows16 = (c.ld % 4 == 0 && (reinterpret_cast<uintptr_t>(C) & 15) == 0) ? 1 : 0;
-      if ((rc_ = launch_f16_update(c.h, trailing, M, Nn, Kp, c.sc->h16a.p, c.sc->h16b.p, c.sc->colscale.p, C, c.ld,
-                                   rows16, max_wgs, c.st)))
+      if ((rc_ = launch_f16_update(c.h, trailing, M, Nn, Kp, a16, c.sc->h16b.p, c.sc->colscale.p, C, c.ld, rows16,
+                                   max_wgs, c.st)))
         return rc_;
       QD_HIP(c.h, hipGetLastError());
       return QD_OK;
@@ -2356,7 +2498,7 @@ static bool block_panels() {
 }
 
 // Diagonal-pivot panel [j, j+jb) with rows below it (jb a multiple of 64):
-// the jb x jb diagonal block factored in 64-column steps of two launches,
+// the jb x jb diagonal block factored in 64-column steps of one launch each,
 // U11^-1 (whose diagonal blocks the steps leave in Y) completed by recursive
 // doubling, then L21 = A21 U11^-1 in one FP16 product into W.  The growth of
 // L21 is taken by the trailing update's conversion of it (update_cols).
@@ -2371,22 +2513,20 @@ static int panel_block(LuCtx<float>& c, long long j, int jb) {
     return set_error(c.h, QD_ERR_STATE, "diagonal-block panel of %d columns (a multiple of 64 from 128 up)", jb);
   if (s->binv.count < (size_t)npad * npad || s->binv_t.count < (size_t)npad * npad / 4 ||
       s->bdiag.count < (size_t)2 * npad || sc.h16a.count < (size_t)(M * Kp) || sc.h16b.count < (size_t)jb * Kp ||
-      sc.colscale.count < (size_t)nb)
+      sc.colscale.count < (size_t)nb || s->bstep.count < (size_t)(4 * T + 4) * 4096)
     return set_error(c.h, QD_ERR_STATE, "diagonal-block panel buffers sized for fewer than %d columns", jb);
   float* Y = s->binv.p;
   float* Tm = s->binv_t.p;
   float* dg = s->bdiag.p;
   float* linv = s->ts_linv.p;
-  // the diagonal block: tile 0, then per step k the solves of tile row and
-  // column k and the trailing update with the factorisation of tile k+1
-  hipLaunchKernelGGL(k_blk_update, dim3(1), dim3(256), 0, c.st, c.W, c.ld, j, -1, 1, Y, npad, dg, linv,
-                     s->ipiv_dev.p, s->growth.p);
+  // the diagonal block: tile 0 (with the copies of tile column and row 0),
+  // then one launch per step
+  BlkBufs bb{s->bstep.p, s->bstep.p + (size_t)2 * T * 4096, s->bstep.p + (size_t)4 * T * 4096, T};
+  hipLaunchKernelGGL(k_blk_step, dim3((unsigned)(2 * T - 1)), dim3(256), 0, c.st, c.W, c.ld, j, -1, 0, bb, Y, npad, dg,
+                     linv, s->ipiv_dev.p, s->growth.p);
   for (int k = 0; k + 1 < T; ++k) {
-    const int rest = jb - 64 * (k + 1), nt = T - 1 - k;
-    const int hb = (rest + 255) / 256;
-    hipLaunchKernelGGL(k_blk_solve, dim3((unsigned)(2 * hb)), dim3(256), 0, c.st, c.W, c.ld, j, jb, k, hb,
-                       s->growth.p);
-    hipLaunchKernelGGL(k_blk_update, dim3((unsigned)(nt * nt)), dim3(256), 0, c.st, c.W, c.ld, j, k, nt, Y, npad, dg,
+    const int nt = T - 1 - k;
+    hipLaunchKernelGGL(k_blk_step, dim3((unsigned)(nt * nt)), dim3(256), 0, c.st, c.W, c.ld, j, k, nt, bb, Y, npad, dg,
                        linv, s->ipiv_dev.p, s->growth.p);
   }
   QD_HIP(c.h, hipGetLastError());
@@ -2424,7 +2564,25 @@ static int panel_block(LuCtx<float>& c, long long j, int jb) {
                      c.ld, (const float*)(dg + npad), sc.h16a.p, va);
   QD_HIP(c.h, hipGetLastError());
   const int rows16 = (c.ld % 4 == 0 && (reinterpret_cast<uintptr_t>(A21) & 15) == 0) ? 1 : 0;
-  return launch_f16_update(c.h, false, M, jb, Kp, sc.h16a.p, sc.h16b.p, sc.colscale.p, A21, c.ld, rows16, 0, c.st);
+  // the product also leaves L21 in FP16 for the trailing updates, with its
+  // growth (the panels alternate between two copies: the main stream still
+  // reads the previous panel's while this one is written)
+  const int b = s->l21h_next;
+  Out16 o16{nullptr, 0, nullptr};
+  if (s->l21h[b].count >= (size_t)(M * Kp)) {
+    o16 = Out16{s->l21h[b].p, (long long)Kp, s->growth.p};
+    s->l21h_col[b] = j;
+    s->l21h_next = b ^ 1;
+  }
+  return launch_f16_update(c.h, false, M, jb, Kp, sc.h16a.p, sc.h16b.p, sc.colscale.p, A21, c.ld, rows16, 0, c.st,
+                           o16);
+}
+
+// The FP16 L21 of the panel at column j, if its product wrote one.
+static const _Float16* l21_copy(const qd_lu_state* s, long long j) {
+  for (int b = 0; b < 2; ++b)
+    if (s->l21h_col[b] == j) return s->l21h[b].p;
+  return nullptr;
 }
 
 // Panel of columns [j, j+jb) on the context's stream, timed as one lu_panel.
@@ -2482,17 +2640,18 @@ static int update_split(LuCtx<T>& c, long long j, int jb, long long c0, long lon
                      c.sc->h16b.p, vb);
   float* C = reinterpret_cast<float*>(c.W + r0 * c.ld + c0);
   const int rows16 = (c.ld % 4 == 0 && (reinterpret_cast<uintptr_t>(C) & 15) == 0) ? 1 : 0;
+  const _Float16* a16 = c.a16 ? c.a16 : c.sc->h16a.p;
   int rc;
   {
     ProfScope ps(c.h, QD_PROF_LU_TRAILING_GEMM, 2.0 * (double)split * (double)Nn * K, c.st);
-    if ((rc = launch_f16_update(c.h, true, split, Nn, Kp, c.sc->h16a.p, c.sc->h16b.p, c.sc->colscale.p, C, c.ld,
-                                rows16, 0, c.st)))
+    if ((rc = launch_f16_update(c.h, true, split, Nn, Kp, a16, c.sc->h16b.p, c.sc->colscale.p, C, c.ld, rows16, 0,
+                                c.st)))
       return rc;
   }
   QD_HIP(c.h, hipEventRecord(ev, c.st));
   {
     ProfScope ps(c.h, QD_PROF_LU_TRAILING_GEMM, 2.0 * (double)(M - split) * (double)Nn * K, c.st);
-    if ((rc = launch_f16_update(c.h, true, M - split, Nn, Kp, c.sc->h16a.p + (size_t)split * Kp, c.sc->h16b.p,
+    if ((rc = launch_f16_update(c.h, true, M - split, Nn, Kp, a16 + (size_t)split * Kp, c.sc->h16b.p,
                                 c.sc->colscale.p, C + split * c.ld, c.ld, rows16, 0, c.st)))
       return rc;
   }
@@ -2589,6 +2748,9 @@ static int factor(LuCtx<T>& c, const PanelWidths& pw) {
       QD_ALLOC(c.h, s->binv, np * np);
       QD_ALLOC(c.h, s->binv_t, np * np / 4);
       QD_ALLOC(c.h, s->bdiag, 2 * np);
+      QD_ALLOC(c.h, s->bstep, (size_t)(4 * (NB / 64) + 4) * 4096);
+      QD_ALLOC(c.h, s->l21h[0], (size_t)N * Kp);
+      QD_ALLOC(c.h, s->l21h[1], (size_t)N * Kp);
       QD_ALLOC(c.h, s->sc_l21.h16a, (size_t)N * Kp);
       QD_ALLOC(c.h, s->sc_l21.h16b, (size_t)NB * Kp);
       QD_ALLOC(c.h, s->sc_l21.colscale, (size_t)(NB / 64 + 2));
@@ -2632,6 +2794,7 @@ static int factor_steps(LuCtx<T>& c, const PanelWidths& pw, bool look) {
   ca.sc_solve = &s->sc_u12;
   bool u12_ready = false;  // this step's U12 beyond the next panel is solved (ev_u12)
   QD_HIP(c.h, hipMemsetAsync(s->piv_state.p, 0, 2 * sizeof(int), c.st));
+  s->l21h_col[0] = s->l21h_col[1] = -1;  // no FP16 L21 of this factorisation yet
   int rc;
   const int jb0 = pw.at(0, N);
   if (c.src) {
@@ -2668,6 +2831,7 @@ static int factor_steps(LuCtx<T>& c, const PanelWidths& pw, bool look) {
   for (long long j = 0, step = 0; j < N; j += step) {
     const int jb = pw.at(j, N);
     step = jb;
+    c.a16 = c.f16 ? l21_copy(s, j) : nullptr;
     const long long nj = j + jb;
     const int njb = nj < N ? pw.at(nj, N) : 0;
     // interchanges on the columns left and right of the panel (none with
