@@ -66,7 +66,8 @@ struct qd_lu_state {
   F16Scratch sc_u12;    // those of the aux stream's U12 solve, which can overlap the main one
   F16Scratch sc_l21;    // the product L21 = A21 U11^-1 of a diagonal-block panel (aux stream)
   qd::DevBuf<float> binv;   // a diagonal-block panel's U11^-1 (npad x npad)
-  qd::DevBuf<float> binv_t; // its recursive doubling's intermediate products
+  qd::DevBuf<float> binvl;  // and its L11^-1
+  qd::DevBuf<float> binv_t; // their recursive doubling's intermediate products
   qd::DevBuf<float> bdiag;  // u_kk, then -1 / u_kk (2 x npad)
   qd::DevBuf<float> bstep;  // step buffers of the diagonal block's 64-column steps
   // FP16 copies of the L21 that the last two diagonal-block panels wrote (the
@@ -74,6 +75,12 @@ struct qd_lu_state {
   qd::DevBuf<_Float16> l21h[2];
   long long l21h_col[2] = {-1, -1};
   int l21h_next = 0;
+  // likewise the strictly lower part of L11^-1, negated, in FP16 ([jb][Kp]),
+  // for the U12 solves of those panels' steps, with its panel's first column
+  // and width
+  qd::DevBuf<_Float16> linvh[2];
+  long long linvh_col[2] = {-1, -1};
+  int linvh_w[2] = {0, 0};
   hipStream_t aux = nullptr;     // panel stream of the look-ahead (non-blocking, default priority)
   hipEvent_t ev_cols = nullptr, ev_panel = nullptr;
   hipEvent_t ev_top = nullptr, ev_u12 = nullptr;  // split steps: next U12 rows updated / solved
@@ -1748,14 +1755,16 @@ __device__ __forceinline__ void blk_bwd(std::integer_sequence<int, Rs...>, const
 // 256-thread workgroup: wave 0 eliminates with register rows and readlane
 // pivot-row broadcasts (the diagonal-pivot leaf's code), stores the LU rows
 // into W, ipiv = identity, and folds its largest multiplier (a zero pivot
-// counts as +inf) into *growth; then wave 1 forms L_tt^-1 (into inv, and its
-// two 32 x 32 diagonal blocks into linv for the strip solves of U12 outside
-// the block), wave 2 U_tt^-1 (into inv + 4096 and Y's diagonal block t, row
-// pitch npad), wave 3 dg[k] = u_kk, dg[npad + k] = -1 / u_kk.
+// counts as +inf) into *growth; then wave 1 forms L_tt^-1 (into inv, Z's
+// diagonal block t, and its two 32 x 32 diagonal blocks into linv for the
+// strip solves of U12 outside the block), wave 2 U_tt^-1 (into inv + 4096
+// and Y's diagonal block t; Y and Z have row pitch npad), wave 3 dg[k] =
+// u_kk, dg[npad + k] = -1 / u_kk.
 __device__ __forceinline__ void blk_factor_tile(float (*sT)[68], float* __restrict__ W, long long ld, long long c0,
-                                                int t, float* __restrict__ inv, float* __restrict__ Y, int npad,
-                                                float* __restrict__ dg, float* __restrict__ linv,
-                                                int* __restrict__ ipiv, unsigned* __restrict__ growth) {
+                                                int t, float* __restrict__ inv, float* __restrict__ Y,
+                                                float* __restrict__ Z, int npad, float* __restrict__ dg,
+                                                float* __restrict__ linv, int* __restrict__ ipiv,
+                                                unsigned* __restrict__ growth) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const long long g0 = c0 + 64LL * t;
   if (wave == 0) {
@@ -1780,6 +1789,9 @@ __device__ __forceinline__ void blk_factor_tile(float (*sT)[68], float* __restri
     blk_fwd(std::make_integer_sequence<int, kDW>{}, sT, lane, y);
 #pragma unroll
     for (int r = 0; r < kDW; ++r) inv[r * kDW + lane] = y[r];
+    float* zo = Z + (64LL * t) * npad + 64LL * t + lane;
+#pragma unroll
+    for (int r = 0; r < kDW; ++r) zo[(long long)r * npad] = y[r];
     // the 32 x 32 diagonal blocks of L^-1 are the inverses of L's
     // (two guarded loops, so that y is indexed by constants only)
     const int cc = lane & 31;
@@ -1863,6 +1875,7 @@ struct BlkBufs {
   float* row;
   float* inv;
   int T;
+  float* Z;  // L11^-1 (npad x npad, the diagonal blocks written by the steps)
 };
 
 // Step k of the diagonal block (see above); k = -1: workgroup 0 factors tile
@@ -1886,7 +1899,7 @@ __global__ __launch_bounds__(256, 1) void k_blk_step(float* __restrict__ W, long
       blk_load(W + c0 * ld + c0, ld, v);
       blk_put(s0, v);
       __syncthreads();
-      blk_factor_tile(s0, W, ld, c0, 0, bb.inv, Y, npad, dg, linv, ipiv, growth);
+      blk_factor_tile(s0, W, ld, c0, 0, bb.inv, Y, bb.Z, npad, dg, linv, ipiv, growth);
       return;
     }
     // tile (i, 0) or (0, j): row segments of 16 floats, one float4 per thread
@@ -1985,7 +1998,8 @@ __global__ __launch_bounds__(256, 1) void k_blk_step(float* __restrict__ W, long
 #pragma unroll
         for (int r = 0; r < 4; ++r) s2[wm + 16 * i + 4 * lk + r][wn + 16 * j + lr] = cv[i][j][r];
     __syncthreads();
-    blk_factor_tile(s2, W, ld, c0, k + 1, bb.inv + (size_t)(par ^ 1) * 2 * TS, Y, npad, dg, linv, ipiv, growth);
+    blk_factor_tile(s2, W, ld, c0, k + 1, bb.inv + (size_t)(par ^ 1) * 2 * TS, Y, bb.Z, npad, dg, linv, ipiv,
+                    growth);
     return;
   }
   float* ncol = (tj == k + 1) ? bb.col + ((size_t)(par ^ 1) * T + ti) * TS : nullptr;
@@ -2003,33 +2017,54 @@ __global__ __launch_bounds__(256, 1) void k_blk_step(float* __restrict__ W, long
       }
 }
 
-// Identity in Y's diagonal blocks [t0, npad / 64) (the padding of U11^-1).
-__global__ __launch_bounds__(256) void k_binv_identity(int t0, int npad, float* __restrict__ Y) {
+// Identity in the diagonal blocks [t0, npad / 64) of Y and Z (the padding
+// of U11^-1 and L11^-1).
+__global__ __launch_bounds__(256) void k_binv_identity(int t0, int npad, float* __restrict__ Y, float* __restrict__ Z) {
   const long long b0 = 64LL * (t0 + blockIdx.x);
-  for (int i = threadIdx.x; i < kDW * kDW; i += 256) Y[(b0 + i / kDW) * npad + b0 + i % kDW] = (i / kDW == i % kDW);
+  for (int i = threadIdx.x; i < kDW * kDW; i += 256) {
+    const long long o = (b0 + i / kDW) * npad + b0 + i % kDW;
+    Y[o] = Z[o] = (i / kDW == i % kDW);
+  }
 }
 
-// Batched FP32 product C_p = alpha * A_p B_p (C not read), each operand of
-// batch p = blockIdx.z at base + p * stride; all of M, N, K multiples of TM.
-// A's entries at row ar0 + p * astep + r >= alim or column ac0 + p * astep +
-// k >= alim read as zero (the identity padding of U's off-diagonal blocks).
-// kTriA / kTriB: that operand is upper triangular, its entries below the
-// diagonal read as zero (they are not stored).  TM x TM tile per 256-thread
+// One batched FP32 product C_p = alpha * A_p B_p (C not read), each operand
+// of batch p at base + p * stride; M, N, K multiples of the tile.  A's
+// entries at row ar0 + p * astep + r >= alim or column ac0 + p * astep + k >=
+// alim read as zero (the identity padding of the factors' off-diagonal
+// blocks).  triA / triB: 1 = that operand is upper triangular, 2 = lower; its
+// entries on the other side of the diagonal read as zero (they are not
+// stored).
+struct BatOp {
+  const float* A;
+  long long lda, sa;
+  int ar0, ac0, astep, alim, triA;
+  const float* B;
+  long long ldb, sb;
+  int triB;
+  float* C;
+  long long ldc, sc;
+  float alpha;
+};
+
+__device__ __forceinline__ bool bat_keep(int tri, int r, int c) { return tri == 0 || (tri == 1 ? c >= r : c <= r); }
+
+// Two batched products of the same shape in one launch: blockIdx.z < P takes
+// o0's batch blockIdx.z, the others o1's.  TM x TM tile per 256-thread
 // workgroup, 2 x 2 waves of (TM/2)^2, MFMA f32 16x16x4, K staged through LDS
 // in steps of 16.
-template <int TM, bool kTriA, bool kTriB>
-__global__ __launch_bounds__(256) void k_gemm_bat(int K, const float* __restrict__ A, long long lda, long long sa,
-                                                  int ar0, int ac0, int astep, int alim,
-                                                  const float* __restrict__ B, long long ldb, long long sb,
-                                                  float* __restrict__ C, long long ldc, long long sc, float alpha) {
+template <int TM>
+__global__ __launch_bounds__(256) void k_gemm_bat(int K, int P, BatOp o0, BatOp o1) {
   constexpr int BK = 16, MI = TM / 32, PAD = 4;
   __shared__ float sA[BK][TM + PAD];
   __shared__ float sB[BK][TM + PAD];
-  const int p = blockIdx.z;
-  A += (long long)p * sa;
-  B += (long long)p * sb;
-  C += (long long)p * sc;
-  const int rlim = alim - (ar0 + p * astep), klim = alim - (ac0 + p * astep);
+  const bool second = (int)blockIdx.z >= P;
+  const BatOp& o = second ? o1 : o0;
+  const int p = second ? (int)blockIdx.z - P : (int)blockIdx.z;
+  const float* A = o.A + (long long)p * o.sa;
+  const float* B = o.B + (long long)p * o.sb;
+  float* C = o.C + (long long)p * o.sc;
+  const int rlim = o.alim - (o.ar0 + p * o.astep), klim = o.alim - (o.ac0 + p * o.astep);
+  const int triA = o.triA, triB = o.triB;
   const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TM;
   const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
   const int wm = (wave / 2) * (TM / 2), wn = (wave % 2) * (TM / 2);
@@ -2046,13 +2081,13 @@ __global__ __launch_bounds__(256) void k_gemm_bat(int K, const float* __restrict
     for (int i = 0; i < NA; ++i) {
       const int idx = tid + 256 * i;
       const int r = m0 + idx / BK, k = k0 + idx % BK;
-      va[i] = (r < rlim && k < klim && (!kTriA || k >= r)) ? A[(long long)r * lda + k] : 0.0f;
+      va[i] = (r < rlim && k < klim && bat_keep(triA, r, k)) ? A[(long long)r * o.lda + k] : 0.0f;
     }
 #pragma unroll
     for (int i = 0; i < NA; ++i) {
       const int idx = tid + 256 * i;
       const int k = k0 + idx / TM, n = n0 + idx % TM;
-      vb[i] = (!kTriB || n >= k) ? B[(long long)k * ldb + n] : 0.0f;
+      vb[i] = bat_keep(triB, k, n) ? B[(long long)k * o.ldb + n] : 0.0f;
     }
 #pragma unroll
     for (int i = 0; i < NA; ++i) {
@@ -2081,7 +2116,25 @@ __global__ __launch_bounds__(256) void k_gemm_bat(int K, const float* __restrict
     for (int j = 0; j < MI; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-        C[(long long)(m0 + wm + 16 * i + 4 * lk + r) * ldc + n0 + wn + 16 * j + lr] = alpha * acc[i][j][r];
+        C[(long long)(m0 + wm + 16 * i + 4 * lk + r) * o.ldc + n0 + wn + 16 * j + lr] = o.alpha * acc[i][j][r];
+}
+
+// A16[i][k] = -Z[i][k] for k < i < jb (Z = L11^-1, row pitch npad), zero
+// elsewhere up to Kp: the strictly lower part of L11^-1, negated, so that
+// C -= A16 B16^T on C = A12 leaves L11^-1 A12.  One thread per 8 halves.
+__global__ __launch_bounds__(256) void k_linv_a16(int jb, int npad, int Kp, const float* __restrict__ Z,
+                                                  _Float16* __restrict__ A16) {
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  const int per = Kp / 8;
+  if (idx >= (long long)jb * per) return;
+  const int i = (int)(idx / per), k0 = (int)(idx % per) * 8;
+  h8 v;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int k = k0 + q;
+    v[q] = (_Float16)((k < i && k < jb) ? -Z[(long long)i * npad + k] : 0.0f);
+  }
+  *(h8*)(A16 + (long long)i * Kp + k0) = v;
 }
 
 // Y[k][n] = dg[k] Y[k][n] for k < n < jb, 0 for n <= k (row pitch npad, a
@@ -2364,11 +2417,44 @@ static int gemm_sub(LuCtx<T>& c, long long M, long long Nn, int K, const T* A, c
   return QD_OK;
 }
 
+// X = L^-1 B for a diagonal-block panel's L11 (rows [r0, r0+m)) whose
+// inverse's strictly lower part, negated, is at li in FP16 ([m][Kp]): X = B +
+// (L11^-1 - I) B as one FP16 update of B in place (only the correction goes
+// through FP16).  The B operand copy goes to the solve's scratch, sized
+// before the factorisation.
+static int trsm_lower_inv(LuCtx<float>& c, long long r0, int m, long long c0, long long c1, const _Float16* li) {
+  qd_lu_state::F16Scratch* sc = c.sc_solve ? c.sc_solve : c.sc;
+  const long long n = c1 - c0, nb = (n + 63) / 64;
+  const int Kp = f16_kpad(m);
+  if (sc->h16b.count < (size_t)(n * Kp) || sc->colscale.count < (size_t)nb)
+    return set_error(c.h, QD_ERR_STATE, "U12 solve scratch sized for fewer than %lld x %d", n, Kp);
+  float* B = c.W + r0 * c.ld + c0;
+  const int vb = ((reinterpret_cast<uintptr_t>(B) & 15) == 0 && c.ld % 4 == 0) ? 1 : 0;
+  hipLaunchKernelGGL(k_scale_cvt_b16t, dim3((unsigned)nb), dim3(256), 0, c.st, m, n, Kp, (const float*)B, c.ld,
+                     sc->colscale.p, sc->h16b.p, vb);
+  QD_HIP(c.h, hipGetLastError());
+  const int rows16 = (c.ld % 4 == 0 && (reinterpret_cast<uintptr_t>(B) & 15) == 0) ? 1 : 0;
+  return launch_f16_update(c.h, false, m, n, Kp, li, sc->h16b.p, sc->colscale.p, B, c.ld, rows16, 0, c.st);
+}
+
+// QUADESTH_INV_U12=0 keeps the strip solves for U12 where a panel's L11^-1
+// is at hand.
+static bool inv_u12() {
+  static const bool on = [] {
+    const char* e = std::getenv("QUADESTH_INV_U12");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
 // X = L^-1 B, L unit lower at W[r0:r0+m, r0:r0+m], B = W[r0:r0+m, c0:c1]
 template <typename T>
 static int trsm_lower(LuCtx<T>& c, long long r0, int m, long long c0, long long c1) {
   if (c1 <= c0 || m <= 0) return QD_OK;
   if constexpr (std::is_same<T, float>::value) {
+    if (c.f16 && inv_u12())
+      for (int b = 0; b < 2; ++b)
+        if (c.s->linvh_col[b] == r0 && c.s->linvh_w[b] == m) return trsm_lower_inv(c, r0, m, c0, c1, c.s->linvh[b].p);
     // One strip solve when there are few columns (narrow strips: more
     // workgroups); for many columns the recursion below stops at 256 rows,
     // where wide strips re-read little of L, and the GEMMs between carry the
@@ -2511,7 +2597,8 @@ static int panel_block(LuCtx<float>& c, long long j, int jb) {
   qd_lu_state::F16Scratch& sc = s->sc_l21;
   if (jb % 64 != 0 || T < 2)
     return set_error(c.h, QD_ERR_STATE, "diagonal-block panel of %d columns (a multiple of 64 from 128 up)", jb);
-  if (s->binv.count < (size_t)npad * npad || s->binv_t.count < (size_t)npad * npad / 4 ||
+  if (s->binv.count < (size_t)npad * npad || s->binvl.count < (size_t)npad * npad ||
+      s->binv_t.count < (size_t)npad * npad / 2 ||
       s->bdiag.count < (size_t)2 * npad || sc.h16a.count < (size_t)(M * Kp) || sc.h16b.count < (size_t)jb * Kp ||
       sc.colscale.count < (size_t)nb || s->bstep.count < (size_t)(4 * T + 4) * 4096)
     return set_error(c.h, QD_ERR_STATE, "diagonal-block panel buffers sized for fewer than %d columns", jb);
@@ -2521,7 +2608,8 @@ static int panel_block(LuCtx<float>& c, long long j, int jb) {
   float* linv = s->ts_linv.p;
   // the diagonal block: tile 0 (with the copies of tile column and row 0),
   // then one launch per step
-  BlkBufs bb{s->bstep.p, s->bstep.p + (size_t)2 * T * 4096, s->bstep.p + (size_t)4 * T * 4096, T};
+  float* Z = s->binvl.p;
+  BlkBufs bb{s->bstep.p, s->bstep.p + (size_t)2 * T * 4096, s->bstep.p + (size_t)4 * T * 4096, T, Z};
   hipLaunchKernelGGL(k_blk_step, dim3((unsigned)(2 * T - 1)), dim3(256), 0, c.st, c.W, c.ld, j, -1, 0, bb, Y, npad, dg,
                      linv, s->ipiv_dev.p, s->growth.p);
   for (int k = 0; k + 1 < T; ++k) {
@@ -2531,27 +2619,42 @@ static int panel_block(LuCtx<float>& c, long long j, int jb) {
   }
   QD_HIP(c.h, hipGetLastError());
   if (npad > jb)
-    hipLaunchKernelGGL(k_binv_identity, dim3((unsigned)(npad / 64 - T)), dim3(256), 0, c.st, T, npad, Y);
-  const float* U = c.W + j * c.ld + j;
+    hipLaunchKernelGGL(k_binv_identity, dim3((unsigned)(npad / 64 - T)), dim3(256), 0, c.st, T, npad, Y, Z);
+  // U11^-1 and L11^-1 by recursive doubling, both in each launch.  Pair p
+  // covers rows and columns [2pb, 2pb + 2b): T_p = U12_p Y2_p, Y12_p =
+  // -Y1_p T_p (Y upper triangular), and T'_p = L21_p Z1_p, Z21_p = -Z2_p T'_p
+  // (Z lower triangular)
+  const float* Wd = c.W + j * c.ld + j;
   constexpr int kNoLim = 1 << 30;
+  float* Tl = Tm + (size_t)npad * npad / 4;
   for (int b = 64; b < npad; b *= 2) {
-    // pair p covers rows and columns [2pb, 2pb + 2b): T_p = U12_p Y2_p, then
-    // Y12_p = -Y1_p T_p (Y1, Y2 upper triangular)
-    const unsigned P = (unsigned)(npad / (2 * b));
+    const int P = npad / (2 * b);
     const long long sy = 2LL * b * (npad + 1), su = 2LL * b * (c.ld + 1), st = (long long)b * b;
-    const float* Y2 = Y + (long long)b * npad + b;
+    const long long bn = (long long)b * npad;
+    const BatOp u1{Wd + b, c.ld, su, 0, b, 2 * b, jb, 0, Y + bn + b, (long long)npad, sy, 1, Tm, (long long)b, st, 1.0f};
+    const BatOp l1{Wd + (long long)b * c.ld, c.ld, su, b, 0, 2 * b, jb, 0, Z, (long long)npad, sy, 2, Tl, (long long)b,
+                   st, 1.0f};
+    const BatOp u2{Y, (long long)npad, sy, 0, 0, 0, kNoLim, 1, Tm, (long long)b, st, 0, Y + b, (long long)npad, sy,
+                   -1.0f};
+    const BatOp l2{Z + bn + b, (long long)npad, sy, 0, 0, 0, kNoLim, 2, Tl, (long long)b, st, 0, Z + bn,
+                   (long long)npad, sy, -1.0f};
     if (b == 64) {
-      hipLaunchKernelGGL((k_gemm_bat<64, false, true>), dim3(1, 1, P), dim3(256), 0, c.st, b, U + b, c.ld, su, 0, b,
-                         2 * b, jb, Y2, (long long)npad, sy, Tm, (long long)b, st, 1.0f);
-      hipLaunchKernelGGL((k_gemm_bat<64, true, false>), dim3(1, 1, P), dim3(256), 0, c.st, b, (const float*)Y,
-                         (long long)npad, sy, 0, 0, 0, kNoLim, (const float*)Tm, (long long)b, st, Y + b,
-                         (long long)npad, sy, -1.0f);
+      hipLaunchKernelGGL(k_gemm_bat<64>, dim3(1, 1, 2 * P), dim3(256), 0, c.st, b, P, u1, l1);
+      hipLaunchKernelGGL(k_gemm_bat<64>, dim3(1, 1, 2 * P), dim3(256), 0, c.st, b, P, u2, l2);
     } else {
-      const dim3 g((unsigned)(b / 128), (unsigned)(b / 128), P);
-      hipLaunchKernelGGL((k_gemm_bat<128, false, true>), g, dim3(256), 0, c.st, b, U + b, c.ld, su, 0, b, 2 * b, jb,
-                         Y2, (long long)npad, sy, Tm, (long long)b, st, 1.0f);
-      hipLaunchKernelGGL((k_gemm_bat<128, true, false>), g, dim3(256), 0, c.st, b, (const float*)Y, (long long)npad,
-                         sy, 0, 0, 0, kNoLim, (const float*)Tm, (long long)b, st, Y + b, (long long)npad, sy, -1.0f);
+      const dim3 g((unsigned)(b / 128), (unsigned)(b / 128), (unsigned)(2 * P));
+      hipLaunchKernelGGL(k_gemm_bat<128>, g, dim3(256), 0, c.st, b, P, u1, l1);
+      hipLaunchKernelGGL(k_gemm_bat<128>, g, dim3(256), 0, c.st, b, P, u2, l2);
+    }
+  }
+  {
+    // L11^-1's strictly lower part, negated, in FP16 for the U12 solves
+    const int lb = (s->linvh_col[0] == -1 || (s->linvh_col[1] != -1 && s->linvh_col[0] < s->linvh_col[1])) ? 0 : 1;
+    if (s->linvh[lb].count >= (size_t)jb * Kp) {
+      hipLaunchKernelGGL(k_linv_a16, dim3((unsigned)(((long long)jb * (Kp / 8) + 255) / 256)), dim3(256), 0, c.st, jb,
+                         npad, Kp, (const float*)Z, s->linvh[lb].p);
+      s->linvh_col[lb] = j;
+      s->linvh_w[lb] = jb;
     }
   }
   hipLaunchKernelGGL(k_binv_rowscale, dim3((unsigned)(((long long)jb * (npad / 4) + 255) / 256)), dim3(256), 0, c.st,
@@ -2741,12 +2844,16 @@ static int factor(LuCtx<T>& c, const PanelWidths& pw) {
     QD_ALLOC(c.h, s->sc_panel.h16b, Kh * Kh);
     QD_ALLOC(c.h, s->sc_panel.colscale, (size_t)(NB / 64 + 2));
     QD_ALLOC(c.h, s->sc_solve.h16a, Kh * Kh);
-    QD_ALLOC(c.h, s->sc_solve.h16b, (size_t)N * Kh);
+    // (B operands of the U12 solves through L11^-1 take K = the panel width)
+    QD_ALLOC(c.h, s->sc_solve.h16b, (size_t)N * Kp);
     QD_ALLOC(c.h, s->sc_solve.colscale, (size_t)(N / 64 + 2));
     if (c.calu && c.diag && block_panels()) {
       const size_t np = (size_t)binv_pad(NB);
       QD_ALLOC(c.h, s->binv, np * np);
-      QD_ALLOC(c.h, s->binv_t, np * np / 4);
+      QD_ALLOC(c.h, s->binvl, np * np);
+      QD_ALLOC(c.h, s->binv_t, np * np / 2);
+      QD_ALLOC(c.h, s->linvh[0], (size_t)NB * Kp);
+      QD_ALLOC(c.h, s->linvh[1], (size_t)NB * Kp);
       QD_ALLOC(c.h, s->bdiag, 2 * np);
       QD_ALLOC(c.h, s->bstep, (size_t)(4 * (NB / 64) + 4) * 4096);
       QD_ALLOC(c.h, s->l21h[0], (size_t)N * Kp);
@@ -2757,7 +2864,7 @@ static int factor(LuCtx<T>& c, const PanelWidths& pw) {
     }
     if (look) {
       QD_ALLOC(c.h, s->sc_u12.h16a, Kh * Kh);
-      QD_ALLOC(c.h, s->sc_u12.h16b, (size_t)N * Kh);
+      QD_ALLOC(c.h, s->sc_u12.h16b, (size_t)N * Kp);
       QD_ALLOC(c.h, s->sc_u12.colscale, (size_t)(N / 64 + 2));
     }
   }
@@ -2794,7 +2901,8 @@ static int factor_steps(LuCtx<T>& c, const PanelWidths& pw, bool look) {
   ca.sc_solve = &s->sc_u12;
   bool u12_ready = false;  // this step's U12 beyond the next panel is solved (ev_u12)
   QD_HIP(c.h, hipMemsetAsync(s->piv_state.p, 0, 2 * sizeof(int), c.st));
-  s->l21h_col[0] = s->l21h_col[1] = -1;  // no FP16 L21 of this factorisation yet
+  s->l21h_col[0] = s->l21h_col[1] = -1;  // no FP16 L21 or L11^-1 of this factorisation yet
+  s->linvh_col[0] = s->linvh_col[1] = -1;
   int rc;
   const int jb0 = pw.at(0, N);
   if (c.src) {
