@@ -2051,10 +2051,10 @@ __device__ __forceinline__ bool bat_keep(int tri, int r, int c) { return tri == 
 // Two batched products of the same shape in one launch: blockIdx.z < P takes
 // o0's batch blockIdx.z, the others o1's.  TM x TM tile per 256-thread
 // workgroup, 2 x 2 waves of (TM/2)^2, MFMA f32 16x16x4, K staged through LDS
-// in steps of 16.
+// in steps of 32, the next step's loads in flight during this one's MFMAs.
 template <int TM>
 __global__ __launch_bounds__(256) void k_gemm_bat(int K, int P, BatOp o0, BatOp o1) {
-  constexpr int BK = 16, MI = TM / 32, PAD = 4;
+  constexpr int BK = 32, MI = TM / 32, PAD = 4;
   __shared__ float sA[BK][TM + PAD];
   __shared__ float sB[BK][TM + PAD];
   const bool second = (int)blockIdx.z >= P;
@@ -2075,8 +2075,10 @@ __global__ __launch_bounds__(256) void k_gemm_bat(int K, int P, BatOp o0, BatOp 
 #pragma unroll
     for (int j = 0; j < MI; ++j) acc[i][j] = (f4){0, 0, 0, 0};
   constexpr int NA = TM * BK / 256;  // elements per thread and operand
-  for (int k0 = 0; k0 < K; k0 += BK) {
-    float va[NA], vb[NA];
+  // the next K step's operands are loaded into registers while this one's
+  // MFMAs run
+  float va[NA], vb[NA];
+  auto load = [&](int k0) {
 #pragma unroll
     for (int i = 0; i < NA; ++i) {
       const int idx = tid + 256 * i;
@@ -2089,6 +2091,10 @@ __global__ __launch_bounds__(256) void k_gemm_bat(int K, int P, BatOp o0, BatOp 
       const int k = k0 + idx / TM, n = n0 + idx % TM;
       vb[i] = bat_keep(triB, k, n) ? B[(long long)k * o.ldb + n] : 0.0f;
     }
+  };
+  load(0);
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    __syncthreads();  // the previous step's fragment reads are done
 #pragma unroll
     for (int i = 0; i < NA; ++i) {
       const int idx = tid + 256 * i;
@@ -2096,6 +2102,7 @@ __global__ __launch_bounds__(256) void k_gemm_bat(int K, int P, BatOp o0, BatOp 
       sB[idx / TM][idx % TM] = vb[i];
     }
     __syncthreads();
+    if (k0 + BK < K) load(k0 + BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       float a[MI], b[MI];
@@ -2108,7 +2115,6 @@ __global__ __launch_bounds__(256) void k_gemm_bat(int K, int P, BatOp o0, BatOp 
 #pragma unroll
         for (int j = 0; j < MI; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
     }
-    __syncthreads();
   }
 #pragma unroll
   for (int i = 0; i < MI; ++i)
