@@ -1021,7 +1021,7 @@ __global__ __launch_bounds__(256) void k_scale_cvt_b16t(int K, long long N, int 
   const int cq = (tid % 16) * 4, rq = tid / 16;  // 16 rows per pass, 4 columns per thread
   float m = 0.0f;
   if (fullw) {
-#pragma unroll 4
+#pragma unroll 8
     for (int r = rq; r < K; r += 16) {
       const float4 v = *reinterpret_cast<const float4*>(B + (long long)r * ldb + c0 + cq);
       m = fmaxf(m, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
@@ -1040,41 +1040,53 @@ __global__ __launch_bounds__(256) void k_scale_cvt_b16t(int K, long long N, int 
   const float mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
   const float scale = mx > 0.0f ? mx : 1.0f;
   if (tid == 0) colscale[blockIdx.x] = scale;
-  // transpose: slice rows [k0, k0+32) into tile[k][n], then 64 rows of B16
+  // transpose: slice rows [k0, k0+32) into tile[k][n], then 64 rows of B16.
+  // The next slice's loads are issued before this slice's LDS round trip,
+  // whose barriers wait on LDS only (s_barrier after lgkmcnt(0)), so they
+  // stay in flight.
   const int wn = tid / 4, wk = (tid % 4) * 8;  // writer: column n, 8 k's
-  for (int k0 = 0; k0 < Kp; k0 += 32) {
+  float xa[2][4];
+  auto fetch = [&](int k0) {
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      const int kr = rq + 16 * h, k = k0 + kr;
-      float x0 = 0.0f, x1 = 0.0f, x2 = 0.0f, x3 = 0.0f;
+      const int k = k0 + rq + 16 * h;
+      xa[h][0] = xa[h][1] = xa[h][2] = xa[h][3] = 0.0f;
       if (k < K) {
         if (fullw) {
           const float4 v = *reinterpret_cast<const float4*>(B + (long long)k * ldb + c0 + cq);
-          x0 = v.x;
-          x1 = v.y;
-          x2 = v.z;
-          x3 = v.w;
+          xa[h][0] = v.x;
+          xa[h][1] = v.y;
+          xa[h][2] = v.z;
+          xa[h][3] = v.w;
         } else {
           const float* src = B + (long long)k * ldb + c0 + cq;
-          if (cq < cw) x0 = src[0];
-          if (cq + 1 < cw) x1 = src[1];
-          if (cq + 2 < cw) x2 = src[2];
-          if (cq + 3 < cw) x3 = src[3];
+#pragma unroll
+          for (int t = 0; t < 4; ++t)
+            if (cq + t < cw) xa[h][t] = src[t];
         }
       }
-      tile[kr][cq] = x0 / scale;
-      tile[kr][cq + 1] = x1 / scale;
-      tile[kr][cq + 2] = x2 / scale;
-      tile[kr][cq + 3] = x3 / scale;
     }
-    __syncthreads();
+  };
+  fetch(0);
+  for (int k0 = 0; k0 < Kp; k0 += 32) {
+    float xc[2][4];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) xc[h][t] = xa[h][t];
+    if (k0 + 32 < Kp) fetch(k0 + 32);
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) tile[rq + 16 * h][cq + t] = xc[h][t] / scale;
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
     if (wn < cw) {
       h8 o;
 #pragma unroll
       for (int q = 0; q < 8; ++q) o[q] = (_Float16)tile[wk + q][wn];
       *(h8*)(B16 + (c0 + wn) * Kp + k0 + wk) = o;
     }
-    __syncthreads();
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
   }
 }
 
@@ -2644,14 +2656,11 @@ static int panel_block(LuCtx<float>& c, long long j, int jb) {
                    -1.0f};
     const BatOp l2{Z + bn + b, (long long)npad, sy, 0, 0, 0, kNoLim, 2, Tl, (long long)b, st, 0, Z + bn,
                    (long long)npad, sy, -1.0f};
-    if (b == 64) {
-      hipLaunchKernelGGL(k_gemm_bat<64>, dim3(1, 1, 2 * P), dim3(256), 0, c.st, b, P, u1, l1);
-      hipLaunchKernelGGL(k_gemm_bat<64>, dim3(1, 1, 2 * P), dim3(256), 0, c.st, b, P, u2, l2);
-    } else {
-      const dim3 g((unsigned)(b / 128), (unsigned)(b / 128), (unsigned)(2 * P));
-      hipLaunchKernelGGL(k_gemm_bat<128>, g, dim3(256), 0, c.st, b, P, u1, l1);
-      hipLaunchKernelGGL(k_gemm_bat<128>, g, dim3(256), 0, c.st, b, P, u2, l2);
-    }
+    // 64 x 64 tiles at every level: 4x the workgroups of 128 x 128 ones,
+    // which the few large products of the last levels need
+    const dim3 g((unsigned)(b / 64), (unsigned)(b / 64), (unsigned)(2 * P));
+    hipLaunchKernelGGL(k_gemm_bat<64>, g, dim3(256), 0, c.st, b, P, u1, l1);
+    hipLaunchKernelGGL(k_gemm_bat<64>, g, dim3(256), 0, c.st, b, P, u2, l2);
   }
   {
     // L11^-1's strictly lower part, negated, in FP16 for the U12 solves
