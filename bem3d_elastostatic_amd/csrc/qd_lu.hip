@@ -2458,11 +2458,8 @@ static int trsm_lower_inv(LuCtx<float>& c, long long r0, int m, long long c0, lo
 // QUADESTH_INV_U12=0 keeps the strip solves for U12 where a panel's L11^-1
 // is at hand.
 static bool inv_u12() {
-  static const bool on = [] {
-    const char* e = std::getenv("QUADESTH_INV_U12");
-    return !(e && e[0] == '0');
-  }();
-  return on;
+  const char* e = std::getenv("QUADESTH_INV_U12");
+  return !(e && e[0] == '0');
 }
 
 // X = L^-1 B, L unit lower at W[r0:r0+m, r0:r0+m], B = W[r0:r0+m, c0:c1]
@@ -2594,11 +2591,8 @@ static int panel(LuCtx<T>& c, long long c0, int w, long long p0, long long p1) {
 // QUADESTH_BLOCK_PANEL=0 keeps diagonal-pivot panels as tall recursive
 // factorisations of all their rows.
 static bool block_panels() {
-  static const bool on = [] {
-    const char* e = std::getenv("QUADESTH_BLOCK_PANEL");
-    return !(e && e[0] == '0');
-  }();
-  return on;
+  const char* e = std::getenv("QUADESTH_BLOCK_PANEL");
+  return !(e && e[0] == '0');
 }
 
 // Diagonal-pivot panel [j, j+jb) with rows below it (jb a multiple of 64):

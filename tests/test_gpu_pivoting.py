@@ -58,14 +58,17 @@ def test_reversed_rows_fall_back_to_tournament(precision):
 
 
 @pytest.mark.parametrize("precision", ["fp16", "fp32"])
-@pytest.mark.parametrize("row", [10, 300])
+@pytest.mark.parametrize("row", [10, 300, 2000])
 def test_nan_multiplier_falls_back(precision, row):
-    """A NaN entry of A in column 5 gives a NaN multiplier in the first leaf:
-    row 10 inside the leaf's 64 x 64 top block (the one-wave elimination),
-    row 300 below it (the L21 rows).  The largest-multiplier word must rank
-    the NaN above every bound, so the factors are redone with tournament
-    pivoting (pivoting == 0); the refinement then reports QD_ERR_NOCONV
-    instead of a result."""
+    """A NaN entry of A in column 5 gives a NaN multiplier in the first
+    panel (1024 columns of N = 2598): row 10 inside its first 64 x 64 tile
+    (the one-wave elimination), row 300 further down its diagonal block (the
+    64-column steps' L tiles, or the recursive panel's L21 rows in FP32), row
+    2000 below the diagonal block (with FP16 updates: L21 = A21 U11^-1 by the
+    FP16 product, whose epilogue takes the growth).  The largest-multiplier
+    word must rank the NaN above every bound, so the factors are redone with
+    tournament pivoting (pivoting == 0); the refinement then reports
+    QD_ERR_NOCONV instead of a result."""
     import torch
     from bem3d_elastostatic_amd import _lib
     from bem3d_elastostatic_amd.quadesth import Quadesth

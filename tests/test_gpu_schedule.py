@@ -6,7 +6,9 @@ than the strip solve takes (2048) give that main-stream solve inner FP16
 updates, so the two solves need separate scratch: shared scratch corrupts the
 factors (the refinement then needs tens of sweeps instead of a handful).  The
 schedules below cover 1024 throughout, 2048 throughout, 2048 then 1024, 4096
-then 1024 (the default schedule's widths) and 768 throughout."""
+then 1024 (the default schedule's widths) and 768 throughout, and the default
+widths with the diagonal-block panels or the U12 solves through L11^-1 turned
+off."""
 import os
 
 import pytest
@@ -21,8 +23,15 @@ SCHEDULES = {
     "wide_then_narrow": {"QUADESTH_NB": "1024", "QUADESTH_NB_WIDE": "2048", "QUADESTH_NB_WIDE_REM": "8192"},
     "wider_then_narrow": {"QUADESTH_NB": "1024", "QUADESTH_NB_WIDE": "4096", "QUADESTH_NB_WIDE_REM": "8192"},
     # a first panel that is not whole 512-column slices: A's conversion into
-    # the factors' matrix rounds its first launch up to whole slices
+    # the factors' matrix rounds its first launch up to whole slices (and
+    # U11^-1 of a 768-column block is padded to 1024)
     "odd_width": {"QUADESTH_NB": "768", "QUADESTH_NB_WIDE": "0"},
+    # the recursive panels over all rows and the strip U12 solves, which the
+    # FP32 path and the fallback keep
+    "recursive_panels": {"QUADESTH_NB": "1024", "QUADESTH_NB_WIDE": "4096", "QUADESTH_NB_WIDE_REM": "8192",
+                         "QUADESTH_BLOCK_PANEL": "0"},
+    "strip_u12": {"QUADESTH_NB": "1024", "QUADESTH_NB_WIDE": "4096", "QUADESTH_NB_WIDE_REM": "8192",
+                  "QUADESTH_INV_U12": "0"},
 }
 
 
