@@ -1887,7 +1887,9 @@ struct BlkBufs {
   float* row;
   float* inv;
   int T;
-  float* Z;  // L11^-1 (npad x npad, the diagonal blocks written by the steps)
+  float* Z;   // L11^-1 (npad x npad, the diagonal blocks written by the steps)
+  float* lt;  // two-launch steps: L_ik and U_kj of step k ([T][4096] each)
+  float* ut;
 };
 
 // Step k of the diagonal block (see above); k = -1: workgroup 0 factors tile
@@ -2026,6 +2028,116 @@ __global__ __launch_bounds__(256, 1) void k_blk_step(float* __restrict__ W, long
         W[(rC + rr) * ld + cC + cc] = cv[i][j][r];
         if (ncol) ncol[rr * kDW + cc] = cv[i][j][r];
         if (nrow) nrow[rr * kDW + cc] = cv[i][j][r];
+      }
+}
+
+// Two-launch form of step k, for steps with many trailing tiles (the fused
+// step forms each L_ik and U_kj again in every workgroup of its tile row or
+// column: three products per trailing tile).  k_blk_solve2 forms tile column
+// and row k once (workgroups [0, nt): L_ik into W and bb.lt, with their
+// growth; [nt, 2 nt): U_kj into W and bb.ut); k_blk_update2 then subtracts
+// one product per trailing tile, and its workgroup 0 factors tile (k+1, k+1)
+// as the fused step's does.
+__global__ __launch_bounds__(256) void k_blk_solve2(float* __restrict__ W, long long ld, long long c0, int k, int nt,
+                                                    BlkBufs bb, unsigned* __restrict__ growth) {
+  __shared__ __attribute__((aligned(16))) float s0[kDW][68];
+  __shared__ __attribute__((aligned(16))) float s1[kDW][68];
+  constexpr int TS = kDW * kDW;
+  const int par = k & 1, T = bb.T;
+  const bool lcol = (int)blockIdx.x < nt;
+  const int q = lcol ? (int)blockIdx.x : (int)blockIdx.x - nt;  // tile k+1+q of the column / row
+  const float* inv = bb.inv + (size_t)par * 2 * TS;
+  float4 va[4], vb[4];
+  if (lcol) {
+    blk_load(bb.col + ((size_t)par * T + k + 1 + q) * TS, kDW, va);  // A_ik
+    blk_load(inv + TS, kDW, vb);                                     // U_kk^-1
+  } else {
+    blk_load(inv, kDW, va);                                          // L_kk^-1
+    blk_load(bb.row + ((size_t)par * T + k + 1 + q) * TS, kDW, vb);  // A_kj
+  }
+  blk_put_t(s0, va);
+  blk_put(s1, vb);
+  __syncthreads();
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  f4 acc[2][2];
+  blk_mm(s0, s1, wm, wn, lr, lk, acc);
+  const long long r0 = c0 + 64LL * (lcol ? k + 1 + q : k), cc0 = c0 + 64LL * (lcol ? k : k + 1 + q);
+  float* out = (lcol ? bb.lt : bb.ut) + (size_t)q * TS;
+  float g = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int rr = wm + 16 * i + 4 * lk + r, cc = wn + 16 * j + lr;
+        W[(r0 + rr) * ld + cc0 + cc] = acc[i][j][r];
+        out[rr * kDW + cc] = acc[i][j][r];
+        g = abs_max_keep_nan(g, acc[i][j][r]);
+      }
+  if (lcol) ts_growth_max(g, growth);
+}
+
+__global__ __launch_bounds__(256, 1) void k_blk_update2(float* __restrict__ W, long long ld, long long c0, int k,
+                                                        int nt, BlkBufs bb, float* __restrict__ Y, int npad,
+                                                        float* __restrict__ dg, float* __restrict__ linv,
+                                                        int* __restrict__ ipiv, unsigned* __restrict__ growth) {
+  __shared__ __attribute__((aligned(16))) float s0[kDW][68];
+  __shared__ __attribute__((aligned(16))) float s1[kDW][68];
+  __shared__ __attribute__((aligned(16))) float s2[kDW][68];
+  constexpr int TS = kDW * kDW;
+  const int par = k & 1, T = bb.T;
+  const int i = (int)blockIdx.x / nt, j = (int)blockIdx.x % nt;
+  const int ti = k + 1 + i, tj = k + 1 + j;
+  float4 va[4], vb[4];
+  blk_load(bb.lt + (size_t)i * TS, kDW, va);
+  blk_load(bb.ut + (size_t)j * TS, kDW, vb);
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const long long rC = c0 + 64LL * ti, cC = c0 + 64LL * tj;
+  float cv[2][2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) cv[a][b][r] = W[(rC + wm + 16 * a + 4 * lk + r) * ld + cC + wn + 16 * b + lr];
+  blk_put_t(s0, va);
+  blk_put(s1, vb);
+  __syncthreads();
+  f4 acc[2][2];
+  blk_mm(s0, s1, wm, wn, lr, lk, acc);
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) cv[a][b][r] -= acc[a][b][r];
+  if (blockIdx.x == 0) {
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) s2[wm + 16 * a + 4 * lk + r][wn + 16 * b + lr] = cv[a][b][r];
+    __syncthreads();
+    blk_factor_tile(s2, W, ld, c0, k + 1, bb.inv + (size_t)(par ^ 1) * 2 * TS, Y, bb.Z, npad, dg, linv, ipiv,
+                    growth);
+    return;
+  }
+  float* ncol = (tj == k + 1) ? bb.col + ((size_t)(par ^ 1) * T + ti) * TS : nullptr;
+  float* nrow = (ti == k + 1) ? bb.row + ((size_t)(par ^ 1) * T + tj) * TS : nullptr;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int rr = wm + 16 * a + 4 * lk + r, cc = wn + 16 * b + lr;
+        W[(rC + rr) * ld + cC + cc] = cv[a][b][r];
+        if (ncol) ncol[rr * kDW + cc] = cv[a][b][r];
+        if (nrow) nrow[rr * kDW + cc] = cv[a][b][r];
       }
 }
 
@@ -2595,6 +2707,19 @@ static bool block_panels() {
   return !(e && e[0] == '0');
 }
 
+// Steps of a diagonal block with at least this many trailing tiles per side
+// take the two-launch form (QUADESTH_BLK_SPLIT; 0: never).  The default 16
+// keeps the 1024-column panels' steps (at most 15) fused, where the panel's
+// chain of launches is the critical path, and splits the early steps of the
+// wide panels, where the chip is full and the fused form's extra products
+// cost the trailing update time.  Factor at 76,806 DOF, one box: never
+// 421.8 ms, from 48 / 32 / 16 / 8 / 4 / 1 tiles 416.1 / 413.8 / 410.9 / 413.4 /
+// 415.6 / 413.8 ms.
+static int blk_split_nt() {
+  const char* e = std::getenv("QUADESTH_BLK_SPLIT");
+  return e ? std::atoi(e) : 16;
+}
+
 // Diagonal-pivot panel [j, j+jb) with rows below it (jb a multiple of 64):
 // the jb x jb diagonal block factored in 64-column steps of one launch each,
 // U11^-1 (whose diagonal blocks the steps leave in Y) completed by recursive
@@ -2612,7 +2737,7 @@ static int panel_block(LuCtx<float>& c, long long j, int jb) {
   if (s->binv.count < (size_t)npad * npad || s->binvl.count < (size_t)npad * npad ||
       s->binv_t.count < (size_t)npad * npad / 2 ||
       s->bdiag.count < (size_t)2 * npad || sc.h16a.count < (size_t)(M * Kp) || sc.h16b.count < (size_t)jb * Kp ||
-      sc.colscale.count < (size_t)nb || s->bstep.count < (size_t)(4 * T + 4) * 4096)
+      sc.colscale.count < (size_t)nb || s->bstep.count < (size_t)(6 * T + 4) * 4096)
     return set_error(c.h, QD_ERR_STATE, "diagonal-block panel buffers sized for fewer than %d columns", jb);
   float* Y = s->binv.p;
   float* Tm = s->binv_t.p;
@@ -2621,13 +2746,22 @@ static int panel_block(LuCtx<float>& c, long long j, int jb) {
   // the diagonal block: tile 0 (with the copies of tile column and row 0),
   // then one launch per step
   float* Z = s->binvl.p;
-  BlkBufs bb{s->bstep.p, s->bstep.p + (size_t)2 * T * 4096, s->bstep.p + (size_t)4 * T * 4096, T, Z};
+  BlkBufs bb{s->bstep.p, s->bstep.p + (size_t)2 * T * 4096, s->bstep.p + (size_t)4 * T * 4096, T, Z,
+             s->bstep.p + (size_t)(4 * T + 4) * 4096, s->bstep.p + (size_t)(5 * T + 4) * 4096};
+  const int split_nt = blk_split_nt();
   hipLaunchKernelGGL(k_blk_step, dim3((unsigned)(2 * T - 1)), dim3(256), 0, c.st, c.W, c.ld, j, -1, 0, bb, Y, npad, dg,
                      linv, s->ipiv_dev.p, s->growth.p);
   for (int k = 0; k + 1 < T; ++k) {
     const int nt = T - 1 - k;
-    hipLaunchKernelGGL(k_blk_step, dim3((unsigned)(nt * nt)), dim3(256), 0, c.st, c.W, c.ld, j, k, nt, bb, Y, npad, dg,
-                       linv, s->ipiv_dev.p, s->growth.p);
+    if (split_nt > 0 && nt >= split_nt) {
+      hipLaunchKernelGGL(k_blk_solve2, dim3((unsigned)(2 * nt)), dim3(256), 0, c.st, c.W, c.ld, j, k, nt, bb,
+                         s->growth.p);
+      hipLaunchKernelGGL(k_blk_update2, dim3((unsigned)(nt * nt)), dim3(256), 0, c.st, c.W, c.ld, j, k, nt, bb, Y,
+                         npad, dg, linv, s->ipiv_dev.p, s->growth.p);
+    } else {
+      hipLaunchKernelGGL(k_blk_step, dim3((unsigned)(nt * nt)), dim3(256), 0, c.st, c.W, c.ld, j, k, nt, bb, Y, npad,
+                         dg, linv, s->ipiv_dev.p, s->growth.p);
+    }
   }
   QD_HIP(c.h, hipGetLastError());
   if (npad > jb)
@@ -2864,7 +2998,7 @@ static int factor(LuCtx<T>& c, const PanelWidths& pw) {
       QD_ALLOC(c.h, s->linvh[0], (size_t)NB * Kp);
       QD_ALLOC(c.h, s->linvh[1], (size_t)NB * Kp);
       QD_ALLOC(c.h, s->bdiag, 2 * np);
-      QD_ALLOC(c.h, s->bstep, (size_t)(4 * (NB / 64) + 4) * 4096);
+      QD_ALLOC(c.h, s->bstep, (size_t)(6 * (NB / 64) + 4) * 4096);
       QD_ALLOC(c.h, s->l21h[0], (size_t)N * Kp);
       QD_ALLOC(c.h, s->l21h[1], (size_t)N * Kp);
       QD_ALLOC(c.h, s->sc_l21.h16a, (size_t)N * Kp);

@@ -32,6 +32,9 @@ SCHEDULES = {
                          "QUADESTH_BLOCK_PANEL": "0"},
     "strip_u12": {"QUADESTH_NB": "1024", "QUADESTH_NB_WIDE": "4096", "QUADESTH_NB_WIDE_REM": "8192",
                   "QUADESTH_INV_U12": "0"},
+    # the two-launch steps of the diagonal blocks (from 8 trailing tiles on)
+    "split_steps": {"QUADESTH_NB": "1024", "QUADESTH_NB_WIDE": "4096", "QUADESTH_NB_WIDE_REM": "8192",
+                    "QUADESTH_BLK_SPLIT": "8"},
 }
 
 
