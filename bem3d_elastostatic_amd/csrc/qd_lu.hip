@@ -2216,8 +2216,16 @@ __global__ __launch_bounds__(256) void k_gemm_bat(int K, int P, BatOp o0, BatOp 
       vb[i] = bat_keep(triB, k, n) ? B[(long long)k * o.ldb + n] : 0.0f;
     }
   };
-  load(0);
-  for (int k0 = 0; k0 < K; k0 += BK) {
+  // a triangular operand's zero triangle is skipped: K runs only over the
+  // steps where the tile's rows of A / columns of B can be nonzero
+  int kb = 0, ke = K;
+  if (triA == 1) kb = max(kb, m0);       // A upper: A[r][k] = 0 for k < r
+  if (triA == 2) ke = min(ke, m0 + TM);  // A lower: A[r][k] = 0 for k > r
+  if (triB == 1) ke = min(ke, n0 + TM);  // B upper: B[k][n] = 0 for n < k
+  if (triB == 2) kb = max(kb, n0);       // B lower: B[k][n] = 0 for n > k
+  kb = kb / BK * BK;
+  if (kb < ke) load(kb);
+  for (int k0 = kb; k0 < ke; k0 += BK) {
     __syncthreads();  // the previous step's fragment reads are done
 #pragma unroll
     for (int i = 0; i < NA; ++i) {
@@ -2226,7 +2234,7 @@ __global__ __launch_bounds__(256) void k_gemm_bat(int K, int P, BatOp o0, BatOp 
       sB[idx / TM][idx % TM] = vb[i];
     }
     __syncthreads();
-    if (k0 + BK < K) load(k0 + BK);
+    if (k0 + BK < ke) load(k0 + BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       float a[MI], b[MI];
