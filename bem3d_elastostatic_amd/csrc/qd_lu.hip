@@ -1188,7 +1188,7 @@ __device__ __forceinline__ void out16_put4(const Out16& o, long long r, long lon
 }
 
 template <bool kOut16>
-__device__ __forceinline__ void gemm16_tile(long long m0, long long n0, long long M, long long N, int Kp,
+__device__ __forceinline__ void gemm16_tile(long long m0, long long n0, long long M, long long N, int Kp, int Kt,
                                             const _Float16* __restrict__ A16, const _Float16* __restrict__ B16,
                                             const float* __restrict__ colscale, float* __restrict__ C, long long ldc,
                                             bool rows16, G16Smem& sm, const Out16& o16, float& gmax) {
@@ -1200,7 +1200,7 @@ __device__ __forceinline__ void gemm16_tile(long long m0, long long n0, long lon
   for (int i = 0; i < kG16Mi; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = (f4){0, 0, 0, 0};
-  const int nk = Kp / H_BK;
+  const int nk = Kt / H_BK;  // K steps of this tile (Kp is the operands' row pitch)
   // fragment offsets within a stage (bytes): A row wm+16i+lr, B row wn+16j+lr, chunk lk
   int offa[kG16Mi], offb[4];
 #pragma unroll
@@ -1360,11 +1360,21 @@ __device__ __forceinline__ void gemm16_tile(long long m0, long long n0, long lon
 
 // rows16: C and ldc keep every row start 16-byte aligned.  kRole 2 also
 // writes the result as FP16 into o16 and its largest |c| into o16.growth.
+// ktri: 0 = the whole K range; 1 = B16[n][k] is zero for k >= n (a tile
+// runs K below its last column), 2 = A16[m][k] is zero for k >= m (K below
+// its last row).
+__device__ __forceinline__ int tile_kext(int Kp, int ktri, long long m0, long long n0, int bm, int bn) {
+  if (ktri == 1) return (int)min((long long)Kp, n0 + bn);
+  if (ktri == 2) return (int)min((long long)Kp, m0 + bm);
+  return Kp;
+}
+
 template <int kRole>
 __global__ __launch_bounds__(256, QD_G16_OCC) void k_gemm16(long long M, long long N, int Kp,
                                                             const _Float16* __restrict__ A16,
                                                 const _Float16* __restrict__ B16, const float* __restrict__ colscale,
-                                                float* __restrict__ C, long long ldc, int rows16, Out16 o16) {
+                                                float* __restrict__ C, long long ldc, int rows16, Out16 o16,
+                                                int ktri) {
   __shared__ __attribute__((aligned(16))) G16Smem sm;
   float gmax = 0.0f;
   const unsigned tiles_n = (unsigned)((N + H_BN - 1) / H_BN);
@@ -1383,7 +1393,8 @@ __global__ __launch_bounds__(256, QD_G16_OCC) void k_gemm16(long long M, long lo
     const unsigned gsz = min(tiles_m - first_m, (unsigned)kG16Group);
     const unsigned in_grp = vid - grp * kG16Group * tiles_n;
     const unsigned tm = first_m + in_grp % gsz, tn = in_grp / gsz;
-    gemm16_tile<kRole == 2>((long long)tm * H_BM, (long long)tn * H_BN, M, N, Kp, A16, B16, colscale, C, ldc,
+    const long long m0 = (long long)tm * H_BM, n0 = (long long)tn * H_BN;
+    gemm16_tile<kRole == 2>(m0, n0, M, N, Kp, tile_kext(Kp, ktri, m0, n0, H_BM, H_BN), A16, B16, colscale, C, ldc,
                             rows16 != 0, sm, o16, gmax);
   }
   if constexpr (kRole == 2) out16_growth(gmax, o16.growth);
@@ -1521,7 +1532,7 @@ __device__ __forceinline__ void hsteps(std::integer_sequence<int, Us...>, int hs
 }
 
 template <bool kOut16>
-__device__ __forceinline__ void tile(long long m0, long long n0, long long M, long long N, int Kp,
+__device__ __forceinline__ void tile(long long m0, long long n0, long long M, long long N, int Kp, int Kt,
                                      const _Float16* __restrict__ A16, const _Float16* __restrict__ B16,
                                      const float* __restrict__ colscale, float* __restrict__ C, long long ldc,
                                      bool rows16, Smem& sm, const Out16& o16, float& gmax) {
@@ -1558,7 +1569,7 @@ __device__ __forceinline__ void tile(long long m0, long long n0, long long M, lo
     for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
-  const int nk = Kp / BK;
+  const int nk = Kt / BK;  // K steps of this tile (Kp is the operands' row pitch)
   __syncthreads();  // every wave is done with the previous tile's stages
   ld.issue(sm.st[0], 0, wave);
   if (nk > 1) {
@@ -1673,7 +1684,7 @@ template <int kRole>
 __global__ __launch_bounds__(256, 1) void k_gemm16w(long long M, long long N, int Kp, const _Float16* __restrict__ A16,
                                                      const _Float16* __restrict__ B16,
                                                      const float* __restrict__ colscale, float* __restrict__ C,
-                                                     long long ldc, int rows16, Out16 o16) {
+                                                     long long ldc, int rows16, Out16 o16, int ktri) {
   __shared__ __attribute__((aligned(16))) w16::Smem sm;
   float gmax = 0.0f;
   const unsigned tiles_n = (unsigned)((N + w16::BN - 1) / w16::BN);
@@ -1689,8 +1700,9 @@ __global__ __launch_bounds__(256, 1) void k_gemm16w(long long M, long long N, in
     const unsigned gsz = min(tiles_m - first_m, (unsigned)kG16Group);
     const unsigned in_grp = vid - grp * kG16Group * tiles_n;
     const unsigned tm = first_m + in_grp % gsz, tn = in_grp / gsz;
-    w16::tile<kRole == 2>((long long)tm * w16::BM, (long long)tn * w16::BN, M, N, Kp, A16, B16, colscale, C, ldc,
-                          rows16 != 0, sm, o16, gmax);
+    const long long m0 = (long long)tm * w16::BM, n0 = (long long)tn * w16::BN;
+    w16::tile<kRole == 2>(m0, n0, M, N, Kp, tile_kext(Kp, ktri, m0, n0, w16::BM, w16::BN), A16, B16, colscale, C,
+                          ldc, rows16 != 0, sm, o16, gmax);
   }
   if constexpr (kRole == 2) out16_growth(gmax, o16.growth);
 }
@@ -2423,9 +2435,10 @@ static long long wide_gemm_min_tiles() {
 
 // o16.h non-null: the result also goes to o16 as FP16, its largest |c| to
 // o16.growth (kernel role 2).
+// ktri (tile_kext): a triangular operand's zero part is skipped per tile.
 static int launch_f16_update(qd_context* h, bool trailing, long long M, long long N, int Kp, const _Float16* A16,
                              const _Float16* B16, const float* colscale, float* C, long long ldc, int rows16,
-                             int max_wgs, hipStream_t st, Out16 o16 = Out16{nullptr, 0, nullptr}) {
+                             int max_wgs, hipStream_t st, Out16 o16 = Out16{nullptr, 0, nullptr}, int ktri = 0) {
   const long long wmin = wide_gemm_min_tiles();
   const long long wtiles = ((N + w16::BN - 1) / w16::BN) * ((M + w16::BM - 1) / w16::BM);
   // the wide kernel takes K steps of 64 and 32-bit element offsets into A16 / B16
@@ -2433,24 +2446,24 @@ static int launch_f16_update(qd_context* h, bool trailing, long long M, long lon
       std::max(M, N) * (long long)Kp < (1LL << 32)) {
     if (o16.h)
       hipLaunchKernelGGL(k_gemm16w<2>, dim3((unsigned)wtiles), dim3(256), 0, st, M, N, Kp, A16, B16, colscale, C, ldc,
-                         rows16, o16);
+                         rows16, o16, ktri);
     else if (trailing)
       hipLaunchKernelGGL(k_gemm16w<1>, dim3((unsigned)wtiles), dim3(256), 0, st, M, N, Kp, A16, B16, colscale, C, ldc,
-                         rows16, o16);
+                         rows16, o16, ktri);
     else
       hipLaunchKernelGGL(k_gemm16w<0>, dim3((unsigned)wtiles), dim3(256), 0, st, M, N, Kp, A16, B16, colscale, C, ldc,
-                         rows16, o16);
+                         rows16, o16, ktri);
     QD_HIP(h, hipGetLastError());
     return QD_OK;
   }
   const long long tiles = ((N + H_BN - 1) / H_BN) * ((M + H_BM - 1) / H_BM);
   const dim3 g((unsigned)(max_wgs > 0 ? std::min<long long>(tiles, max_wgs) : tiles));
   if (o16.h)
-    hipLaunchKernelGGL(k_gemm16<2>, g, dim3(256), 0, st, M, N, Kp, A16, B16, colscale, C, ldc, rows16, o16);
+    hipLaunchKernelGGL(k_gemm16<2>, g, dim3(256), 0, st, M, N, Kp, A16, B16, colscale, C, ldc, rows16, o16, ktri);
   else if (trailing)
-    hipLaunchKernelGGL(k_gemm16<1>, g, dim3(256), 0, st, M, N, Kp, A16, B16, colscale, C, ldc, rows16, o16);
+    hipLaunchKernelGGL(k_gemm16<1>, g, dim3(256), 0, st, M, N, Kp, A16, B16, colscale, C, ldc, rows16, o16, ktri);
   else
-    hipLaunchKernelGGL(k_gemm16<0>, g, dim3(256), 0, st, M, N, Kp, A16, B16, colscale, C, ldc, rows16, o16);
+    hipLaunchKernelGGL(k_gemm16<0>, g, dim3(256), 0, st, M, N, Kp, A16, B16, colscale, C, ldc, rows16, o16, ktri);
   QD_HIP(h, hipGetLastError());
   return QD_OK;
 }
@@ -2572,7 +2585,9 @@ static int trsm_lower_inv(LuCtx<float>& c, long long r0, int m, long long c0, lo
                      sc->colscale.p, sc->h16b.p, vb);
   QD_HIP(c.h, hipGetLastError());
   const int rows16 = (c.ld % 4 == 0 && (reinterpret_cast<uintptr_t>(B) & 15) == 0) ? 1 : 0;
-  return launch_f16_update(c.h, false, m, n, Kp, li, sc->h16b.p, sc->colscale.p, B, c.ld, rows16, 0, c.st);
+  // L11^-1 - I is strictly lower triangular: a tile's K runs below its last row
+  return launch_f16_update(c.h, false, m, n, Kp, li, sc->h16b.p, sc->colscale.p, B, c.ld, rows16, 0, c.st,
+                           Out16{nullptr, 0, nullptr}, 2);
 }
 
 // QUADESTH_INV_U12=0 keeps the strip solves for U12 where a panel's L11^-1
@@ -2828,8 +2843,9 @@ static int panel_block(LuCtx<float>& c, long long j, int jb) {
     s->l21h_col[b] = j;
     s->l21h_next = b ^ 1;
   }
+  // N is strictly upper triangular: a tile's K runs below its last column
   return launch_f16_update(c.h, false, M, jb, Kp, sc.h16a.p, sc.h16b.p, sc.colscale.p, A21, c.ld, rows16, 0, c.st,
-                           o16);
+                           o16, 1);
 }
 
 // The FP16 L21 of the panel at column j, if its product wrote one.
