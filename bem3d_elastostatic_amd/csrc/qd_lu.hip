@@ -1891,6 +1891,23 @@ __device__ __forceinline__ void blk_mm(const float (*sAt)[68], const float (*sB)
   }
 }
 
+// acc += sAt^T sB (as blk_mm, without clearing acc).
+__device__ __forceinline__ void blk_mm_acc(const float (*sAt)[68], const float (*sB)[68], int wm, int wn, int lr,
+                                           int lk, f4 (&acc)[2][2]) {
+#pragma unroll
+  for (int kk = 0; kk < kDW; kk += 4) {
+    float a[2], b[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) a[i] = sAt[kk + lk][wm + 16 * i + lr];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) b[j] = sB[kk + lk][wn + 16 * j + lr];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
+  }
+}
+
 // Step buffers of a diagonal-block factorisation (floats): [2][T][4096]
 // copies of tile column k, the same for tile row k, [2][2][4096] inverses of
 // tile k's factors (L, then U).
@@ -1903,6 +1920,78 @@ struct BlkBufs {
   float* lt;  // two-launch steps: L_ik and U_kj of step k ([T][4096] each)
   float* ut;
 };
+
+// Incremental inverses of a panel whose steps all take one launch: in step
+// k's launch, tile column k of U11^-1 (Y) and tile row k of L11^-1 (Z), from
+// tiles final in earlier launches:
+//   Y[r, k] = -(sum_{t=r}^{k-1} Y[r, t] U[t, k]) Y[k, k]      (r < k)
+//   Z[k, c] = -Z[k, k] (sum_{t=c}^{k-1} L[k, t] Z[t, c])      (c < k)
+// with U[t, k], L[k, t] in W (solved in launch t) and Y[k, k], Z[k, k] the
+// inverses of tile k's factors (its factorisation in launch k-1).  kY: Y's
+// row tile o, else Z's column tile o.  s0, s1 are free LDS.
+template <bool kY>
+__device__ __forceinline__ void blk_inc_inverse(const float* __restrict__ W, long long ld, long long c0, int k, int o,
+                                                float* __restrict__ Y, float* __restrict__ Z, int npad,
+                                                float (*s0)[68], float (*s1)[68]) {
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const long long tk = 64LL * k, oo = 64LL * o;
+  f4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = (f4){0, 0, 0, 0};
+#pragma unroll 1
+  for (int t = o; t < k; ++t) {
+    const long long tt = 64LL * t;
+    __syncthreads();  // the previous term's reads are done (before the loads:
+                      // loaded values kept across a barrier are spilled)
+    float4 va[4], vb[4];
+    if constexpr (kY) {
+      blk_load(Y + oo * npad + tt, npad, va);          // Y[r, t]
+      blk_load(W + (c0 + tt) * ld + c0 + tk, ld, vb);  // U[t, k]
+    } else {
+      blk_load(W + (c0 + tk) * ld + c0 + tt, ld, va);  // L[k, t]
+      blk_load(Z + tt * npad + oo, npad, vb);          // Z[t, c]
+    }
+    blk_put_t(s0, va);
+    blk_put(s1, vb);
+    __syncthreads();
+    blk_mm_acc(s0, s1, wm, wn, lr, lk, acc);
+  }
+  // the sum times the diagonal inverse: Y: S Y[k, k]; Z: Z[k, k] S
+  __syncthreads();
+  float4 vd[4];
+  blk_load((kY ? Y : Z) + tk * npad + tk, npad, vd);
+  if constexpr (kY) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) s0[wn + 16 * j + lr][wm + 16 * i + 4 * lk + r] = acc[i][j][r];  // S^T
+    blk_put(s1, vd);
+  } else {
+    blk_put_t(s0, vd);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) s1[wm + 16 * i + 4 * lk + r][wn + 16 * j + lr] = acc[i][j][r];  // S
+  }
+  __syncthreads();
+  f4 res[2][2];
+  blk_mm(s0, s1, wm, wn, lr, lk, res);
+  float* out = kY ? Y + oo * npad + tk : Z + tk * npad + oo;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        out[(long long)(wm + 16 * i + 4 * lk + r) * npad + wn + 16 * j + lr] = -res[i][j][r];
+}
 
 // Step k of the diagonal block (see above); k = -1: workgroup 0 factors tile
 // 0, workgroups 1 .. T-1 copy tile column 0 and T .. 2T-2 tile row 0 into the
@@ -1939,6 +2028,13 @@ __global__ __launch_bounds__(256, 1) void k_blk_step(float* __restrict__ W, long
       const int idx = tid + 256 * p, r = idx / 16, c4 = (idx % 16) * 4;
       dst[idx] = *reinterpret_cast<const float4*>(src + r * ld + c4);
     }
+    return;
+  }
+  if ((int)blockIdx.x >= nt * nt) {
+    // incremental inverses (panels whose steps all take one launch)
+    const int q = (int)blockIdx.x - nt * nt;
+    if (q < k) blk_inc_inverse<true>(W, ld, c0, k, q, Y, bb.Z, npad, s0, s1);
+    else blk_inc_inverse<false>(W, ld, c0, k, q - k, Y, bb.Z, npad, s0, s1);
     return;
   }
   const int par = k & 1, T = bb.T;
@@ -2738,6 +2834,13 @@ static bool block_panels() {
 // cost the trailing update time.  Factor at 76,806 DOF, one box: never
 // 421.8 ms, from 48 / 32 / 16 / 8 / 4 / 1 tiles 416.1 / 413.8 / 410.9 / 413.4 /
 // 415.6 / 413.8 ms.
+// QUADESTH_INC_INVERSE=0 forms U11^-1 and L11^-1 by recursive doubling in
+// every panel.
+static bool inc_inverses() {
+  const char* e = std::getenv("QUADESTH_INC_INVERSE");
+  return !(e && e[0] == '0');
+}
+
 static int blk_split_nt() {
   const char* e = std::getenv("QUADESTH_BLK_SPLIT");
   return e ? std::atoi(e) : 16;
@@ -2774,6 +2877,10 @@ static int panel_block(LuCtx<float>& c, long long j, int jb) {
   const int split_nt = blk_split_nt();
   hipLaunchKernelGGL(k_blk_step, dim3((unsigned)(2 * T - 1)), dim3(256), 0, c.st, c.W, c.ld, j, -1, 0, bb, Y, npad, dg,
                      linv, s->ipiv_dev.p, s->growth.p);
+  // a panel whose steps all take one launch forms U11^-1 and L11^-1 in them
+  // (tile column and row k in launch k, the last ones in a launch of their
+  // own) instead of by recursive doubling afterwards
+  const bool inc = inc_inverses() && !(split_nt > 0 && T - 1 >= split_nt);
   for (int k = 0; k + 1 < T; ++k) {
     const int nt = T - 1 - k;
     if (split_nt > 0 && nt >= split_nt) {
@@ -2782,12 +2889,15 @@ static int panel_block(LuCtx<float>& c, long long j, int jb) {
       hipLaunchKernelGGL(k_blk_update2, dim3((unsigned)(nt * nt)), dim3(256), 0, c.st, c.W, c.ld, j, k, nt, bb, Y,
                          npad, dg, linv, s->ipiv_dev.p, s->growth.p);
     } else {
-      hipLaunchKernelGGL(k_blk_step, dim3((unsigned)(nt * nt)), dim3(256), 0, c.st, c.W, c.ld, j, k, nt, bb, Y, npad,
-                         dg, linv, s->ipiv_dev.p, s->growth.p);
+      hipLaunchKernelGGL(k_blk_step, dim3((unsigned)(nt * nt + (inc ? 2 * k : 0))), dim3(256), 0, c.st, c.W, c.ld, j,
+                         k, nt, bb, Y, npad, dg, linv, s->ipiv_dev.p, s->growth.p);
     }
   }
+  if (inc)
+    hipLaunchKernelGGL(k_blk_step, dim3((unsigned)(2 * (T - 1))), dim3(256), 0, c.st, c.W, c.ld, j, T - 1, 0, bb, Y,
+                       npad, dg, linv, s->ipiv_dev.p, s->growth.p);
   QD_HIP(c.h, hipGetLastError());
-  if (npad > jb)
+  if (npad > jb && !inc)
     hipLaunchKernelGGL(k_binv_identity, dim3((unsigned)(npad / 64 - T)), dim3(256), 0, c.st, T, npad, Y, Z);
   // U11^-1 and L11^-1 by recursive doubling, both in each launch.  Pair p
   // covers rows and columns [2pb, 2pb + 2b): T_p = U12_p Y2_p, Y12_p =
@@ -2796,7 +2906,7 @@ static int panel_block(LuCtx<float>& c, long long j, int jb) {
   const float* Wd = c.W + j * c.ld + j;
   constexpr int kNoLim = 1 << 30;
   float* Tl = Tm + (size_t)npad * npad / 4;
-  for (int b = 64; b < npad; b *= 2) {
+  for (int b = 64; b < npad && !inc; b *= 2) {
     const int P = npad / (2 * b);
     const long long sy = 2LL * b * (npad + 1), su = 2LL * b * (c.ld + 1), st = (long long)b * b;
     const long long bn = (long long)b * npad;
