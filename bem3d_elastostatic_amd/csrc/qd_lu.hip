@@ -1458,25 +1458,6 @@ __device__ __forceinline__ void read_frags(const unsigned char* st, const int (&
   }
 }
 
-__device__ __forceinline__ void mfmas(const Frag& f, f16v (&acc)[4][4]) {
-#pragma unroll
-  for (int s = 0; s < 2; ++s)
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(f.a[s][i], f.b[s][j], acc[i][j], 0, 0, 0);
-}
-
-// 16 fragment reads among the 32 MFMAs of a half step
-__device__ __forceinline__ void interleave16() {
-#pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-    __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
-  }
-}
-
 // Lane l of instruction q covers row (q*4 + wave)*8 + l/8 of A (and of B),
 // 16-byte slot l%8; its offsets are 32-bit element offsets from A16 / B16
 // (rows past M or N read row M-1 or N-1: they feed only outputs not stored).
@@ -1494,26 +1475,63 @@ struct Loader {
     for (int q = 0; q < 8; ++q)
       __builtin_amdgcn_global_load_lds(b + ob[q], (lds_ptr_t)(st + ((q + 8) * 4 + wave) * 1024), 16, 0, 0);
   }
+  // load Q (0..15) of issue() alone
+  template <int Q>
+  __device__ __forceinline__ void issue_one(unsigned char* st, int kb, int wave) const {
+    if constexpr (Q < 8)
+      __builtin_amdgcn_global_load_lds(A16 + kb * BK + oa[Q], (lds_ptr_t)(st + (Q * 4 + wave) * 1024), 16, 0, 0);
+    else
+      __builtin_amdgcn_global_load_lds(B16 + kb * BK + ob[Q - 8], (lds_ptr_t)(st + (Q * 4 + wave) * 1024), 16, 0, 0);
+  }
 };
+
+template <int... Ts, class F>
+__device__ __forceinline__ void seq_each(std::integer_sequence<int, Ts...>, F&& f) {
+  (f(std::integral_constant<int, Ts>{}), ...);
+}
+
+// fragment U (0..15) of half H of a stage, in the order the MFMAs use them:
+// substep U/8, A (U/4 even) or B, 32-row block U%4
+template <int H, int U>
+__device__ __forceinline__ void read_frag(const unsigned char* st, const int (&offa)[4], const int (&offb)[4],
+                                          Frag& f) {
+  constexpr int s = U / 8, i = U % 4;
+  if constexpr ((U / 4) % 2 == 0) f.a[s][i] = *reinterpret_cast<const h8*>(st + offa[2 * H + s] + i * 32 * 128);
+  else f.b[s][i] = *reinterpret_cast<const h8*>(st + offb[2 * H + s] + i * 32 * 128);
+}
 
 // Half step (stage k, half H) on the stage in slot SL.  At H = 1 stage k+1
 // must have landed (its first half is read from now on) and stage k's slot,
 // whose fragments are all in registers, takes stage k+2 (kLoad 1: it exists,
-// 2: only if it does).
+// 2: only if it does).  The instructions go in a fixed order, one MFMA per
+// group: the next half's 16 fragment reads beside the first 16 MFMAs (they
+// have landed long before the next half needs them), and at H = 1 the 16
+// operand loads one per two MFMAs.  Issued together after the barrier, the 64
+// loads of a CU's four waves queue at the address unit and hold each wave's
+// in-order issue, MFMAs included (trailing-update replay 944 -> 996 TFLOP/s).
 template <int SL, int H, int kLoad>
 __device__ __forceinline__ void hstep(int k, int nk, Smem& sm, const Loader& ld, int wave, const int (&offa)[4],
                                       const int (&offb)[4], Frag& cur, Frag& nxt, f16v (&acc)[4][4]) {
+  bool load = false;
   if constexpr (H == 1) {
     wait_vm<0>();  // stage k+1's loads were the only ones in flight
     wait_lgkm0();
     asm volatile("s_barrier" ::: "memory");
-    if (kLoad == 1 || k + 2 < nk) ld.issue(sm.st[SL], k + 2, wave);
-    read_frags<0>(sm.st[SL ^ 1], offa, offb, nxt);  // past the last stage: stale, never used
-  } else {
-    read_frags<1>(sm.st[SL], offa, offb, nxt);
+    load = kLoad == 1 || k + 2 < nk;
   }
-  mfmas(cur, acc);
-  interleave16();
+  // past the last stage the reads are stale and never used
+  const unsigned char* rs = H == 1 ? sm.st[SL ^ 1] : sm.st[SL];
+  auto one = [&](auto tc) {
+    constexpr int t = decltype(tc)::value;
+    constexpr int s = t / 16, i = (t / 4) % 4, j = t % 4;
+    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(cur.a[s][i], cur.b[s][j], acc[i][j], 0, 0, 0);
+    if constexpr (t < 16) read_frag<H == 1 ? 0 : 1, t>(rs, offa, offb, nxt);
+    if constexpr (H == 1 && t % 2 == 1) {
+      if (kLoad == 1 || load) ld.issue_one<t / 2>(sm.st[SL], k + 2, wave);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  seq_each(std::make_integer_sequence<int, 32>{}, one);
 }
 
 template <int kLoad, int... Us>
