@@ -1500,34 +1500,47 @@ __device__ __forceinline__ void read_frag(const unsigned char* st, const int (&o
   else f.b[s][i] = *reinterpret_cast<const h8*>(st + offb[2 * H + s] + i * 32 * 128);
 }
 
-// Half step (stage k, half H) on the stage in slot SL.  At H = 1 stage k+1
-// must have landed (its first half is read from now on) and stage k's slot,
-// whose fragments are all in registers, takes stage k+2 (kLoad 1: it exists,
-// 2: only if it does).  The instructions go in a fixed order, one MFMA per
-// group: the next half's 16 fragment reads beside the first 16 MFMAs (they
-// have landed long before the next half needs them), and at H = 1 the 16
-// operand loads one per two MFMAs.  Issued together after the barrier, the 64
-// loads of a CU's four waves queue at the address unit and hold each wave's
-// in-order issue, MFMAs included (trailing-update replay 944 -> 996 TFLOP/s).
+// Half step (stage k, half H) on the stage in slot SL.  Stage k's slot takes
+// stage k+2 (kLoad 1: it exists, 2: only if it does) once every wave has read
+// its second half, from the middle of H = 0 on; at H = 1 stage k+1 must have
+// landed (its first half is read from now on).  The instructions go in a
+// fixed order, one MFMA per group: the next half's 16 fragment reads beside
+// the first 16 MFMAs (they have landed long before the next half needs them),
+// and the 16 operand loads one per two MFMAs over the last 16 MFMAs of H = 0
+// and the first 16 of H = 1.  Issued together after a barrier, the 64 loads
+// of a CU's four waves queue at the address unit and hold each wave's
+// in-order issue, MFMAs included (trailing-update replay 944 -> 1032 TFLOP/s
+// with both changes).
 template <int SL, int H, int kLoad>
 __device__ __forceinline__ void hstep(int k, int nk, Smem& sm, const Loader& ld, int wave, const int (&offa)[4],
                                       const int (&offb)[4], Frag& cur, Frag& nxt, f16v (&acc)[4][4]) {
-  bool load = false;
+  const bool load = kLoad == 1 || k + 2 < nk;
   if constexpr (H == 1) {
-    wait_vm<0>();  // stage k+1's loads were the only ones in flight
-    wait_lgkm0();
+    // stage k+1 has landed once at most the 8 loads of stage k+2 issued in
+    // the first half are in flight
+    if (kLoad == 1 || load) wait_vm<8>();
+    else wait_vm<0>();
     asm volatile("s_barrier" ::: "memory");
-    load = kLoad == 1 || k + 2 < nk;
   }
   // past the last stage the reads are stale and never used
   const unsigned char* rs = H == 1 ? sm.st[SL ^ 1] : sm.st[SL];
   auto one = [&](auto tc) {
     constexpr int t = decltype(tc)::value;
     constexpr int s = t / 16, i = (t / 4) % 4, j = t % 4;
+    if constexpr (H == 0 && t == 17) {
+      if (kLoad == 1 || load) {
+        // every wave has read stage k (slot SL): it takes stage k+2
+        wait_lgkm0();
+        asm volatile("s_barrier" ::: "memory");
+      }
+    }
     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(cur.a[s][i], cur.b[s][j], acc[i][j], 0, 0, 0);
     if constexpr (t < 16) read_frag<H == 1 ? 0 : 1, t>(rs, offa, offb, nxt);
-    if constexpr (H == 1 && t % 2 == 1) {
-      if (kLoad == 1 || load) ld.issue_one<t / 2>(sm.st[SL], k + 2, wave);
+    if constexpr (H == 0 && t >= 17 && t % 2 == 1) {
+      if (kLoad == 1 || load) ld.issue_one<(t - 17) / 2>(sm.st[SL], k + 2, wave);
+    }
+    if constexpr (H == 1 && t < 16 && t % 2 == 1) {
+      if (kLoad == 1 || load) ld.issue_one<8 + t / 2>(sm.st[SL], k + 2, wave);
     }
     __builtin_amdgcn_sched_barrier(0);
   };
